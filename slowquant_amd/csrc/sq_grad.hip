@@ -1,0 +1,141 @@
+// Energy + theta gradient drivers (replaces ups_wavefunction.py:1019-1142
+// for the theta block) and workspace sizing.
+//
+//   adjoint          forward sweep psi = U psi0, lambda = H psi, reverse sweep
+//                    g_i = 2 <lambda|T_i psi>, un-applying gate i on psi and
+//                    lambda in place (no stored intermediates).
+//   plain            per-parameter derivative propagation from scratch,
+//                    d_i psi = U_N..U_{i+1} U_i T_i U_{i-1}..U_1 psi0 (O(N^2)).
+//   reference sweep  the reference's product-rule loop (ups_wavefunction.py:1090-1141).
+
+#include <vector>
+
+#include "sq_internal.h"
+
+namespace sq {
+
+// Gradient workspace (doubles): psi | lam | t1 | t2 | partials | scalars(64+N) | sigma work
+static int64_t grad_doubles(const Space& s, int n_gates) {
+  return 4 * s.dim + nblocks2d(s) + 1024 + 64 + n_gates + sigma_work_bytes(s) / 8;
+}
+
+struct GradBufs {
+  double *psi, *lam, *t1, *t2, *partial, *scal, *grad, *sigw;
+};
+
+static GradBufs carve(const Space& s, int n_gates, double* work) {
+  GradBufs b;
+  b.psi = work;
+  b.lam = b.psi + s.dim;
+  b.t1 = b.lam + s.dim;
+  b.t2 = b.t1 + s.dim;
+  b.partial = b.t2 + s.dim;
+  b.scal = b.partial + nblocks2d(s) + 1024;
+  b.grad = b.scal + 64;
+  b.sigw = b.grad + n_gates;
+  return b;
+}
+
+static void finish(const GradBufs& b, int n, double* energy, double* grad, hipStream_t st) {
+  if (n > 0) SQ_HIP(hipMemcpyAsync(grad, b.grad, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipMemcpyAsync(energy, b.scal, sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+}
+
+static void forward(const Space& s, const Program& p, const double* theta, const double* psi0, double* psi,
+                    double* t1, hipStream_t st) {
+  copy_vec(psi0, psi, s.dim, st);
+  for (int g = 0; g < (int)p.gates.size(); ++g) apply_gate_impl(s, p, g, theta, false, psi, t1, st);
+}
+
+void energy_grad_adjoint(const Space& s, const Program& p, const Hamiltonian& H, const double* theta,
+                         const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
+  const int n = (int)p.gates.size();
+  GradBufs b = carve(s, n, work);
+  forward(s, p, theta, psi0, b.psi, b.t1, st);
+  sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
+  for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, b.psi, b.lam, b.grad, b.t1, b.partial, st);
+  finish(b, n, energy, grad, st);
+}
+
+void energy_grad_plain(const Space& s, const Program& p, const Hamiltonian& H, const double* theta,
+                       const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
+  const int n = (int)p.gates.size();
+  GradBufs b = carve(s, n, work);
+  forward(s, p, theta, psi0, b.psi, b.t1, st);
+  sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
+  // b.psi is free now; use it as the derivative vector, t2 for T_i products.
+  double* v = b.psi;
+  for (int i = 0; i < n; ++i) {
+    copy_vec(psi0, v, s.dim, st);
+    for (int k = 0; k < n; ++k) {
+      apply_gate_impl(s, p, k, theta, false, v, b.t1, st);
+      if (k == i) {
+        grad_action_impl(s, p, k, v, b.t2, st);
+        copy_vec(b.t2, v, s.dim, st);
+      }
+    }
+    dot_dev(b.lam, v, s.dim, b.grad, i, 2.0, b.partial, st);
+  }
+  finish(b, n, energy, grad, st);
+}
+
+void energy_grad_reference_sweep(const Space& s, const Program& p, const Hamiltonian& H, const double* theta,
+                                 const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
+  const int n = (int)p.gates.size();
+  GradBufs b = carve(s, n, work);
+  forward(s, p, theta, psi0, b.psi, b.t1, st);
+  sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
+  // bra = U^+ H psi
+  for (int g = n - 1; g >= 0; --g) apply_gate_impl(s, p, g, theta, true, b.lam, b.t1, st);
+  double* ket = b.psi;
+  copy_vec(psi0, ket, s.dim, st);
+  for (int i = 0; i < n; ++i) {
+    int64_t nb = tdot_impl(s, p, i, ket, b.lam, b.partial, st);
+    reduce_impl(b.partial, nb, b.grad, i, false, 2.0, st);
+    apply_gate_impl(s, p, i, theta, false, b.lam, b.t1, st);
+    apply_gate_impl(s, p, i, theta, false, ket, b.t1, st);
+  }
+  finish(b, n, energy, grad, st);
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+extern "C" {
+
+int64_t sq_work_bytes(const sq_space* sp, int what) {
+  if (!sp) return -1;
+  const Space& s = sp->s;
+  switch (what) {
+    case 0:  // state preparation / single gates
+      return 8 * (2 * s.dim + 64);
+    case 1:  // sigma
+      return sigma_work_bytes(s);
+    case 2:  // gradients, for programs of up to 65536 gates
+      return 8 * grad_doubles(s, 65536);
+    case 3:  // RDMs
+      return rdm_work_bytes(s, nullptr, nullptr, nullptr);
+    default:
+      return -1;
+  }
+}
+
+#define SQ_GRAD_ENTRY(name, impl)                                                                               \
+  int name(const sq_space* sp, const sq_program* pr, const sq_hamiltonian* H, const double* theta,            \
+           const double* psi0, double* energy, double* grad, double* work, void* stream) {                    \
+    return guarded([&] {                                                                                        \
+      SQ_CHECK(sp && pr && H && psi0 && energy && work, SQ_ERR_ARG, "NULL argument");                          \
+      SQ_CHECK(theta || pr->p.gates.empty(), SQ_ERR_ARG, "theta is NULL");                                      \
+      SQ_CHECK(grad || pr->p.gates.empty(), SQ_ERR_ARG, "grad is NULL");                                        \
+      SQ_CHECK((int)pr->p.gates.size() <= 65536, SQ_ERR_ARG, "too many gates");                                 \
+      impl(sp->s, pr->p, H->h, theta, psi0, energy, grad, work, as_stream(stream));                             \
+    });                                                                                                         \
+  }
+
+SQ_GRAD_ENTRY(sq_energy_grad_adjoint, energy_grad_adjoint)
+SQ_GRAD_ENTRY(sq_energy_grad_plain, energy_grad_plain)
+SQ_GRAD_ENTRY(sq_energy_grad_reference_sweep, energy_grad_reference_sweep)
+
+}  // extern "C"

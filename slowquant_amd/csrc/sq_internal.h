@@ -1,0 +1,190 @@
+// Internal definitions shared by the engine's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/sqhip.h"
+
+namespace sq {
+
+void set_error(const std::string& msg);
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+#define SQ_HIP(call)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (call);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      throw ::sq::Error{SQ_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)};      \
+    }                                                                                       \
+  } while (0)
+
+#define SQ_CHECK(cond, code, msg)                    \
+  do {                                               \
+    if (!(cond)) throw ::sq::Error{(code), (msg)};   \
+  } while (0)
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return SQ_OK;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return SQ_ERR_ARG;
+  }
+}
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Maximum active orbitals: rank tables are direct-indexed by the 2^n bit mask.
+constexpr int kMaxOrb = 16;
+
+// One spin-resolved raw operator product: ops applied right to left
+// (ops[n-1] first).  orb = spatial orbital, cre = creation flag.
+struct SpinOps {
+  int32_t n;
+  int8_t orb[12];
+  int8_t cre[12];
+};
+
+// A raw product term of a generator in the blocked convention:
+// coef * (alpha ops) (beta ops).
+struct Term {
+  double coef;
+  SpinOps a;
+  SpinOps b;
+};
+
+// Device-side gate record.
+struct GateDev {
+  int32_t kind;          // internal class, see GateClass
+  int32_t term_begin;    // generator terms [term_begin, term_end)
+  int32_t term_end;
+  int32_t poly;          // for POLY gates: index of the frequency table (2..5)
+  double scale;          // angle multiplier for rotation gates
+  double pairs[2];       // rotation pairs touched by each rotation half (for byte counts)
+};
+
+enum GateClass : int32_t {
+  GC_ROT = 1,        // single string G with distinct indices: exp(th (G - G^+)) = pair rotation
+  GC_SA_SINGLE = 2,  // two commuting rotations (alpha, beta) with the same angle
+  GC_POLY = 3,       // multi-frequency polynomial in T (sa_double_2..5)
+  GC_TAYLOR = 4      // generic: closed form 1 + sin T + (1 - cos) T^2 through T applications
+};
+
+struct Space {
+  int n, na, nb;
+  int nsa, nsb;
+  int64_t dim;
+  int npair;              // n(n+1)/2 packed (p>=q) pairs
+  int device;
+  uint32_t* d_astr = nullptr;  // [nsa]
+  uint32_t* d_bstr = nullptr;  // [nsb]
+  int32_t* d_arank = nullptr;  // [1<<n]
+  int32_t* d_brank = nullptr;
+  int32_t* d_aexc = nullptr;   // [npair][nsa]: sign*(target+1) for E+_k, p>q; occupation for p==q
+  int32_t* d_bexc = nullptr;   // [npair][nsb]
+  int32_t* d_aexc_full = nullptr;  // [n*n][nsa]: sign*(target+1) for E_pq|s> (x = p*n+q)
+  int32_t* d_bexc_full = nullptr;
+  std::vector<uint32_t> h_astr, h_bstr;
+};
+
+struct Program {
+  const Space* space;
+  std::vector<GateDev> gates;
+  std::vector<Term> terms;
+  GateDev* d_gates = nullptr;
+  Term* d_terms = nullptr;
+  std::vector<int> pair_p, pair_q;  // orbital pair (p, p+1) of "brick" gates, -1 otherwise
+};
+
+struct Hamiltonian {
+  const Space* space;
+  int npair;
+  int kpad;            // padded K (= M) of the packed GEMM, multiple of 16
+  double e_core;
+  double* d_A = nullptr;  // [kpad][kpad] transposed coefficient matrix (see sq_sigma.hip)
+};
+
+// Kernel timing classes (sq_profile.hip).
+enum ProfClass {
+  PC_ROTATE = 0,
+  PC_ROTATE_ADJ,
+  PC_TAPPLY,
+  PC_DBUILD,
+  PC_GEMM,
+  PC_SIGMA_GATHER,
+  PC_REDUCE,
+  PC_COPY,
+  PC_TILE_FWD,
+  PC_TILE_REV,
+  PC_PERM,
+  PC_RDM_GRAM,
+  PC_OTHER,
+  kProfClasses
+};
+bool prof_on();
+void prof_begin(hipStream_t st);
+void prof_end(int cls, hipStream_t st, double bytes, double flops);
+
+#define SQ_PROF(cls, st, bytes, flops, ...)          \
+  do {                                               \
+    ::sq::prof_begin(st);                            \
+    __VA_ARGS__;                                     \
+    SQ_HIP(hipGetLastError());                       \
+    ::sq::prof_end((cls), (st), (bytes), (flops));   \
+  } while (0)
+
+// Workspace layout helpers (bytes).
+int64_t sigma_work_bytes(const Space& s);
+
+// Implemented in sq_sigma.hip
+void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* energy_host,
+                double* work, hipStream_t st);
+void sigma_energy_dev(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* e_dev,
+                      double* work, hipStream_t st);
+void dot_dev(const double* x, const double* y, int64_t n, double* out_dev, int slot, double factor, double* partial,
+             hipStream_t st);
+int64_t rdm_work_bytes(const Space& s, int* rt_out, int* nchunks_out, int64_t* chunk_out);
+void rdm_impl(const Space& s, const double* psi, double* rdm1, double* rdm2, double* work, hipStream_t st);
+
+// Implemented in sq_gates.hip
+void apply_gate_impl(const Space& s, const Program& p, int g, const double* theta_dev, bool negate, double* psi,
+                     double* work, hipStream_t st);
+void grad_action_impl(const Space& s, const Program& p, int g, const double* in, double* out, hipStream_t st);
+void copy_vec(const double* in, double* out, int64_t n, hipStream_t st);
+int64_t nblocks2d(const Space& s);
+int64_t tdot_impl(const Space& s, const Program& p, int g, const double* psi, const double* lam, double* partial,
+                  hipStream_t st);
+void reduce_impl(const double* partial, int64_t n, double* out_dev, int slot, bool accumulate, double factor,
+                 hipStream_t st);
+void reverse_step_impl(const Space& s, const Program& p, int g, const double* theta_dev, double* psi, double* lam,
+                       double* grad_dev, double* work, double* partial, hipStream_t st);
+
+}  // namespace sq
+
+struct sq_space {
+  sq::Space s;
+};
+struct sq_program {
+  sq::Program p;
+};
+struct sq_hamiltonian {
+  sq::Hamiltonian h;
+};
+struct sq_operator {
+  const sq::Space* space = nullptr;
+  std::vector<sq::Term> terms;
+  sq::Term* d_terms = nullptr;
+};

@@ -1,0 +1,541 @@
+// sigma = H psi, energies and RDMs (replaces propagate_state([H]) /
+// expectation_value, operator_state_algebra.py:472-824, and the rdm1/rdm2
+// expectation-value loops of ups_wavefunction.py:409-476).
+//
+// Packed pair index k = p(p+1)/2 + q (p >= q), E+_k = E_pq + E_qp (p > q),
+// E+_pp = E_pp.  With h'_pq = h_pq - 1/2 sum_r g_prrq and the 8-fold symmetry
+// of g:
+//     sigma = e_core psi + sum_k E+_k G'_k,
+//     G'_k  = sum_{k'} A_{k k'} D_{k'},   D_{k'} = E+_{k'} psi (k' < npair),
+//             D_npair = psi,  A = [ 1/2 g(k,k') | h'_k ].
+// Kernel 1 (bandwidth): D build by single-excitation gathers.
+// Kernel 2 (FP64 MFMA, v_mfma_f64_16x16x4f64): G' = A . D.
+// Kernel 3 (bandwidth): sigma gather + <psi|sigma> partials.
+
+#include <cmath>
+#include <cstring>
+
+#include "sq_internal.h"
+
+namespace sq {
+
+void copy_vec(const double* in, double* out, int64_t n, hipStream_t st);
+void reduce_impl(const double* partial, int64_t n, double* out_dev, int slot, bool accumulate, double factor,
+                 hipStream_t st);
+
+constexpr int kB = 256;
+
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+struct SigmaDims {
+  int npair, K4, kpad;
+  int64_t dim;
+};
+
+static SigmaDims sigma_dims(const Space& s) {
+  SigmaDims d;
+  d.npair = s.npair;
+  d.K4 = round_up(s.npair + 1, 4);
+  d.kpad = round_up(s.npair + 1, 16);
+  d.dim = s.dim;
+  return d;
+}
+
+// Work layout (doubles): D [K4*dim] | G [npair*dim] | partials [nsa*ceil(nsb/256)] | scalars [8]
+int64_t sigma_work_bytes(const Space& s) {
+  SigmaDims d = sigma_dims(s);
+  int64_t nb = (int64_t)s.nsa * ((s.nsb + kB - 1) / kB);
+  return 8 * ((int64_t)d.K4 * d.dim + (int64_t)d.npair * d.dim + nb + 64);
+}
+
+// ---------------------------------------------------------------------------
+// Kernel 1: D[k][I] = (E+_k psi)[I]; row npair = psi; rows npair+1..K4-1 = 0.
+// Excitation tables are k-major: exc[k*nstr + s] = sign*(target+1), or the
+// occupation of orbital p for diagonal k.
+__global__ void k_dbuild(int n, int nsa, int nsb, int npair, int K4, const int32_t* __restrict__ aexc,
+                         const int32_t* __restrict__ bexc, const double* __restrict__ psi, double* __restrict__ D) {
+  int ia = blockIdx.y;
+  int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= nsb) return;
+  const size_t dim = (size_t)nsa * nsb;
+  const size_t I = (size_t)ia * nsb + ib;
+  const double* row = psi + (size_t)ia * nsb;
+  const double pI = psi[I];
+  int k = 0;
+  for (int p = 0; p < n; ++p) {
+    for (int q = 0; q <= p; ++q, ++k) {
+      int32_t ea = aexc[(size_t)k * nsa + ia];
+      int32_t eb = bexc[(size_t)k * nsb + ib];
+      double v;
+      if (p == q) {
+        v = (double)(ea + eb) * pI;
+      } else {
+        v = 0.0;
+        if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
+        if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
+      }
+      D[(size_t)k * dim + I] = v;
+    }
+  }
+  D[(size_t)npair * dim + I] = pI;
+  for (int r = npair + 1; r < K4; ++r) D[(size_t)r * dim + I] = 0.0;
+}
+
+// Kernel 3: sigma[I] = e_core psi[I] + sum_k (E+_k G_k)[I]; partial <psi|sigma>.
+__global__ void k_sigma_gather(int n, int nsa, int nsb, const int32_t* __restrict__ aexc,
+                               const int32_t* __restrict__ bexc, const double* __restrict__ G,
+                               const double* __restrict__ psi, double e_core, double* __restrict__ sigma,
+                               double* __restrict__ partial) {
+  __shared__ double red[kB / 64];
+  int ia = blockIdx.y;
+  int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t dim = (size_t)nsa * nsb;
+  double e = 0.0;
+  if (ib < nsb) {
+    const size_t I = (size_t)ia * nsb + ib;
+    double v = e_core * psi[I];
+    int k = 0;
+    for (int p = 0; p < n; ++p) {
+      for (int q = 0; q <= p; ++q, ++k) {
+        const double* Gk = G + (size_t)k * dim;
+        int32_t ea = aexc[(size_t)k * nsa + ia];
+        int32_t eb = bexc[(size_t)k * nsb + ib];
+        if (p == q) {
+          v += (double)(ea + eb) * Gk[I];
+        } else {
+          if (ea) v += (ea > 0 ? 1.0 : -1.0) * Gk[(size_t)(abs(ea) - 1) * nsb + ib];
+          if (eb) v += (eb > 0 ? 1.0 : -1.0) * Gk[(size_t)ia * nsb + abs(eb) - 1];
+        }
+      }
+    }
+    sigma[I] = v;
+    e = v * psi[I];
+  }
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = e;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// Kernel 2: G[m][c] = sum_k At[k][m] D[k][c] for m < npair, k < K4, with
+// v_mfma_f64_16x16x4f64.  Lane l supplies A[l%16][l/16] and B[l/16][l%16];
+// accumulator register r of lane l holds C[(l/16) + 4r][l%16] (measured on
+// gfx950 against a host GEMM).  One wave owns 16 columns and MT
+// row tiles starting at row m0 = blockIdx.y * MT * 16; the matching column
+// slice of At (K4 x 16*MT) is staged in LDS and shared by the block.
+template <int MT>
+__global__ __launch_bounds__(512) void k_gemm_mfma(const double* __restrict__ At, const double* __restrict__ D,
+                                                   double* __restrict__ G, int npair, int K4, int kpad,
+                                                   int64_t dim) {
+  extern __shared__ double sA[];
+  constexpr int W = MT * 16;
+  const int m0 = blockIdx.y * W;
+  for (int i = threadIdx.x; i < K4 * W; i += blockDim.x) {
+    int k = i / W, m = m0 + i % W;
+    sA[i] = m < kpad ? At[(size_t)k * kpad + m] : 0.0;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t c = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 16 + (lane & 15);
+  const int kq = lane >> 4;
+  const bool valid = c < dim;
+  typedef double double4_t __attribute__((ext_vector_type(4)));
+  double4_t acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  const double* Dp = D + (size_t)kq * dim + (valid ? c : 0);
+  double bnext = valid ? Dp[0] : 0.0;
+  for (int k4 = 0; k4 < K4; k4 += 4) {
+    double b = bnext;
+    if (k4 + 4 < K4) bnext = valid ? Dp[(size_t)(k4 + 4) * dim] : 0.0;
+    const double* arow = sA + (size_t)(k4 + kq) * W + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < MT; ++t) {
+      double a = arow[t * 16];
+      acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int m = m0 + t * 16 + kq + 4 * r;
+      if (m < npair) G[(size_t)m * dim + c] = acc[t][r];
+    }
+  }
+}
+
+static void launch_gemm(const SigmaDims& d, const double* At, const double* D, double* G, hipStream_t st) {
+  const int threads = 512;
+  const int64_t cols_per_block = (threads / 64) * 16;
+  const int mt_total = d.kpad / 16;
+  // largest tile group whose At slice fits in 64 KiB of LDS
+  int mt = mt_total;
+  while (mt > 1 && (size_t)d.K4 * mt * 16 * sizeof(double) > 65536) --mt;
+  const int groups = (mt_total + mt - 1) / mt;
+  dim3 grid((unsigned)((d.dim + cols_per_block - 1) / cols_per_block), groups);
+  size_t lds = sizeof(double) * (size_t)d.K4 * mt * 16;
+  const double gbytes = 8.0 * (double)d.dim * (d.K4 + d.npair);
+  const double gflops = 2.0 * (double)d.npair * (d.npair + 1) * (double)d.dim;
+  switch (mt) {
+#define SQ_G(MT)                                                                              \
+  case MT:                                                                                    \
+    SQ_PROF(PC_GEMM, st, gbytes, gflops,                                                      \
+            k_gemm_mfma<MT><<<grid, threads, lds, st>>>(At, D, G, d.npair, d.K4, d.kpad, d.dim)); \
+    break;
+    SQ_G(1) SQ_G(2) SQ_G(3) SQ_G(4) SQ_G(5) SQ_G(6) SQ_G(7) SQ_G(8) SQ_G(9)
+#undef SQ_G
+    default:
+      throw Error{SQ_ERR_ARG, "unsupported orbital count for the sigma GEMM"};
+  }
+}
+
+void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* energy_host,
+                double* work, hipStream_t st) {
+  SigmaDims d = sigma_dims(s);
+  if (s.dim == 0) return;
+  const int32_t* ae = s.d_aexc;
+  const int32_t* be = s.d_bexc;
+  double* D = work;
+  double* G = D + (size_t)d.K4 * d.dim;
+  double* partial = G + (size_t)d.npair * d.dim;
+  int64_t nb = (int64_t)s.nsa * ((s.nsb + kB - 1) / kB);
+  double* scal = partial + nb;
+  dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
+  const double d8 = 8.0 * (double)d.dim;
+  SQ_PROF(PC_DBUILD, st, d8 * (d.K4 + 1), 0.0,
+          k_dbuild<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, d.npair, d.K4, ae, be, psi, D));
+  launch_gemm(d, H.d_A, D, G, st);
+  SQ_PROF(PC_SIGMA_GATHER, st, d8 * (d.npair + 2), 0.0,
+          k_sigma_gather<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
+  if (energy_host) {
+    reduce_impl(partial, nb, scal, 0, false, 1.0, st);
+    SQ_HIP(hipMemcpyAsync(energy_host, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+    SQ_HIP(hipStreamSynchronize(st));
+  }
+}
+
+// Device-resident energy: same as sigma_impl but leaves <psi|sigma> in e_dev[0].
+void sigma_energy_dev(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* e_dev,
+                      double* work, hipStream_t st) {
+  SigmaDims d = sigma_dims(s);
+  const int32_t* ae = s.d_aexc;
+  const int32_t* be = s.d_bexc;
+  double* D = work;
+  double* G = D + (size_t)d.K4 * d.dim;
+  double* partial = G + (size_t)d.npair * d.dim;
+  int64_t nb = (int64_t)s.nsa * ((s.nsb + kB - 1) / kB);
+  dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
+  const double d8 = 8.0 * (double)d.dim;
+  SQ_PROF(PC_DBUILD, st, d8 * (d.K4 + 1), 0.0,
+          k_dbuild<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, d.npair, d.K4, ae, be, psi, D));
+  launch_gemm(d, H.d_A, D, G, st);
+  SQ_PROF(PC_SIGMA_GATHER, st, d8 * (d.npair + 2), 0.0,
+          k_sigma_gather<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
+  reduce_impl(partial, nb, e_dev, 0, false, 1.0, st);
+}
+
+// ---------------------------------------------------------------------------
+// Dot products (two-stage, deterministic).
+__global__ void k_dot_partial(const double* __restrict__ x, const double* __restrict__ y, int64_t n,
+                              double* __restrict__ partial) {
+  __shared__ double red[kB / 64];
+  double v = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    v += x[i] * y[i];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+void dot_dev(const double* x, const double* y, int64_t n, double* out_dev, int slot, double factor, double* partial,
+             hipStream_t st) {
+  int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (n + kB - 1) / kB));
+  k_dot_partial<<<nb, kB, 0, st>>>(x, y, n, partial);
+  SQ_HIP(hipGetLastError());
+  reduce_impl(partial, nb, out_dev, slot, false, factor, st);
+}
+
+// ---------------------------------------------------------------------------
+// RDMs.  Rows of Dfull: x = p*n + q holds (E_pq psi), row n*n holds psi, the
+// remaining padding rows are zero.  M = Dfull Dfull^T is accumulated with
+// v_mfma_f64_16x16x4f64 over determinant chunks (one partial per chunk,
+// summed in a fixed order), then
+//   rdm1_pq   = M[p*n+q][n*n]                  (= <E_pq psi|psi>)
+//   rdm2_pqrs = M[q*n+p][r*n+s] - delta_qr rdm1_ps   (<psi|E_pq E_rs|psi> - ...)
+__global__ void k_dbuild_full(int n, int nsa, int nsb, int nrow_pad, const int32_t* __restrict__ afull,
+                              const int32_t* __restrict__ bfull, const double* __restrict__ psi,
+                              double* __restrict__ D) {
+  int ia = blockIdx.y;
+  int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= nsb) return;
+  const size_t dim = (size_t)nsa * nsb;
+  const size_t I = (size_t)ia * nsb + ib;
+  const double* row = psi + (size_t)ia * nsb;
+  const double pI = psi[I];
+  for (int p = 0; p < n; ++p) {
+    for (int q = 0; q < n; ++q) {
+      // (E_pq psi)[I] = <I|E_pq|J> psi[J] with J the E_qp image of I.
+      int x = q * n + p;
+      int32_t ea = afull[(size_t)x * nsa + ia];
+      int32_t eb = bfull[(size_t)x * nsb + ib];
+      double v;
+      if (p == q) {
+        v = (double)(ea + eb) * pI;
+      } else {
+        v = 0.0;
+        if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
+        if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
+      }
+      D[(size_t)(p * n + q) * dim + I] = v;
+    }
+  }
+  D[(size_t)(n * n) * dim + I] = pI;
+  for (int r = n * n + 1; r < nrow_pad; ++r) D[(size_t)r * dim + I] = 0.0;
+}
+
+// Partial M over one chunk of columns: WAVES waves per block; wave w owns
+// upper-triangle output tiles (ti <= tj) w, w+WAVES, ... of the RT x RT grid.
+template <int RT, int WAVES>
+__global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(const double* __restrict__ D, int64_t dim, int64_t chunk,
+                                                          double* __restrict__ partial) {
+  typedef double double4_t __attribute__((ext_vector_type(4)));
+  constexpr int NT = RT * (RT + 1) / 2;
+  constexpr int PER = (NT + WAVES - 1) / WAVES;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int kq = lane >> 4, li = lane & 15;
+  int tile_i[PER], tile_j[PER];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    int tile = wave + WAVES * t, ti = 0;
+    while (tile >= RT - ti && ti < RT) {
+      tile -= RT - ti;
+      ++ti;
+    }
+    tile_i[t] = ti;
+    tile_j[t] = ti + tile;
+  }
+  double4_t acc[PER];
+#pragma unroll
+  for (int t = 0; t < PER; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  const int64_t c0 = (int64_t)blockIdx.x * chunk;
+  const int64_t c1 = c0 + chunk < dim ? c0 + chunk : dim;
+  for (int64_t c = c0; c < c1; c += 4) {
+    const int64_t col = c + kq;
+    const bool ok = col < c1;
+    double frag[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) frag[r] = ok ? D[(size_t)(r * 16 + li) * dim + col] : 0.0;
+#pragma unroll
+    for (int t = 0; t < PER; ++t) {
+      if (wave + WAVES * t < NT) {
+        // A[i][k] = D[ti*16+i][c+k], B[k][j] = D[tj*16+j][c+k]
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+          if (r == tile_i[t]) a = frag[r];
+          if (r == tile_j[t]) b = frag[r];
+        }
+        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+      }
+    }
+  }
+  const int ld = RT * 16;
+  double* out = partial + (size_t)blockIdx.x * ld * ld;
+#pragma unroll
+  for (int t = 0; t < PER; ++t) {
+    if (wave + WAVES * t < NT) {
+      const int ti = tile_i[t], tj = tile_j[t];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[(size_t)(ti * 16 + kq + 4 * r) * ld + tj * 16 + li] = acc[t][r];
+    }
+  }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int nchunks, int64_t len,
+                               double* __restrict__ out) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  double v = 0.0;
+  for (int c = 0; c < nchunks; ++c) v += partial[(size_t)c * len + i];
+  out[i] = v;
+}
+
+int64_t rdm_work_bytes(const Space& s, int* rt_out, int* nchunks_out, int64_t* chunk_out) {
+  const int rows = s.n * s.n + 1;
+  const int rt = (rows + 15) / 16;
+  int64_t chunk = 4096;
+  int64_t nchunks = (s.dim + chunk - 1) / chunk;
+  while (nchunks > 1024) {
+    chunk *= 2;
+    nchunks = (s.dim + chunk - 1) / chunk;
+  }
+  if (nchunks < 1) nchunks = 1;
+  if (rt_out) *rt_out = rt;
+  if (nchunks_out) *nchunks_out = (int)nchunks;
+  if (chunk_out) *chunk_out = chunk;
+  const int64_t ld = (int64_t)rt * 16;
+  return 8 * (ld * s.dim + nchunks * ld * ld + ld * ld + 64);
+}
+
+void rdm_impl(const Space& s, const double* psi, double* rdm1, double* rdm2, double* work, hipStream_t st) {
+  int rt, nchunks;
+  int64_t chunk;
+  rdm_work_bytes(s, &rt, &nchunks, &chunk);
+  const int n = s.n;
+  const int64_t ld = (int64_t)rt * 16;
+  double* D = work;
+  double* partial = D + (size_t)ld * s.dim;
+  double* M = partial + (size_t)nchunks * ld * ld;
+  dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
+  SQ_PROF(PC_DBUILD, st, 8.0 * (double)s.dim * (ld + 1), 0.0,
+          k_dbuild_full<<<grid, kB, 0, st>>>(n, s.nsa, s.nsb, (int)ld, s.d_aexc_full, s.d_bexc_full, psi, D));
+  const double gram_flops = 2.0 * 256.0 * (rt * (rt + 1) / 2) * (double)s.dim;
+  switch (rt) {
+#define SQ_R(RT)                                                                                 \
+  case RT:                                                                                       \
+    if (RT >= 6)                                                                                 \
+      SQ_PROF(PC_RDM_GRAM, st, 8.0 * (double)s.dim * ld, gram_flops,                             \
+              k_gram_mfma<RT, 16><<<nchunks, 1024, 0, st>>>(D, s.dim, chunk, partial));         \
+    else                                                                                         \
+      SQ_PROF(PC_RDM_GRAM, st, 8.0 * (double)s.dim * ld, gram_flops,                             \
+              k_gram_mfma<RT, 4><<<nchunks, 256, 0, st>>>(D, s.dim, chunk, partial));           \
+    break;
+    SQ_R(1) SQ_R(2) SQ_R(3) SQ_R(4) SQ_R(5) SQ_R(6) SQ_R(7) SQ_R(8) SQ_R(9) SQ_R(10) SQ_R(11) SQ_R(12) SQ_R(13)
+#undef SQ_R
+    default:
+      throw Error{SQ_ERR_ARG, "unsupported orbital count for the RDM kernel"};
+  }
+  SQ_HIP(hipGetLastError());
+  const int64_t len = ld * ld;
+  k_sum_partials<<<(unsigned)((len + kB - 1) / kB), kB, 0, st>>>(partial, nchunks, len, M);
+  SQ_HIP(hipGetLastError());
+  std::vector<double> hM(len);
+  SQ_HIP(hipMemcpyAsync(hM.data(), M, sizeof(double) * len, hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  const int nn = n * n;
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q) {
+      const int x = p * n + q;
+      rdm1[p * n + q] = (x / 16 <= nn / 16) ? hM[(size_t)x * ld + nn] : hM[(size_t)nn * ld + x];
+    }
+  for (int p = 0; p < n; ++p)
+    for (int q = 0; q < n; ++q)
+      for (int r = 0; r < n; ++r)
+        for (int t = 0; t < n; ++t) {
+          const int x = q * n + p, y = r * n + t;
+          // only tiles with row tile <= column tile were accumulated
+          double v = (x / 16 <= y / 16) ? hM[(size_t)x * ld + y] : hM[(size_t)y * ld + x];
+          if (q == r) v -= rdm1[p * n + t];
+          rdm2[((size_t)(p * n + q) * n + r) * n + t] = v;
+        }
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+extern "C" {
+
+int sq_hamiltonian_create(const sq_space* sp, const double* h, const double* g, double e_core, sq_hamiltonian** out) {
+  return guarded([&] {
+    SQ_CHECK(sp && h && g && out, SQ_ERR_ARG, "NULL argument");
+    const Space& s = sp->s;
+    SigmaDims d = sigma_dims(s);
+    const int n = s.n;
+    auto H = [&](int p, int q) { double v = h[p * n + q]; return std::fabs(v) > 1e-14 ? v : 0.0; };
+    auto Gf = [&](int p, int q, int r, int t) {
+      double v = g[((size_t)(p * n + q) * n + r) * n + t];
+      return std::fabs(v) > 1e-14 ? v : 0.0;
+    };
+    // symmetry check (the packed contraction relies on it)
+    for (int p = 0; p < n; ++p)
+      for (int q = 0; q < n; ++q) {
+        SQ_CHECK(std::fabs(H(p, q) - H(q, p)) <= 1e-12 * (1.0 + std::fabs(H(p, q))), SQ_ERR_ARG,
+                 "h must be symmetric");
+        for (int r = 0; r < n; ++r)
+          for (int t = 0; t < n; ++t) {
+            double v = Gf(p, q, r, t);
+            double tol = 1e-12 * (1.0 + std::fabs(v));
+            SQ_CHECK(std::fabs(v - Gf(q, p, r, t)) <= tol && std::fabs(v - Gf(p, q, t, r)) <= tol &&
+                         std::fabs(v - Gf(r, t, p, q)) <= tol,
+                     SQ_ERR_ARG, "g must have the 8-fold real symmetry");
+          }
+      }
+    std::vector<double> At((size_t)d.K4 * d.kpad, 0.0);
+    std::vector<int> kp(d.npair), kq(d.npair);
+    for (int p = 0, k = 0; p < n; ++p)
+      for (int q = 0; q <= p; ++q, ++k) kp[k] = p, kq[k] = q;
+    for (int m = 0; m < d.npair; ++m) {
+      const int p = kp[m], q = kq[m];
+      for (int k = 0; k < d.npair; ++k) At[(size_t)k * d.kpad + m] = 0.5 * Gf(p, q, kp[k], kq[k]);
+      double hp = H(p, q);
+      for (int r = 0; r < n; ++r) hp -= 0.5 * Gf(p, r, r, q);
+      At[(size_t)d.npair * d.kpad + m] = hp;
+    }
+    auto* ho = new sq_hamiltonian();
+    ho->h.space = &s;
+    ho->h.npair = d.npair;
+    ho->h.kpad = d.kpad;
+    ho->h.e_core = e_core;
+    hipError_t e = hipMalloc(&ho->h.d_A, sizeof(double) * At.size());
+    if (e == hipSuccess) e = hipMemcpy(ho->h.d_A, At.data(), sizeof(double) * At.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (ho->h.d_A) (void)hipFree(ho->h.d_A);
+      delete ho;
+      throw Error{SQ_ERR_HIP, hipGetErrorString(e)};
+    }
+    *out = ho;
+  });
+}
+
+int sq_hamiltonian_destroy(sq_hamiltonian* H) {
+  return guarded([&] {
+    if (!H) return;
+    if (H->h.d_A) (void)hipFree(H->h.d_A);
+    delete H;
+  });
+}
+
+int sq_sigma(const sq_space* sp, const sq_hamiltonian* H, const double* psi, double* sigma, double* energy,
+             double* work, void* stream) {
+  return guarded([&] {
+    SQ_CHECK(sp && H && psi && sigma && work, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(psi != sigma, SQ_ERR_ARG, "psi and sigma must differ");
+    sigma_impl(sp->s, H->h, psi, sigma, energy, work, as_stream(stream));
+  });
+}
+
+int sq_rdm12(const sq_space* sp, const double* psi, double* rdm1, double* rdm2, double* work, void* stream) {
+  return guarded([&] {
+    SQ_CHECK(sp && psi && rdm1 && rdm2 && work, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(sp->s.n * sp->s.n + 1 <= 13 * 16, SQ_ERR_ARG, "RDM kernel supports at most 14 orbitals");
+    rdm_impl(sp->s, psi, rdm1, rdm2, work, as_stream(stream));
+  });
+}
+
+}  // extern "C"
+
+extern "C" int sq_debug_gemm(const double* At, const double* D, double* G, int npair, int K4, int kpad, int64_t dim,
+                             void* stream) {
+  return guarded([&] {
+    SQ_CHECK(At && D && G && npair > 0 && K4 % 4 == 0 && kpad % 16 == 0 && npair <= kpad && K4 <= kpad, SQ_ERR_ARG,
+             "invalid GEMM shape");
+    SigmaDims d;
+    d.npair = npair;
+    d.K4 = K4;
+    d.kpad = kpad;
+    d.dim = dim;
+    launch_gemm(d, At, D, G, as_stream(stream));
+  });
+}

@@ -1,0 +1,251 @@
+"""Host objects over the HIP engine: CI space, gate programs, Hamiltonians.
+
+Device memory and streams come from PyTorch (plumbing only); every numerical
+operation runs in libsqhip.so.  State vectors are float64 CUDA tensors in the
+engine's internal (alpha block, beta block) sign convention; use
+``Space.from_reference`` / ``Space.to_reference`` at the boundary with
+reference-convention ``ci_coeffs``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+
+def _stream() -> int:
+    return torch.cuda.current_stream().cuda_stream
+
+
+def _ptr(t: torch.Tensor | None) -> int | None:
+    if t is None:
+        return None
+    if t.dtype != torch.float64 or not t.is_cuda or not t.is_contiguous():
+        raise ValueError("engine buffers must be contiguous float64 CUDA tensors")
+    return t.data_ptr()
+
+
+def require_gpu() -> None:
+    if not torch.cuda.is_available():
+        raise RuntimeError("the HIP state-vector engine needs a GPU (torch.cuda.is_available() is False)")
+    N.lib()
+
+
+class Space:
+    """CI space of (n_orb, n_alpha, n_beta) built on the device (ci_spaces.get_indexing)."""
+
+    def __init__(self, n_orb: int, n_alpha: int, n_beta: int) -> None:
+        require_gpu()
+        self.n_orb, self.n_alpha, self.n_beta = n_orb, n_alpha, n_beta
+        h = ctypes.c_void_p()
+        N.check(N.lib().sq_space_create(n_orb, n_alpha, n_beta, ctypes.byref(h)))
+        self._h = h
+        self.dim = int(N.lib().sq_space_dim(h))
+        na, nb = ctypes.c_int32(), ctypes.c_int32()
+        N.check(N.lib().sq_space_nstrings(h, ctypes.byref(na), ctypes.byref(nb)))
+        self.n_alpha_strings, self.n_beta_strings = na.value, nb.value
+        self._work: dict[int, torch.Tensor] = {}
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and N._lib is not None:
+            N.lib().sq_space_destroy(h)
+            self._h = None
+
+    def work(self, what: int) -> torch.Tensor:
+        if what not in self._work:
+            nbytes = int(N.lib().sq_work_bytes(self._h, what))
+            if nbytes < 0:
+                raise ValueError("unknown workspace kind")
+            self._work[what] = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device="cuda")
+        return self._work[what]
+
+    def strings(self) -> tuple[np.ndarray, np.ndarray]:
+        a = np.zeros(self.n_alpha_strings, dtype=np.uint32)
+        b = np.zeros(self.n_beta_strings, dtype=np.uint32)
+        N.check(N.lib().sq_space_copy_strings(self._h, a.ctypes.data, b.ctypes.data))
+        return a, b
+
+    def idx2det(self) -> np.ndarray:
+        out = np.zeros(self.dim, dtype=np.int64)
+        N.check(N.lib().sq_space_idx2det(self._h, out.ctypes.data, _stream()))
+        return out
+
+    def from_reference(self, ref) -> torch.Tensor:
+        x = torch.as_tensor(np.ascontiguousarray(ref, dtype=np.float64) if not torch.is_tensor(ref) else ref)
+        x = x.to(device="cuda", dtype=torch.float64).contiguous()
+        out = torch.empty_like(x)
+        N.check(N.lib().sq_state_from_reference(self._h, _ptr(x), _ptr(out), _stream()))
+        return out
+
+    def to_reference(self, internal: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(internal)
+        N.check(N.lib().sq_state_to_reference(self._h, _ptr(internal), _ptr(out), _stream()))
+        return out
+
+    def to_reference_numpy(self, internal: torch.Tensor) -> np.ndarray:
+        return self.to_reference(internal).cpu().numpy()
+
+
+class Program:
+    """A gate list (UpsStructure / UccStructure layout) compiled for a space."""
+
+    def __init__(self, space: Space, types, indices) -> None:
+        self.space = space
+        self.types = list(types)
+        self.indices = [tuple(int(x) for x in t) for t in indices]
+        descs = N.gate_descs(self.types, self.indices)
+        h = ctypes.c_void_p()
+        N.check(N.lib().sq_program_create(space.handle, descs, len(self.types), ctypes.byref(h)))
+        self._h = h
+        self.n_gates = len(self.types)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and N._lib is not None:
+            N.lib().sq_program_destroy(h)
+            self._h = None
+
+    def theta_tensor(self, theta) -> torch.Tensor:
+        t = torch.as_tensor(np.asarray(theta, dtype=np.float64)).to("cuda")
+        if t.numel() != self.n_gates:
+            raise ValueError(f"Expected {self.n_gates} theta values got {t.numel()}")
+        return t.contiguous()
+
+    def state_prep(self, theta, psi_in: torch.Tensor, dagger: bool = False, out: torch.Tensor | None = None):
+        th = self.theta_tensor(theta)
+        if out is None:
+            out = torch.empty_like(psi_in)
+        N.check(
+            N.lib().sq_state_prep(
+                self.space.handle, self._h, _ptr(th), _ptr(psi_in), _ptr(out), int(dagger), _ptr(self.space.work(0)), _stream()
+            )
+        )
+        return out
+
+    def apply_gate(self, gate: int, theta, psi: torch.Tensor) -> torch.Tensor:
+        th = self.theta_tensor(theta)
+        N.check(N.lib().sq_apply_gate(self.space.handle, self._h, gate, _ptr(th), _ptr(psi), _ptr(self.space.work(0)), _stream()))
+        return psi
+
+    def grad_action(self, gate: int, psi: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(psi)
+        N.check(N.lib().sq_grad_action(self.space.handle, self._h, gate, _ptr(psi), _ptr(out), _stream()))
+        return out
+
+    def energy_grad(self, H: "Hamiltonian", theta, psi0: torch.Tensor, method: str = "adjoint"):
+        th = self.theta_tensor(theta)
+        e = ctypes.c_double()
+        g = np.zeros(max(self.n_gates, 1))
+        fn = {
+            "adjoint": N.lib().sq_energy_grad_adjoint,
+            "plain": N.lib().sq_energy_grad_plain,
+            "reference": N.lib().sq_energy_grad_reference_sweep,
+        }[method]
+        N.check(
+            fn(self.space.handle, self._h, H.handle, _ptr(th), _ptr(psi0), ctypes.byref(e), g.ctypes.data,
+               _ptr(self.space.work(2)), _stream())
+        )
+        return e.value, g[: self.n_gates].copy()
+
+
+class Hamiltonian:
+    """Active-space Hamiltonian e_core + h.E + 1/2 g.e on the device."""
+
+    def __init__(self, space: Space, h: np.ndarray, g: np.ndarray, e_core: float = 0.0) -> None:
+        n = space.n_orb
+        h = np.ascontiguousarray(h, dtype=np.float64).reshape(n, n)
+        g = np.ascontiguousarray(g, dtype=np.float64).reshape(n, n, n, n)
+        self.space = space
+        self.e_core = float(e_core)
+        hh = ctypes.c_void_p()
+        N.check(N.lib().sq_hamiltonian_create(space.handle, h.ctypes.data, g.ctypes.data, self.e_core, ctypes.byref(hh)))
+        self._h = hh
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and N._lib is not None:
+            N.lib().sq_hamiltonian_destroy(h)
+            self._h = None
+
+    def sigma(self, psi: torch.Tensor, want_energy: bool = True):
+        out = torch.empty_like(psi)
+        e = ctypes.c_double()
+        N.check(
+            N.lib().sq_sigma(self.space.handle, self._h, _ptr(psi), _ptr(out), ctypes.byref(e) if want_energy else None,
+                             _ptr(self.space.work(1)), _stream())
+        )
+        return out, (e.value if want_energy else None)
+
+
+def rdm12(space: Space, psi: torch.Tensor) -> tuple[np.ndarray, np.ndarray]:
+    n = space.n_orb
+    r1 = np.zeros((n, n))
+    r2 = np.zeros((n, n, n, n))
+    N.check(N.lib().sq_rdm12(space.handle, _ptr(psi), r1.ctypes.data, r2.ctypes.data, _ptr(space.work(3)), _stream()))
+    return r1, r2
+
+
+def hf_state(space: Space, occ_det_index: int = 0) -> torch.Tensor:
+    """Reference-convention unit vector at ``occ_det_index`` converted to internal."""
+    ref = torch.zeros(space.dim, dtype=torch.float64, device="cuda")
+    ref[occ_det_index] = 1.0
+    return space.from_reference(ref)
+
+
+class Operator:
+    """A FermionicOperator (active-space, already folded) compiled for a space."""
+
+    def __init__(self, space: Space, items, drop_outside: bool = False) -> None:
+        items = list(items)
+        arr = (N.OpString * max(len(items), 1))()
+        for k, (key, coef) in enumerate(items):
+            if len(key) > 24:
+                raise ValueError("operator string longer than 24 operators")
+            arr[k].coef = float(coef)
+            arr[k].n_ops = len(key)
+            for j, (idx, dag) in enumerate(key):
+                arr[k].spin_orb[j] = int(idx)
+                arr[k].is_creator[j] = 1 if dag else 0
+        h = ctypes.c_void_p()
+        N.check(N.lib().sq_operator_create(space.handle, arr, len(items), int(drop_outside), ctypes.byref(h)))
+        self._h = h
+        self.space = space
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and N._lib is not None:
+            N.lib().sq_operator_destroy(h)
+            self._h = None
+
+    def apply(self, psi: torch.Tensor) -> torch.Tensor:
+        out = torch.empty_like(psi)
+        N.check(N.lib().sq_operator_apply(self.space.handle, self._h, _ptr(psi), _ptr(out), _stream()))
+        return out
+
+
+_SPACES: dict = {}
+
+
+def space_for(n_orb: int, n_alpha: int, n_beta: int) -> Space:
+    """Process-wide cache of device CI spaces."""
+    key = (n_orb, n_alpha, n_beta)
+    if key not in _SPACES:
+        _SPACES[key] = Space(*key)
+    return _SPACES[key]

@@ -1,0 +1,177 @@
+"""Module-level state functions with the reference's signatures
+(operator_state_algebra.py:472-2976), executed by the HIP engine.
+
+Inputs and outputs are host numpy vectors in the reference's index order and
+sign convention; every function returns a new array and never mutates its
+inputs, like the reference.  Internally each call converts to the engine's
+sign convention on the device, runs the kernels and converts back.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Sequence
+
+import numpy as np
+import torch
+
+from . import engine as E
+from .ci_spaces import CI_Info
+from .fermionic_operator import FermionicOperator
+from .util import UccStructure, UpsStructure
+
+
+def _space(ci: CI_Info) -> E.Space:
+    if ci.space_extension_offset != 0:
+        raise NotImplementedError("extended CI spaces (linear response) are not supported by the engine")
+    return E.space_for(ci.num_active_orbs, ci.num_active_elec_alpha, ci.num_active_elec_beta)
+
+
+_PROGRAMS: dict = {}
+
+
+def _program(sp: E.Space, layout) -> E.Program:
+    key = (id(sp), tuple(layout.excitation_operator_type), tuple(tuple(x) for x in layout.excitation_indices))
+    prog = _PROGRAMS.get(key)
+    if prog is None:
+        prog = E.Program(sp, layout.excitation_operator_type, layout.excitation_indices)
+        if len(_PROGRAMS) > 64:
+            _PROGRAMS.clear()
+        _PROGRAMS[key] = prog
+    return prog
+
+
+def _ucc_program(sp: E.Space, layout: UccStructure) -> E.Program:
+    types = ["sa_single_ucc" if t == "sa_single" else t for t in layout.excitation_operator_type]
+
+    class _L:
+        excitation_operator_type = types
+        excitation_indices = layout.excitation_indices
+
+    return _program(sp, _L)
+
+
+def _to_device(sp: E.Space, state) -> torch.Tensor:
+    v = np.asarray(state, dtype=np.float64)
+    if v.shape != (sp.dim,):
+        raise ValueError(f"state has shape {v.shape}, expected ({sp.dim},)")
+    return sp.from_reference(v)
+
+
+def _to_host(sp: E.Space, t: torch.Tensor) -> np.ndarray:
+    return sp.to_reference_numpy(t)
+
+
+def construct_ups_state(state, ci_info: CI_Info, thetas, ups_struct: UpsStructure, dagger: bool = False) -> np.ndarray:
+    sp = _space(ci_info)
+    if len(thetas) != ups_struct.n_params:
+        raise ValueError(f"Expected {ups_struct.n_params} thetas got {len(thetas)}")
+    prog = _program(sp, ups_struct)
+    out = prog.state_prep(np.asarray(thetas, dtype=np.float64), _to_device(sp, state), dagger=dagger)
+    return _to_host(sp, out)
+
+
+def propagate_unitary(state, idx: int, ci_info: CI_Info, thetas, ups_struct: UpsStructure) -> np.ndarray:
+    sp = _space(ci_info)
+    prog = _program(sp, ups_struct)
+    psi = _to_device(sp, state)
+    prog.apply_gate(idx, np.asarray(thetas, dtype=np.float64), psi)
+    return _to_host(sp, psi)
+
+
+def get_grad_action(state, idx: int, ci_info: CI_Info, ups_struct: UpsStructure) -> np.ndarray:
+    sp = _space(ci_info)
+    prog = _program(sp, ups_struct)
+    return _to_host(sp, prog.grad_action(idx, _to_device(sp, state)))
+
+
+def get_ucc_T(thetas, ucc_struct: UccStructure, offset: int = 0) -> FermionicOperator:
+    """T(theta) as a FermionicOperator (operator_state_algebra.py:899-960)."""
+    from .operators import G1, G2, G3, G4, G5, G6, G1_sa, G2_sa
+
+    T = FermionicOperator({})
+    for t, idx, th in zip(ucc_struct.excitation_operator_type, ucc_struct.excitation_indices, thetas):
+        if abs(th) < 1e-28:
+            continue
+        th = float(th)
+        if t == "sa_single":
+            T += th * G1_sa(*(np.array(idx) + offset), True)
+        elif t.startswith("sa_double_"):
+            T += th * G2_sa(*(np.array(idx) + offset), int(t[-1]), True)
+        else:
+            fn = {"single": G1, "double": G2, "triple": G3, "quadruple": G4, "quintuple": G5, "sextuple": G6}.get(t)
+            if fn is None:
+                raise ValueError(f"Got unknown excitation type, {t}")
+            T += th * fn(*(np.array(idx) + 2 * offset), True)
+    return T
+
+
+def construct_ucc_state(state, ci_info: CI_Info, thetas, ucc_struct: UccStructure, dagger: bool = False) -> np.ndarray:
+    """exp(T(theta)) state, matrix-free on the GPU (operator_state_algebra.py:870-896)."""
+    from .ucc import expm_apply
+
+    sp = _space(ci_info)
+    prog = _ucc_program(sp, ucc_struct)
+    th = np.asarray(thetas, dtype=np.float64)
+    out = expm_apply(prog, -th if dagger else th, _to_device(sp, state))
+    return _to_host(sp, out)
+
+
+def _fold(op: FermionicOperator, ci: CI_Info, do_folding: bool) -> FermionicOperator:
+    if do_folding:
+        return op.get_folded_operator(ci.num_inactive_orbs, ci.num_active_orbs, ci.num_virtual_orbs)
+    return op
+
+
+def _apply_list(operators, psi: torch.Tensor, ci: CI_Info, sp: E.Space, thetas, wf_struct, do_folding, do_unsafe):
+    for op in operators[::-1]:
+        if isinstance(op, str):
+            if op not in ("U", "Ud"):
+                raise ValueError(f"Unknown str operator, expected ('U', 'Ud') got {op}")
+            if thetas is None:
+                raise ValueError("theta must be different from None")
+            th = np.asarray(thetas, dtype=np.float64)
+            if isinstance(wf_struct, UpsStructure):
+                psi = _program(sp, wf_struct).state_prep(th, psi, dagger=(op == "Ud"))
+            elif isinstance(wf_struct, UccStructure):
+                from .ucc import expm_apply
+
+                psi = expm_apply(_ucc_program(sp, wf_struct), -th if op == "Ud" else th, psi)
+            else:
+                raise TypeError(f"Got unknown wave function structure type, {type(wf_struct)}")
+        else:
+            folded = _fold(op, ci, do_folding)
+            psi = E.Operator(sp, folded.operators.items(), drop_outside=do_unsafe).apply(psi)
+    return psi
+
+
+def propagate_state(operators: list, state, ci_info: CI_Info, thetas: Sequence[float] | None = None,
+                    wf_struct: UpsStructure | UccStructure | None = None, do_folding: bool = True,
+                    do_unsafe: bool = False) -> np.ndarray:
+    """Apply a list of FermionicOperators and/or "U"/"Ud" right to left."""
+    if len(operators) == 0:
+        return np.array(state, dtype=np.float64, copy=True)
+    sp = _space(ci_info)
+    psi = _apply_list(operators, _to_device(sp, state), ci_info, sp, thetas, wf_struct, do_folding, do_unsafe)
+    return _to_host(sp, psi)
+
+
+def expectation_value(bra, operators: list, ket, ci_info: CI_Info, thetas=None, wf_struct=None,
+                      do_folding: bool = True, do_unsafe: bool = False) -> float:
+    val = np.asarray(bra, dtype=np.float64) @ propagate_state(operators, ket, ci_info, thetas, wf_struct,
+                                                               do_folding=do_folding, do_unsafe=do_unsafe)
+    return float(val)
+
+
+def build_operator_matrix(op: FermionicOperator, ci_info: CI_Info, do_unsafe: bool = False) -> np.ndarray:
+    """Dense matrix of an active-space operator (operator_state_algebra.py:413-469), column by column."""
+    sp = _space(ci_info)
+    O = E.Operator(sp, op.operators.items(), drop_outside=do_unsafe)
+    d = sp.dim
+    M = np.zeros((d, d))
+    eye = torch.zeros(d, dtype=torch.float64, device="cuda")
+    for j in range(d):
+        eye.zero_()
+        eye[j] = 1.0
+        ref = sp.from_reference(eye)
+        M[:, j] = _to_host(sp, O.apply(ref))
+    return M

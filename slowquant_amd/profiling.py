@@ -1,0 +1,75 @@
+"""Per-kernel timing with HIP events recorded by the engine on its stream."""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as N
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
+MFMA_CLASSES = ("sigma_gemm_mfma", "rdm_gram_mfma")
+
+
+def collect() -> dict:
+    L = N.lib()
+    n = L.sq_profile_nclasses()
+    ms = np.zeros(n)
+    cnt = np.zeros(n, dtype=np.int64)
+    by = np.zeros(n)
+    fl = np.zeros(n)
+    N.check(L.sq_profile_collect(n, ms.ctypes.data, cnt.ctypes.data, by.ctypes.data, fl.ctypes.data))
+    out = {}
+    for c in range(n):
+        if cnt[c] == 0:
+            continue
+        name = L.sq_profile_class_name(c).decode()
+        avg_ms = ms[c] / cnt[c]
+        rec = {
+            "launches": int(cnt[c]),
+            "total_ms": float(ms[c]),
+            "avg_us": 1e3 * float(avg_ms),
+            "bytes_per_launch": float(by[c] / cnt[c]),
+            "flops_per_launch": float(fl[c] / cnt[c]),
+        }
+        secs = ms[c] / 1e3
+        rec["achieved_GBs"] = float(by[c] / secs / 1e9) if secs > 0 else 0.0
+        rec["achieved_TFLOPs"] = float(fl[c] / secs / 1e12) if secs > 0 else 0.0
+        out[name] = rec
+    return out
+
+
+def profile(fn, reps: int = 3) -> dict:
+    """Run fn() reps times with event bracketing of every engine launch."""
+    L = N.lib()
+    torch.cuda.synchronize()
+    N.check(L.sq_profile_enable(1))
+    try:
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        kernels = collect()
+    finally:
+        N.check(L.sq_profile_enable(0))
+    for k in kernels.values():
+        k["launches_per_call"] = k["launches"] / reps
+        k["ms_per_call"] = k["total_ms"] / reps
+    return kernels
+
+
+def dominant(kernels: dict) -> dict:
+    name = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    k = kernels[name]
+    if name in MFMA_CLASSES:
+        return {"name": name, "bound": "mfma", "achieved": k["achieved_TFLOPs"], "peak": FP64_MFMA_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "avg_us": k["avg_us"], "per_launch": k["flops_per_launch"]}
+    return {"name": name, "bound": "hbm", "achieved": k["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "avg_us": k["avg_us"], "per_launch": k["bytes_per_launch"]}
+
+
+def profile_energy_grad(space, prog, H, theta, psi0, reps: int = 3, method: str = "adjoint") -> dict:
+    kernels = profile(lambda: prog.energy_grad(H, theta, psi0, method), reps)
+    return {"kernels": kernels, "dominant": dominant(kernels)}

@@ -1,0 +1,337 @@
+"""WaveFunctionUPS: drop-in for the reference's unitary product state
+(ups_wavefunction.py:38-1194) with the state-vector work on the MI355X engine.
+
+Same constructor, attributes and entry points: thetas (eager state
+preparation), ci_coeffs (host numpy, reference convention), energy_elec,
+rdm1 / rdm2, _calc_energy_optimization, _calc_gradient_optimization,
+run_wf_optimization_1step / 2step, kappa.  The theta gradient uses the
+engine's adjoint sweep by default (``gradient_method``: "adjoint", "reference"
+for the reference's product-rule sweep, or "plain").
+"""
+
+from __future__ import annotations
+
+import time
+from collections.abc import Sequence
+from functools import partial
+from typing import Any
+
+import numpy as np
+
+from .density_matrix import get_electronic_energy, get_orbital_gradient
+from .fermionic_operator import FermionicOperator
+from .operators import hamiltonian_0i_0a
+from .optimizers import Optimizers
+from .util import UpsStructure
+from .wavefunction_base import WaveFunctionBase
+
+
+class WaveFunctionUPS(WaveFunctionBase):
+    def __init__(
+        self,
+        cas: Sequence[int],
+        mo_coeffs: np.ndarray,
+        integral_generator,
+        ansatz: str,
+        ansatz_options: dict[str, Any] | None = None,
+        include_active_kappa: bool = False,
+    ) -> None:
+        if ansatz_options is None:
+            ansatz_options = {}
+        self.ansatz_options = ansatz_options
+        self._init_spaces(cas, integral_generator, include_active_kappa)
+        self.gradient_method = "adjoint"
+        self.csf_coeffs = np.zeros(self.num_det)
+        hf_det = "1" * self.num_active_elec + "0" * (self.num_active_spin_orbs - self.num_active_elec)
+        self._pp = False
+        if ansatz.lower() == "tups" and self.ansatz_options.get("do_pp", False):
+            pp_det = self._perfect_pairing_det()
+            hole = [i for i, (h, p) in enumerate(zip(hf_det, pp_det)) if h == "1" and p == "0"]
+            part = [i for i, (h, p) in enumerate(zip(hf_det, pp_det)) if h == "0" and p == "1"]
+            hole_sp = sorted({i // 2 + self.num_inactive_orbs for i in hole})
+            part_sp = sorted({i // 2 + self.num_inactive_orbs for i in part})
+            c = mo_coeffs.copy()
+            c[:, hole_sp + part_sp] = c[:, part_sp + hole_sp]
+            self._c_mo = c
+            self.csf_coeffs[self.ci_info.det2idx[int(pp_det, 2)]] = 1
+            self._pp = True
+        else:
+            self.csf_coeffs[self.ci_info.det2idx[int(hf_det, 2)]] = 1
+            self._c_mo = mo_coeffs
+        self._ci_host = np.copy(self.csf_coeffs)
+        self._psi_dev = None
+        self.ups_layout = UpsStructure()
+        name = ansatz.lower()
+        if name in ("tups", "qnp"):
+            self.ansatz_options["do_tups" if name == "tups" else "do_qnp"] = True
+            self.ups_layout.create_tiled(self.num_active_orbs, self.ansatz_options)
+        elif name in ("fucc", "fuccsd", "ksafupccgsd", "fuccpd", "safuccsd"):
+            # The reference accepts "fuccpd"/"safuccsd" here but tests "fuccspd"/
+            # "safuccspd" below (ups_wavefunction.py:273-284); kept identical.
+            if name == "fuccsd":
+                self.ansatz_options["S"] = True
+                self.ansatz_options["D"] = True
+            elif name == "ksafupccgsd":
+                self.ansatz_options["SAGS"] = True
+                self.ansatz_options["GpD"] = True
+            elif name == "fuccspd":
+                self.ansatz_options["pD"] = True
+            elif name == "safuccspd":
+                self.ansatz_options["SAS"] = True
+                self.ansatz_options["SAD"] = True
+            self.ansatz_options.setdefault("n_layers", 1)
+            self.ups_layout.create_fUCC(
+                self.active_occ_idx_shifted, self.active_unocc_idx_shifted, self.active_occ_spin_idx_shifted,
+                self.active_unocc_spin_idx_shifted, self.num_active_orbs, self.ansatz_options,
+            )
+        elif name in ("sdsfuccsd", "ksasdsfupccgsd"):
+            self.ansatz_options["D" if name == "sdsfuccsd" else "GpD"] = True
+            self.ansatz_options.setdefault("n_layers", 1)
+            self.ups_layout.create_SDSfUCC(
+                self.active_occ_idx_shifted, self.active_unocc_idx_shifted, self.active_occ_spin_idx_shifted,
+                self.active_unocc_spin_idx_shifted, self.num_active_orbs, self.ansatz_options,
+            )
+        else:
+            raise ValueError(f"Got unknown ansatz, {ansatz}")
+        self._thetas = np.zeros(self.ups_layout.n_params).tolist()
+        self._program = None
+
+    def _perfect_pairing_det(self) -> str:
+        """Perfect-pairing reference determinant (ups_wavefunction.py:218-241)."""
+        pp = ""
+        so = 0
+        ne = self.num_active_elec
+        n = self.num_active_spin_orbs
+        while so < n:
+            if ne >= 2 and (n - so) >= 4 and ne <= (n - so - 2):
+                pp += "1100"
+                ne -= 2
+                so += 4
+            elif ne == 0:
+                pp += "0"
+                so += 1
+            else:
+                pp += "1"
+                so += 1
+                ne -= 1
+        print("perfect-pairing determinant found as:", pp)
+        if len(pp) != n or pp.count("1") != self.num_active_elec:
+            raise ValueError("Perfect pairing determinant violates orbital or electron numbers")
+        return pp
+
+    # ---- device state ------------------------------------------------------
+    def _prog(self):
+        if self._program is None:
+            from . import engine
+
+            self._program = engine.Program(self._space(), self.ups_layout.excitation_operator_type,
+                                           self.ups_layout.excitation_indices)
+        return self._program
+
+    def _csf_dev(self):
+        return self._space().from_reference(self.csf_coeffs)
+
+    def _psi(self):
+        """Device state U(theta)|CSF> in the engine convention."""
+        if self._psi_dev is None:
+            sp = self._space()
+            if self._ci_host is not None:
+                self._psi_dev = sp.from_reference(self._ci_host)
+            else:
+                self._psi_dev = self._prog().state_prep(self._thetas, self._csf_dev())
+        return self._psi_dev
+
+    @property
+    def ci_coeffs(self) -> np.ndarray:
+        if self._ci_host is None:
+            self._ci_host = self._space().to_reference_numpy(self._psi())
+        return self._ci_host
+
+    @ci_coeffs.setter
+    def ci_coeffs(self, value) -> None:
+        self._ci_host = np.array(value, dtype=np.float64, copy=True)
+        self._psi_dev = None
+        self._rdm1 = None
+        self._rdm2 = None
+        self._energy_elec = None
+
+    @property
+    def thetas(self) -> list[float]:
+        return self._thetas.copy()
+
+    @thetas.setter
+    def thetas(self, theta_vals: list[float]) -> None:
+        if len(theta_vals) != len(self._thetas):
+            raise ValueError(f"Expected {len(self._thetas)} theta1 values got {len(theta_vals)}")
+        self._rdm1 = None
+        self._rdm2 = None
+        self._rdm3 = None
+        self._rdm4 = None
+        self._energy_elec = None
+        self._thetas = list(theta_vals).copy()
+        self._ci_host = None
+        self._psi_dev = self._prog().state_prep(np.asarray(self._thetas, dtype=np.float64), self._csf_dev())
+
+    # ---- properties computed on the device -----------------------------------
+    @property
+    def rdm1(self) -> np.ndarray:
+        if self._rdm1 is None:
+            from . import engine
+
+            self._rdm1, self._rdm2 = engine.rdm12(self._space(), self._psi())
+        return self._rdm1
+
+    @property
+    def rdm2(self) -> np.ndarray:
+        if self._rdm2 is None:
+            from . import engine
+
+            self._rdm1, self._rdm2 = engine.rdm12(self._space(), self._psi())
+        return self._rdm2
+
+    @property
+    def energy_elec(self) -> float:
+        if self._energy_elec is None:
+            _, self._energy_elec = self._hamiltonian().sigma(self._psi())
+        return self._energy_elec
+
+    def _get_hamiltonian(self, qiskit_form: bool = False) -> FermionicOperator:
+        H = hamiltonian_0i_0a(self.h_mo, self.g_mo, self.num_inactive_orbs, self.num_active_orbs)
+        H = H.get_folded_operator(self.num_inactive_orbs, self.num_active_orbs, self.num_virtual_orbs)
+        if qiskit_form:
+            raise NotImplementedError("qiskit form is outside the state-vector engine")
+        return H
+
+    # ---- optimisation entry points -------------------------------------------
+    def _calc_energy_optimization(self, parameters, theta_optimization: bool, kappa_optimization: bool) -> float:
+        if np.max(np.abs(np.array(self._old_opt_parameters) - np.array(parameters))) < 10**-14:
+            return self._E_opt_old
+        num_kappa = 0
+        if kappa_optimization:
+            num_kappa = len(self.kappa_idx)
+            self.kappa = list(parameters[:num_kappa])
+        if theta_optimization:
+            self.thetas = list(parameters[num_kappa:])
+        if kappa_optimization:
+            E = get_electronic_energy(self.h_mo, self.g_mo, self.num_inactive_orbs, self.num_active_orbs,
+                                      self.rdm1, self.rdm2)
+        else:
+            _, E = self._hamiltonian().sigma(self._psi())
+        self._E_opt_old = E
+        self._old_opt_parameters = np.copy(parameters)
+        self.num_energy_evals += 1
+        return E
+
+    def _calc_gradient_optimization(self, parameters, theta_optimization: bool, kappa_optimization: bool) -> np.ndarray:
+        gradient = np.zeros(len(parameters))
+        num_kappa = 0
+        if kappa_optimization:
+            num_kappa = len(self.kappa_idx)
+            self.kappa = list(parameters[:num_kappa])
+        if theta_optimization:
+            self.thetas = list(parameters[num_kappa:])
+        if kappa_optimization:
+            gradient[:num_kappa] = get_orbital_gradient(self.h_mo, self.g_mo, self.kappa_idx, self.num_inactive_orbs,
+                                                        self.num_active_orbs, self.rdm1, self.rdm2)
+        if theta_optimization:
+            _, g = self._prog().energy_grad(self._hamiltonian(), np.asarray(self._thetas, dtype=np.float64),
+                                            self._csf_dev(), self.gradient_method)
+            gradient[num_kappa:] += g
+            self.num_energy_evals += 2 * np.sum(list(self.ups_layout.grad_param_R.values()))
+        return gradient
+
+    def energy_and_gradient(self, thetas=None, method: str | None = None) -> tuple[float, np.ndarray]:
+        """E(theta) and dE/dtheta in one engine call (no host state copies)."""
+        th = self._thetas if thetas is None else list(thetas)
+        return self._prog().energy_grad(self._hamiltonian(), np.asarray(th, dtype=np.float64), self._csf_dev(),
+                                        method or self.gradient_method)
+
+    def _calc_energy_rotosolve_optimization(self, parameters, theta_diffs, theta_idx):
+        raise NotImplementedError("RotoSolve is not provided by the MI355X engine")
+
+    def run_wf_optimization_2step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
+                                  maxiter: int = 1000, is_silent_subiterations: bool = False) -> None:
+        print("### Parameters information:")
+        if orbital_optimization:
+            print(f"### Number kappa: {len(self.kappa)}")
+        print(f"### Number theta: {self.ups_layout.n_params}")
+        e_old = 1e12
+        e_new = None
+        print("Full optimization")
+        print("Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+        for full_iter in range(int(maxiter)):
+            t0 = time.time()
+            if not is_silent_subiterations:
+                print("--------Ansatz optimization")
+                print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+            opt = Optimizers(
+                partial(self._calc_energy_optimization, theta_optimization=True, kappa_optimization=False),
+                optimizer_name,
+                grad=partial(self._calc_gradient_optimization, theta_optimization=True, kappa_optimization=False),
+                maxiter=maxiter, tol=tol, is_silent=is_silent_subiterations,
+                energy_eval_callback=lambda: self.num_energy_evals,
+            )
+            self._old_opt_parameters = np.zeros_like(self.thetas) + 10**20
+            self._E_opt_old = 0.0
+            res = opt.minimize(self.thetas)
+            self.thetas = res.x.tolist()
+            if orbital_optimization and len(self.kappa) != 0:
+                if not is_silent_subiterations:
+                    print("--------Orbital optimization")
+                    print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+                opt = Optimizers(
+                    partial(self._calc_energy_optimization, theta_optimization=False, kappa_optimization=True),
+                    "l-bfgs-b",
+                    grad=partial(self._calc_gradient_optimization, theta_optimization=False, kappa_optimization=True),
+                    maxiter=maxiter, tol=tol, is_silent=is_silent_subiterations,
+                    energy_eval_callback=lambda: self.num_energy_evals,
+                )
+                self._old_opt_parameters = np.zeros(len(self.kappa_idx)) + 10**20
+                self._E_opt_old = 0.0
+                res = opt.minimize([0.0] * len(self.kappa_idx))
+                for i in range(len(self.kappa)):
+                    self._kappa[i] = 0.0
+                    self._kappa_old[i] = 0.0
+            else:
+                e_new = res.fun
+                if orbital_optimization and len(self.kappa) == 0:
+                    print("WARNING: No orbital optimization performed, because there is no non-redundant orbital parameters.")
+                break
+            e_new = res.fun
+            print(f"{str(full_iter + 1).center(11)} | {f'{time.time() - t0:7.2f}'.center(18)} | "
+                  f"{f'{e_new:3.12f}'.center(27)} | {str(self.num_energy_evals).center(11)}")
+            if abs(e_new - e_old) < tol:
+                break
+            e_old = e_new
+        self._energy_elec = e_new
+
+    def run_wf_optimization_1step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
+                                  maxiter: int = 1000) -> None:
+        print("### Parameters information:")
+        if orbital_optimization:
+            print(f"### Number kappa: {len(self.kappa)}")
+        print(f"### Number theta: {self.ups_layout.n_params}")
+        if optimizer_name.lower() == "rotosolve" and orbital_optimization and len(self.kappa) != 0:
+            raise ValueError("Cannot use RotoSolve together with orbital optimization in the one-step solver.")
+        print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+        theta_opt = not orbital_optimization or len(self.thetas) > 0
+        kappa_opt = bool(orbital_optimization)
+        energy = partial(self._calc_energy_optimization, theta_optimization=theta_opt, kappa_optimization=kappa_opt)
+        gradient = partial(self._calc_gradient_optimization, theta_optimization=theta_opt, kappa_optimization=kappa_opt)
+        if orbital_optimization:
+            parameters = self.kappa + self.thetas if len(self.thetas) > 0 else self.kappa
+        else:
+            parameters = self.thetas
+        opt = Optimizers(energy, optimizer_name, grad=gradient, maxiter=maxiter, tol=tol,
+                         energy_eval_callback=lambda: self.num_energy_evals)
+        self._old_opt_parameters = np.zeros_like(parameters) + 10**20
+        self._E_opt_old = 0.0
+        res = opt.minimize(parameters)
+        if orbital_optimization:
+            self.thetas = res.x[len(self.kappa):].tolist()
+            for i in range(len(self.kappa)):
+                self._kappa[i] = 0.0
+                self._kappa_old[i] = 0.0
+        else:
+            self.thetas = res.x.tolist()
+        self._energy_elec = res.fun

@@ -1,0 +1,170 @@
+"""Shared orbital-space bookkeeping of the UPS and UCC wave functions.
+
+Follows the constructor logic of ups_wavefunction.py:39-205 /
+ucc_wavefunction.py:36-200: active space chosen around the Fermi level,
+inactive / virtual spin-orbital lists, shifted active indices, non-redundant
+orbital-rotation (kappa) index lists, and the moving expansion point of the
+orbital rotations (c_mo = C_old expm(-kappa)).  Device-side pieces (CI space,
+folded Hamiltonian, RDMs) are created lazily so construction needs no GPU.
+"""
+
+from __future__ import annotations
+
+from collections.abc import Sequence
+
+import numpy as np
+import scipy.linalg
+
+from .ci_spaces import get_indexing
+from .integrals import IntegralManager, one_electron_integral_transform, two_electron_integral_transform
+
+
+class WaveFunctionBase:
+    def _init_spaces(self, cas: Sequence[int], integral_generator, include_active_kappa: bool) -> None:
+        if len(cas) != 2:
+            raise ValueError(f"cas must have two elements, got {len(cas)} elements.")
+        self.int_gen = IntegralManager(integral_generator)
+        self.num_orbs = len(self.int_gen.kinetic_energy)
+        self.num_spin_orbs = 2 * self.num_orbs
+        self._include_active_kappa = include_active_kappa
+        self._rdm1 = None
+        self._rdm2 = None
+        self._h_mo = None
+        self._g_mo = None
+        self._energy_elec = None
+        self.num_energy_evals = 0
+        n_el = self.int_gen.num_elec
+        # active spin orbitals: cas[0] electrons below the Fermi level, the rest above
+        active = list(range(n_el - cas[0], n_el)) + list(range(n_el, n_el + 2 * cas[1] - cas[0]))
+        act_set = set(active)
+        self.inactive_spin_idx = [i for i in range(n_el) if i not in act_set]
+        self.active_occ_spin_idx = [i for i in range(n_el) if i in act_set]
+        self.active_unocc_spin_idx = [i for i in range(n_el, self.num_spin_orbs) if i in act_set]
+        self.virtual_spin_idx = [i for i in range(n_el, self.num_spin_orbs) if i not in act_set]
+        self.active_spin_idx = self.active_occ_spin_idx + self.active_unocc_spin_idx
+        self.num_inactive_spin_orbs = len(self.inactive_spin_idx)
+        self.num_active_spin_orbs = len(self.active_spin_idx)
+        self.num_virtual_spin_orbs = len(self.virtual_spin_idx)
+        self.num_active_elec = len(self.active_occ_spin_idx)
+        if self.num_active_elec % 2 != 0:
+            raise ValueError("Number of active electrons has to be even")
+        self.num_active_elec_alpha = self.num_active_elec // 2
+        self.num_active_elec_beta = self.num_active_elec // 2
+        self.num_inactive_orbs = self.num_inactive_spin_orbs // 2
+        self.num_active_orbs = self.num_active_spin_orbs // 2
+        self.num_virtual_orbs = self.num_virtual_spin_orbs // 2
+
+        def spatial(lst):
+            out = []
+            for i in lst:
+                if i // 2 not in out:
+                    out.append(i // 2)
+            return out
+
+        self.inactive_idx = spatial(self.inactive_spin_idx)
+        self.active_idx = spatial(self.active_spin_idx)
+        self.virtual_idx = spatial(self.virtual_spin_idx)
+        self.active_occ_idx = spatial(self.active_occ_spin_idx)
+        self.active_unocc_idx = spatial(self.active_unocc_spin_idx)
+        s0 = min(self.active_spin_idx) if self.active_spin_idx else 0
+        self.active_spin_idx_shifted = [i - s0 for i in self.active_spin_idx]
+        self.active_occ_spin_idx_shifted = [i - s0 for i in self.active_occ_spin_idx]
+        self.active_unocc_spin_idx_shifted = [i - s0 for i in self.active_unocc_spin_idx]
+        p0 = min(self.active_idx) if self.active_idx else 0
+        self.active_idx_shifted = [i - p0 for i in self.active_idx]
+        self.active_occ_idx_shifted = [i - p0 for i in self.active_occ_idx]
+        self.active_unocc_idx_shifted = [i - p0 for i in self.active_unocc_idx]
+        # non-redundant orbital rotations (q > p)
+        inact, act, virt = set(self.inactive_idx), set(self.active_idx), set(self.virtual_idx)
+        self._kappa, self._kappa_old = [], []
+        kappa_idx, no_aa, no_aa_dag, redundant, hf_like = [], [], [], [], []
+        for p in range(self.num_orbs):
+            for q in range(p + 1, self.num_orbs):
+                if (p in inact and q in inact) or (p in virt and q in virt):
+                    redundant.append((p, q))
+                    continue
+                if not include_active_kappa and p in act and q in act:
+                    redundant.append((p, q))
+                    continue
+                if not (p in act and q in act):
+                    no_aa.append((p, q))
+                    no_aa_dag.append((q, p))
+                self._kappa.append(0.0)
+                self._kappa_old.append(0.0)
+                kappa_idx.append((p, q))
+        occ_a, unocc_a = set(self.active_occ_idx), set(self.active_unocc_idx)
+        for p in range(self.num_orbs):
+            for q in range(p + 1, self.num_orbs):
+                if (p in inact and (q in virt or q in unocc_a)) or (p in occ_a and q in virt):
+                    hf_like.append((p, q))
+        self.kappa_idx = np.array(kappa_idx, dtype=int)
+        self.kappa_no_activeactive_idx = np.array(no_aa, dtype=int)
+        self.kappa_no_activeactive_idx_dagger = np.array(no_aa_dag, dtype=int)
+        self.kappa_redundant_idx = np.array(redundant, dtype=int)
+        self.kappa_hf_like_idx = np.array(hf_like, dtype=int)
+        self.ci_info = get_indexing(
+            self.num_inactive_orbs, self.num_active_orbs, self.num_virtual_orbs,
+            self.num_active_elec_alpha, self.num_active_elec_beta,
+        )
+        self.num_det = len(self.ci_info.idx2det)
+        self._dev_space = None
+        self._dev_ham = None
+        self._dev_ham_key = None
+
+    # ---- orbital rotations -------------------------------------------------
+    @property
+    def kappa(self) -> list[float]:
+        return self._kappa.copy()
+
+    @kappa.setter
+    def kappa(self, k: list[float]) -> None:
+        self._h_mo = None
+        self._g_mo = None
+        self._energy_elec = None
+        self._kappa = list(k).copy()
+        self._c_mo = self.c_mo
+        self._kappa_old = self.kappa
+
+    @property
+    def c_mo(self) -> np.ndarray:
+        kappa_mat = np.zeros_like(self._c_mo)
+        if len(self.kappa) != 0:
+            diff = np.array(self.kappa) - np.array(self._kappa_old)
+            if np.max(np.abs(diff)) > 0.0:
+                for d, (p, q) in zip(diff, self.kappa_idx):
+                    kappa_mat[p, q] = d
+                    kappa_mat[q, p] = -d
+        return np.matmul(self._c_mo, scipy.linalg.expm(-kappa_mat))
+
+    @property
+    def h_mo(self) -> np.ndarray:
+        if self._h_mo is None:
+            self._h_mo = one_electron_integral_transform(self.c_mo, self.int_gen.h_ao)
+        return self._h_mo
+
+    @property
+    def g_mo(self) -> np.ndarray:
+        if self._g_mo is None:
+            self._g_mo = two_electron_integral_transform(self.c_mo, self.int_gen.electron_electron_repulsion)
+        return self._g_mo
+
+    # ---- device pieces -----------------------------------------------------
+    def _space(self):
+        if self._dev_space is None:
+            from . import engine
+
+            self._dev_space = engine.space_for(self.num_active_orbs, self.num_active_elec_alpha, self.num_active_elec_beta)
+        return self._dev_space
+
+    def _hamiltonian(self):
+        """Folded active-space Hamiltonian on the device, rebuilt when h_mo/g_mo change."""
+        from . import engine
+        from .operators import folded_integrals
+
+        h, g = self.h_mo, self.g_mo
+        key = (id(h), id(g))
+        if self._dev_ham is None or self._dev_ham_key != key:
+            e_core, h_eff, g_act = folded_integrals(h, g, self.num_inactive_orbs, self.num_active_orbs)
+            self._dev_ham = engine.Hamiltonian(self._space(), h_eff, g_act, e_core)
+            self._dev_ham_key = key
+        return self._dev_ham
