@@ -145,6 +145,43 @@ int sq_energy_grad_reference_sweep(const sq_space* s, const sq_program* p, const
  * rdm1: host n*n, rdm2: host n^4 (p,q,r,s) = <E_pq E_rs> - delta_qr rdm1_ps. */
 int sq_rdm12(const sq_space* s, const double* psi, double* rdm1, double* rdm2, double* work, void* stream);
 
+/* ---- Unfactorised UCC (construct_ucc_state, operator_state_algebra.py:870-960;
+ * ucc_wavefunction.py:966-1098).  The program holds the UCC excitation list;
+ * its generators T_k (G1_sa for "sa_single_ucc", G2_sa, G1..G6 anti-Hermitian)
+ * form T(theta) = sum_k theta_k T_k.  work: sq_ucc_work_bytes(). */
+int64_t sq_ucc_work_bytes(const sq_space* s, const sq_program* p);
+/* out = exp(t T(theta)) in, matrix-free scaled Taylor series (in != out). */
+int sq_ucc_expm(const sq_space* s, const sq_program* p, const double* theta, double t, const double* in,
+                double* out, void* work, void* stream);
+/* energy = <psi|H|psi>, psi = exp(T(theta)) psi0, and (if grad != NULL) the
+ * analytic dE/dtheta from the block-triangular exponential. */
+int sq_ucc_energy_grad(const sq_space* s, const sq_program* p, const sq_hamiltonian* H, const double* theta,
+                       const double* psi0, double* energy, double* grad, void* work, void* stream);
+
+/* ---- Batched instances (the reference batches equal-dimension states
+ * column-wise in its _SA family: apply_operator_SA_threaded,
+ * operator_state_algebra.py:352-410; construct_ups_state_SA, :1415-1866;
+ * propagate_unitary_SA, :2312-2756; expectation_value_SA, :827-868).
+ * Here each instance has its own space, program and Hamiltonian; one
+ * persistent launch computes every instance's energy and adjoint gradient
+ * from the HF determinant.  Programs may hold rotation gates only (single,
+ * double, sa_single, pair doubles: tUPS / QNP / fUCC-SD layouts); spaces up
+ * to 12 orbitals.  n_workgroups <= 0 picks 256. */
+typedef struct sq_batch sq_batch;
+int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program* const* programs,
+                    const sq_hamiltonian* const* hams, int n_workgroups, sq_batch** out);
+int sq_batch_destroy(sq_batch* b);
+/* Total number of parameters (sum of the programs' gate counts). */
+int64_t sq_batch_num_params(const sq_batch* b);
+/* theta: host array, instances' parameters concatenated in instance order. */
+int sq_batch_set_theta(sq_batch* b, const double* theta, void* stream);
+/* One launch on stream. */
+int sq_batch_run(sq_batch* b, void* stream);
+/* Same launch captured once into a hipGraph, then replayed. */
+int sq_batch_run_graph(sq_batch* b, void* stream);
+/* Host copies: energies[n_inst] (electronic + core), grads[num_params]. */
+int sq_batch_results(const sq_batch* b, double* energies, double* grads, void* stream);
+
 /* ---- Generic operators: propagate_state for FermionicOperator lists ------
  * (operator_state_algebra.py:472-630, scan kernels :139-219).  A string is a
  * raw product of creation/annihilation operators, ops[0] leftmost, with

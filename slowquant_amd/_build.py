@@ -17,6 +17,7 @@ SOURCES = [
     "sq_grad.hip",
     "sq_profile.hip",
     "sq_tiles.hip",
+    "sq_ucc.hip",
     "sq_batch.hip",
 ]
 ARCH = os.environ.get("SQ_OFFLOAD_ARCH", "gfx950")

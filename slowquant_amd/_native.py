@@ -88,6 +88,16 @@ _SIGNATURES = {
     "sq_operator_create": ([vp, ctypes.POINTER(OpString), i32, i32, ctypes.POINTER(vp)], i32),
     "sq_operator_destroy": ([vp], i32),
     "sq_operator_apply": ([vp, vp, dp, dp, vp], i32),
+    "sq_ucc_work_bytes": ([vp, vp], i64),
+    "sq_ucc_expm": ([vp, vp, dp, ctypes.c_double, dp, dp, dp, vp], i32),
+    "sq_ucc_energy_grad": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
+    "sq_batch_create": ([i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), i32, ctypes.POINTER(vp)], i32),
+    "sq_batch_destroy": ([vp], i32),
+    "sq_batch_num_params": ([vp], i64),
+    "sq_batch_set_theta": ([vp, vp, vp], i32),
+    "sq_batch_run": ([vp, vp], i32),
+    "sq_batch_run_graph": ([vp, vp], i32),
+    "sq_batch_results": ([vp, vp, vp, vp], i32),
     "sq_profile_enable": ([i32], i32),
     "sq_profile_nclasses": ([], i32),
     "sq_profile_class_name": ([i32], ctypes.c_char_p),

@@ -1,0 +1,474 @@
+// Batched energy + adjoint gradient for many independent UPS instances of
+// uneven active spaces in ONE persistent launch (the reference's batching
+// primitive is the _SA family, operator_state_algebra.py:282-410,633-781,
+// which needs equal dimensions; here every instance has its own space,
+// Hamiltonian, gate list and theta).
+//
+// One workgroup owns a list of instances (longest-processing-time order,
+// assigned on the host, no atomics).  Per instance, inside the workgroup:
+//   psi = HF;  forward rotations;  D build;  G = A.D with FP64 MFMA;
+//   sigma gather + energy;  reverse sweep with fused <lambda|T_i psi> dots.
+// State vectors and D/G live in a per-workgroup slot of global scratch
+// (L2-resident for the small spaces), so instances never interact.
+// The launch can be captured in a hipGraph and replayed.
+
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <numeric>
+
+#include "sq_internal.h"
+
+namespace sq {
+
+constexpr int kBT = 512;  // threads per batch workgroup (8 waves)
+
+struct BatchSpace {
+  int n, nsa, nsb, npair, K4, kpad;
+  const uint32_t* astr;
+  const uint32_t* bstr;
+  const int32_t* arank;
+  const int32_t* brank;
+  const int32_t* aexc;  // packed, k-major
+  const int32_t* bexc;
+};
+
+struct BatchInstance {
+  int space;        // index into spaces
+  int gate0, ngate; // rotation records [gate0, gate0 + ngate)
+  int theta0;       // offset into theta array (one theta per layout gate)
+  int nparam;
+  int A_off;        // offset into A storage (K4 * kpad doubles)
+  double e_core;
+  int out0;         // offset of the gradient in the output array
+};
+
+// One rotation half: string G = sgn * Ga * Gb with distinct indices; the
+// layout gate's theta index and angle scale.
+struct BatchRot {
+  SpinOps a, b;
+  double sgn, scale;
+  int theta_idx;    // index relative to the instance's theta0
+  int grad_idx;     // same, for the gradient accumulation
+};
+
+struct BatchDev {
+  const BatchSpace* spaces;
+  const BatchInstance* inst;
+  const BatchRot* rots;
+  const double* theta;
+  const double* A;
+  const int32_t* wg_list;  // concatenated instance lists per workgroup
+  const int32_t* wg_off;   // [nwg + 1]
+  double* scratch;         // per-workgroup slots
+  int64_t slot_doubles;
+  int64_t dmax;
+  double* energy;          // [ninst]
+  double* grad;            // concatenated gradients
+};
+
+__device__ __forceinline__ bool bapply(const SpinOps& o, uint32_t& m, int& par) {
+  for (int j = o.n - 1; j >= 0; --j) {
+    uint32_t bit = 1u << o.orb[j];
+    par += __popc(m & (bit - 1u));
+    if (o.cre[j]) {
+      if (m & bit) return false;
+      m |= bit;
+    } else {
+      if (!(m & bit)) return false;
+      m &= ~bit;
+    }
+  }
+  return true;
+}
+
+__device__ double block_reduce(double v, double* red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wave] = v;
+  __syncthreads();
+  double t = 0.0;
+  for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w];
+  return t;  // every thread gets the sum
+}
+
+__global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
+  __shared__ double red[kBT / 64];
+  extern __shared__ double sA[];
+  double* slot = B.scratch + (size_t)blockIdx.x * B.slot_doubles;
+  double* psi = slot;
+  double* lam = psi + B.dmax;
+  double* D = lam + B.dmax;
+  for (int li = B.wg_off[blockIdx.x]; li < B.wg_off[blockIdx.x + 1]; ++li) {
+    const BatchInstance I = B.inst[B.wg_list[li]];
+    const BatchSpace S = B.spaces[I.space];
+    const int nsb = S.nsb;
+    const int64_t dim = (int64_t)S.nsa * nsb;
+    double* G = D + (size_t)S.K4 * dim;
+    const double* th = B.theta + I.theta0;
+    // psi = HF determinant (alpha string 0, beta string 0) in the engine's
+    // sign convention: (-1)^{sum_{p in alpha} #{q in beta : q < p}}.
+    for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) psi[x] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t a = S.astr[0], b = S.bstr[0];
+      int par = 0;
+      for (uint32_t aa = a; aa; aa &= aa - 1) par += __popc(b & ((1u << (__ffs(aa) - 1)) - 1u));
+      psi[0] = (par & 1) ? -1.0 : 1.0;
+    }
+    __syncthreads();
+    // forward sweep
+    for (int r = 0; r < I.ngate; ++r) {
+      const BatchRot& R = B.rots[I.gate0 + r];
+      const double t0 = th[R.theta_idx];
+      if (fabs(t0) >= 1e-28) {
+        double s, c;
+        sincos(R.scale * t0, &s, &c);
+        for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
+          const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
+          uint32_t a = S.astr[ia], b = S.bstr[ib];
+          int par = 0;
+          if (!bapply(R.a, a, par) || !bapply(R.b, b, par)) continue;
+          const double sg = ((par & 1) ? -1.0 : 1.0) * R.sgn;
+          const size_t J = (size_t)S.arank[a] * nsb + S.brank[b];
+          const double pd = psi[x], pj = psi[J];
+          psi[x] = c * pd - sg * s * pj;
+          psi[J] = sg * s * pd + c * pj;
+        }
+      }
+      __syncthreads();
+    }
+    // D build (rows k < npair: E+_k psi; row npair: psi; pad rows: 0)
+    for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
+      const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
+      const double px = psi[x];
+      int k = 0;
+      for (int p = 0; p < S.n; ++p) {
+        for (int q = 0; q <= p; ++q, ++k) {
+          const int32_t ea = S.aexc[(size_t)k * S.nsa + ia];
+          const int32_t eb = S.bexc[(size_t)k * nsb + ib];
+          double v;
+          if (p == q) {
+            v = (double)(ea + eb) * px;
+          } else {
+            v = 0.0;
+            if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
+            if (eb) v += (eb > 0 ? 1.0 : -1.0) * psi[(size_t)ia * nsb + abs(eb) - 1];
+          }
+          D[(size_t)k * dim + x] = v;
+        }
+      }
+      D[(size_t)S.npair * dim + x] = px;
+      for (int k2 = S.npair + 1; k2 < S.K4; ++k2) D[(size_t)k2 * dim + x] = 0.0;
+    }
+    // stage A (K4 x kpad, transposed coefficient matrix) in LDS
+    const double* Ag = B.A + I.A_off;
+    for (int i = threadIdx.x; i < S.K4 * S.kpad; i += blockDim.x) sA[i] = Ag[i];
+    __syncthreads();
+    // G = A . D with v_mfma_f64_16x16x4f64; each wave walks 16-column tiles
+    {
+      typedef double double4_t __attribute__((ext_vector_type(4)));
+      const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, li = lane & 15;
+      const int nwaves = blockDim.x >> 6;
+      const int MT = S.kpad / 16;
+      for (int64_t c0 = (int64_t)wave * 16; c0 < dim; c0 += (int64_t)nwaves * 16) {
+        const int64_t c = c0 + li;
+        const bool ok = c < dim;
+        for (int mt = 0; mt < MT; ++mt) {
+          double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
+          for (int k4 = 0; k4 < S.K4; k4 += 4) {
+            const double b = ok ? D[(size_t)(k4 + kq) * dim + c] : 0.0;
+            const double a = sA[(size_t)(k4 + kq) * S.kpad + mt * 16 + li];
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+          }
+          if (ok) {
+            for (int r = 0; r < 4; ++r) {
+              const int m = mt * 16 + kq + 4 * r;
+              if (m < S.npair) G[(size_t)m * dim + c] = acc[r];
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // sigma gather into lam, energy
+    double e = 0.0;
+    for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
+      const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
+      double v = I.e_core * psi[x];
+      int k = 0;
+      for (int p = 0; p < S.n; ++p) {
+        for (int q = 0; q <= p; ++q, ++k) {
+          const double* Gk = G + (size_t)k * dim;
+          const int32_t ea = S.aexc[(size_t)k * S.nsa + ia];
+          const int32_t eb = S.bexc[(size_t)k * nsb + ib];
+          if (p == q) {
+            v += (double)(ea + eb) * Gk[x];
+          } else {
+            if (ea) v += (ea > 0 ? 1.0 : -1.0) * Gk[(size_t)(abs(ea) - 1) * nsb + ib];
+            if (eb) v += (eb > 0 ? 1.0 : -1.0) * Gk[(size_t)ia * nsb + abs(eb) - 1];
+          }
+        }
+      }
+      lam[x] = v;
+      e += v * psi[x];
+    }
+    const double energy = block_reduce(e, red);
+    if (threadIdx.x == 0) B.energy[B.wg_list[li]] = energy;
+    // reverse sweep with gradient dots
+    for (int p = threadIdx.x; p < I.nparam; p += blockDim.x) B.grad[I.out0 + p] = 0.0;
+    __syncthreads();
+    for (int r = I.ngate - 1; r >= 0; --r) {
+      const BatchRot& R = B.rots[I.gate0 + r];
+      const double t0 = th[R.theta_idx];
+      const bool skip = fabs(t0) < 1e-28;
+      double s, c;
+      sincos(-R.scale * t0, &s, &c);
+      double dot = 0.0;
+      for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
+        const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
+        uint32_t a = S.astr[ia], b = S.bstr[ib];
+        int par = 0;
+        if (!bapply(R.a, a, par) || !bapply(R.b, b, par)) continue;
+        const double sg = ((par & 1) ? -1.0 : 1.0) * R.sgn;
+        const size_t J = (size_t)S.arank[a] * nsb + S.brank[b];
+        const double pd = psi[x], pj = psi[J], ld = lam[x], lj = lam[J];
+        dot += R.scale * sg * (lj * pd - ld * pj);
+        if (!skip) {
+          psi[x] = c * pd - sg * s * pj;
+          psi[J] = sg * s * pd + c * pj;
+          lam[x] = c * ld - sg * s * lj;
+          lam[J] = sg * s * ld + c * lj;
+        }
+      }
+      const double tot = block_reduce(dot, red);
+      if (threadIdx.x == 0) B.grad[I.out0 + R.grad_idx] += 2.0 * tot;
+      __syncthreads();
+    }
+  }
+}
+
+struct Batch {
+  std::vector<BatchInstance> inst;
+  int nwg = 0;
+  int64_t dmax = 0, slot = 0, ngrad = 0;
+  double bytes = 0.0, flops = 0.0;  // algorithmic totals of one launch
+  size_t lds = 0;
+  std::vector<void*> allocs;
+  BatchDev dev{};
+  double* d_theta = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<int32_t> grad_off;
+  ~Batch() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (void* p : allocs) (void)hipFree(p);
+  }
+};
+
+template <class T>
+static T* bupload(Batch& B, const std::vector<T>& v) {
+  T* d = nullptr;
+  SQ_HIP(hipMalloc(&d, sizeof(T) * std::max<size_t>(v.size(), 1)));
+  if (!v.empty()) SQ_HIP(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  B.allocs.push_back(d);
+  return d;
+}
+
+}  // namespace sq
+
+using namespace sq;
+
+struct sq_batch {
+  Batch b;
+};
+
+extern "C" {
+
+int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program* const* programs,
+                    const sq_hamiltonian* const* hams, int n_workgroups, sq_batch** out) {
+  return guarded([&] {
+    SQ_CHECK(n_inst > 0 && spaces && programs && hams && out, SQ_ERR_ARG, "invalid arguments");
+    auto* bo = new sq_batch();
+    Batch& B = bo->b;
+    try {
+      std::map<const sq_space*, int> sp_index;
+      std::vector<BatchSpace> bsp;
+      std::vector<BatchRot> rots;
+      std::vector<double> Astore;
+      std::vector<int64_t> cost(n_inst);
+      int theta_off = 0, grad_off = 0;
+      for (int i = 0; i < n_inst; ++i) {
+        const Space& s = spaces[i]->s;
+        const Program& p = programs[i]->p;
+        const Hamiltonian& H = hams[i]->h;
+        SQ_CHECK(p.space == &s && H.space == &s, SQ_ERR_ARG, "program / Hamiltonian built for another space");
+        if (!sp_index.count(spaces[i])) {
+          sp_index[spaces[i]] = (int)bsp.size();
+          BatchSpace b{};
+          b.n = s.n;
+          b.nsa = s.nsa;
+          b.nsb = s.nsb;
+          b.npair = s.npair;
+          b.K4 = (s.npair + 1 + 3) / 4 * 4;
+          b.kpad = (s.npair + 1 + 15) / 16 * 16;
+          b.astr = s.d_astr;
+          b.bstr = s.d_bstr;
+          b.arank = s.d_arank;
+          b.brank = s.d_brank;
+          b.aexc = s.d_aexc;
+          b.bexc = s.d_bexc;
+          bsp.push_back(b);
+        }
+        const BatchSpace& bs = bsp[sp_index[spaces[i]]];
+        BatchInstance I{};
+        I.space = sp_index[spaces[i]];
+        I.gate0 = (int)rots.size();
+        for (int g = 0; g < (int)p.gates.size(); ++g) {
+          const GateDev& gd = p.gates[g];
+          SQ_CHECK(gd.kind == GC_ROT || gd.kind == GC_SA_SINGLE, SQ_ERR_ARG,
+                   "batched launches support rotation gates (single, double, sa_single, pair doubles) only");
+          const int halves = gd.kind == GC_SA_SINGLE ? 2 : 1;
+          for (int h = 0; h < halves; ++h) {
+            const Term& T = p.terms[gd.term_begin + 2 * h];
+            BatchRot R{};
+            R.a = T.a;
+            R.b = T.b;
+            R.sgn = T.coef < 0 ? -1.0 : 1.0;
+            R.scale = halves == 2 ? 1.0 : gd.scale;
+            R.theta_idx = g;
+            R.grad_idx = g;
+            rots.push_back(R);
+          }
+        }
+        I.ngate = (int)rots.size() - I.gate0;
+        I.theta0 = theta_off;
+        I.nparam = (int)p.gates.size();
+        theta_off += I.nparam;
+        I.out0 = grad_off;
+        B.grad_off.push_back(grad_off);
+        grad_off += I.nparam;
+        // A copied from the device Hamiltonian (K4 x kpad, same packing)
+        I.A_off = (int)Astore.size();
+        std::vector<double> A((size_t)bs.K4 * bs.kpad);
+        SQ_HIP(hipMemcpy(A.data(), H.d_A, sizeof(double) * A.size(), hipMemcpyDeviceToHost));
+        Astore.insert(Astore.end(), A.begin(), A.end());
+        I.e_core = H.e_core;
+        B.inst.push_back(I);
+        B.dmax = std::max<int64_t>(B.dmax, s.dim);
+        B.slot = std::max<int64_t>(B.slot, (int64_t)(2 + bs.K4 + bs.npair) * s.dim);
+        B.lds = std::max<size_t>(B.lds, sizeof(double) * bs.K4 * bs.kpad);
+        cost[i] = s.dim * (int64_t)(2 * I.ngate + 4 + 2 * bs.npair + (bs.npair * bs.K4) / 8);
+      }
+      SQ_CHECK(B.lds <= 65536, SQ_ERR_ARG, "batched spaces limited to 12 orbitals (LDS)");
+      B.ngrad = grad_off;
+      // longest processing time first, each to the least-loaded workgroup
+      int nwg = n_workgroups > 0 ? n_workgroups : 256;
+      nwg = std::min(nwg, n_inst);
+      std::vector<int> order(n_inst);
+      std::iota(order.begin(), order.end(), 0);
+      std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+      std::vector<std::vector<int32_t>> lists(nwg);
+      std::vector<int64_t> load(nwg, 0);
+      for (int i : order) {
+        int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[w].push_back(i);
+        load[w] += cost[i];
+      }
+      std::vector<int32_t> wl, wo{0};
+      for (auto& l : lists) {
+        wl.insert(wl.end(), l.begin(), l.end());
+        wo.push_back((int32_t)wl.size());
+      }
+      B.nwg = nwg;
+      B.dev.spaces = bupload(B, bsp);
+      B.dev.inst = bupload(B, B.inst);
+      B.dev.rots = bupload(B, rots);
+      B.dev.A = bupload(B, Astore);
+      B.dev.wg_list = bupload(B, wl);
+      B.dev.wg_off = bupload(B, wo);
+      B.dev.slot_doubles = B.slot;
+      B.dev.dmax = B.dmax;
+      double* scratch = nullptr;
+      SQ_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)B.slot * nwg));
+      B.allocs.push_back(scratch);
+      B.dev.scratch = scratch;
+      double* en = nullptr;
+      SQ_HIP(hipMalloc(&en, sizeof(double) * n_inst));
+      B.allocs.push_back(en);
+      B.dev.energy = en;
+      double* gr = nullptr;
+      SQ_HIP(hipMalloc(&gr, sizeof(double) * std::max<int64_t>(grad_off, 1)));
+      B.allocs.push_back(gr);
+      B.dev.grad = gr;
+      SQ_HIP(hipMalloc(&B.d_theta, sizeof(double) * std::max(theta_off, 1)));
+      B.allocs.push_back(B.d_theta);
+      B.dev.theta = B.d_theta;
+    } catch (...) {
+      delete bo;
+      throw;
+    }
+    *out = bo;
+  });
+}
+
+int sq_batch_destroy(sq_batch* b) {
+  return guarded([&] { delete b; });
+}
+
+int64_t sq_batch_num_params(const sq_batch* b) { return b ? b->b.ngrad : -1; }
+
+/* theta: host array of all instances' parameters (concatenated in instance
+ * order); copied into the batch's device buffer. */
+int sq_batch_set_theta(sq_batch* bo, const double* theta, void* stream) {
+  return guarded([&] {
+    SQ_CHECK(bo && theta, SQ_ERR_ARG, "NULL argument");
+    SQ_HIP(hipMemcpyAsync(bo->b.d_theta, theta, sizeof(double) * bo->b.ngrad, hipMemcpyHostToDevice,
+                          as_stream(stream)));
+  });
+}
+
+static void batch_launch(Batch& B, hipStream_t st) {
+  SQ_PROF(PC_BATCH, st, B.bytes, B.flops, k_batch<<<B.nwg, kBT, B.lds, st>>>(B.dev));
+}
+
+int sq_batch_run(sq_batch* bo, void* stream) {
+  return guarded([&] { batch_launch(bo->b, as_stream(stream)); });
+}
+
+/* Capture the launch in a hipGraph (once) and replay it. */
+int sq_batch_run_graph(sq_batch* bo, void* stream) {
+  return guarded([&] {
+    Batch& B = bo->b;
+    hipStream_t st = as_stream(stream);
+    if (!B.exec) {
+      hipStream_t cap;
+      SQ_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      SQ_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+      k_batch<<<B.nwg, kBT, B.lds, cap>>>(B.dev);
+      hipError_t e = hipGetLastError();
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(cap, &g);
+      (void)hipStreamDestroy(cap);
+      SQ_CHECK(e == hipSuccess && e2 == hipSuccess, SQ_ERR_HIP, "graph capture failed");
+      B.graph = g;
+      SQ_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));
+    }
+    SQ_HIP(hipGraphLaunch(B.exec, st));
+  });
+}
+
+int sq_batch_results(const sq_batch* bo, double* energies, double* grads, void* stream) {
+  return guarded([&] {
+    SQ_CHECK(bo && energies && grads, SQ_ERR_ARG, "NULL argument");
+    const Batch& B = bo->b;
+    hipStream_t st = as_stream(stream);
+    SQ_HIP(hipMemcpyAsync(energies, B.dev.energy, sizeof(double) * B.inst.size(), hipMemcpyDeviceToHost, st));
+    SQ_HIP(hipMemcpyAsync(grads, B.dev.grad, sizeof(double) * B.ngrad, hipMemcpyDeviceToHost, st));
+    SQ_HIP(hipStreamSynchronize(st));
+  });
+}
+
+}  // extern "C"

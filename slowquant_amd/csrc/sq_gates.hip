@@ -673,6 +673,7 @@ int sq_program_destroy(sq_program* pr) {
     if (!pr) return;
     if (pr->p.d_gates) (void)hipFree(pr->p.d_gates);
     if (pr->p.d_terms) (void)hipFree(pr->p.d_terms);
+    if (pr->p.tiles) destroy_tile_plan(pr->p.tiles);
     delete pr;
   });
 }
@@ -741,6 +742,11 @@ int sq_state_prep(const sq_space* sp, const sq_program* pr, const double* theta,
     SQ_CHECK(theta || p.gates.empty(), SQ_ERR_ARG, "theta is NULL");
     hipStream_t st = as_stream(stream);
     copy_vec(psi_in, psi_out, s.dim, st);
+    const TilePlan* T = tiles_for(s, p);
+    if (T) {
+      tiles_forward(s, p, *T, theta, psi_out, work + 2 * s.dim, work, -1, dagger != 0, st);
+      return;
+    }
     int n = (int)p.gates.size();
     for (int k = 0; k < n; ++k) {
       int g = dagger ? n - 1 - k : k;

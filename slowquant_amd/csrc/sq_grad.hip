@@ -14,13 +14,13 @@
 
 namespace sq {
 
-// Gradient workspace (doubles): psi | lam | t1 | t2 | partials | scalars(64+N) | sigma work
+// Gradient workspace (doubles): psi | lam | t1 | t2 | s1 | s2 | partials | scalars(64) | grad(N) | sigma work
 static int64_t grad_doubles(const Space& s, int n_gates) {
-  return 4 * s.dim + nblocks2d(s) + 1024 + 64 + n_gates + sigma_work_bytes(s) / 8;
+  return 6 * s.dim + nblocks2d(s) + 1024 + 64 + n_gates + sigma_work_bytes(s) / 8;
 }
 
 struct GradBufs {
-  double *psi, *lam, *t1, *t2, *partial, *scal, *grad, *sigw;
+  double *psi, *lam, *t1, *t2, *s1, *s2, *partial, *scal, *grad, *sigw;
 };
 
 static GradBufs carve(const Space& s, int n_gates, double* work) {
@@ -29,7 +29,9 @@ static GradBufs carve(const Space& s, int n_gates, double* work) {
   b.lam = b.psi + s.dim;
   b.t1 = b.lam + s.dim;
   b.t2 = b.t1 + s.dim;
-  b.partial = b.t2 + s.dim;
+  b.s1 = b.t2 + s.dim;
+  b.s2 = b.s1 + s.dim;
+  b.partial = b.s2 + s.dim;
   b.scal = b.partial + nblocks2d(s) + 1024;
   b.grad = b.scal + 64;
   b.sigw = b.grad + n_gates;
@@ -42,19 +44,37 @@ static void finish(const GradBufs& b, int n, double* energy, double* grad, hipSt
   SQ_HIP(hipStreamSynchronize(st));
 }
 
+// psi = U psi0 (t1, t2: per-gate work; s1: tile scratch).  deriv_gate >= 0
+// inserts T_g after gate g.
 static void forward(const Space& s, const Program& p, const double* theta, const double* psi0, double* psi,
-                    double* t1, hipStream_t st) {
+                    const GradBufs& b, int deriv_gate, hipStream_t st) {
   copy_vec(psi0, psi, s.dim, st);
-  for (int g = 0; g < (int)p.gates.size(); ++g) apply_gate_impl(s, p, g, theta, false, psi, t1, st);
+  const TilePlan* T = tiles_for(s, p);
+  if (T) {
+    tiles_forward(s, p, *T, theta, psi, b.s1, b.t1, deriv_gate, false, st);
+    return;
+  }
+  for (int g = 0; g < (int)p.gates.size(); ++g) {
+    apply_gate_impl(s, p, g, theta, false, psi, b.t1, st);
+    if (g == deriv_gate) {
+      grad_action_impl(s, p, g, psi, b.t2, st);
+      copy_vec(b.t2, psi, s.dim, st);
+    }
+  }
 }
 
 void energy_grad_adjoint(const Space& s, const Program& p, const Hamiltonian& H, const double* theta,
                          const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
   const int n = (int)p.gates.size();
   GradBufs b = carve(s, n, work);
-  forward(s, p, theta, psi0, b.psi, b.t1, st);
+  forward(s, p, theta, psi0, b.psi, b, -1, st);
   sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
-  for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, b.psi, b.lam, b.grad, b.t1, b.partial, st);
+  const TilePlan* T = tiles_for(s, p);
+  if (T) {
+    tiles_reverse(s, p, *T, theta, b.psi, b.lam, b.s1, b.s2, b.t1, b.partial, b.grad, st);
+  } else {
+    for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, b.psi, b.lam, b.grad, b.t1, b.partial, st);
+  }
   finish(b, n, energy, grad, st);
 }
 
@@ -62,19 +82,12 @@ void energy_grad_plain(const Space& s, const Program& p, const Hamiltonian& H, c
                        const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
   const int n = (int)p.gates.size();
   GradBufs b = carve(s, n, work);
-  forward(s, p, theta, psi0, b.psi, b.t1, st);
+  forward(s, p, theta, psi0, b.psi, b, -1, st);
   sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
-  // b.psi is free now; use it as the derivative vector, t2 for T_i products.
+  // b.psi is free now: it holds each derivative vector d_i psi in turn.
   double* v = b.psi;
   for (int i = 0; i < n; ++i) {
-    copy_vec(psi0, v, s.dim, st);
-    for (int k = 0; k < n; ++k) {
-      apply_gate_impl(s, p, k, theta, false, v, b.t1, st);
-      if (k == i) {
-        grad_action_impl(s, p, k, v, b.t2, st);
-        copy_vec(b.t2, v, s.dim, st);
-      }
-    }
+    forward(s, p, theta, psi0, v, b, i, st);
     dot_dev(b.lam, v, s.dim, b.grad, i, 2.0, b.partial, st);
   }
   finish(b, n, energy, grad, st);
@@ -84,10 +97,15 @@ void energy_grad_reference_sweep(const Space& s, const Program& p, const Hamilto
                                  const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
   const int n = (int)p.gates.size();
   GradBufs b = carve(s, n, work);
-  forward(s, p, theta, psi0, b.psi, b.t1, st);
+  forward(s, p, theta, psi0, b.psi, b, -1, st);
   sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
   // bra = U^+ H psi
-  for (int g = n - 1; g >= 0; --g) apply_gate_impl(s, p, g, theta, true, b.lam, b.t1, st);
+  const TilePlan* T = tiles_for(s, p);
+  if (T) {
+    tiles_forward(s, p, *T, theta, b.lam, b.s1, b.t1, -1, true, st);
+  } else {
+    for (int g = n - 1; g >= 0; --g) apply_gate_impl(s, p, g, theta, true, b.lam, b.t1, st);
+  }
   double* ket = b.psi;
   copy_vec(psi0, ket, s.dim, st);
   for (int i = 0; i < n; ++i) {
@@ -110,7 +128,7 @@ int64_t sq_work_bytes(const sq_space* sp, int what) {
   const Space& s = sp->s;
   switch (what) {
     case 0:  // state preparation / single gates
-      return 8 * (2 * s.dim + 64);
+      return 8 * (3 * s.dim + 64);
     case 1:  // sigma
       return sigma_work_bytes(s);
     case 2:  // gradients, for programs of up to 65536 gates

@@ -100,8 +100,15 @@ struct Space {
   std::vector<uint32_t> h_astr, h_bstr;
 };
 
+struct TilePlan;
+TilePlan* build_tile_plan(const Space& sp, const struct Program& P);
+void destroy_tile_plan(TilePlan* T);
+bool tile_plan_has_tiles(const TilePlan* T);
+
 struct Program {
   const Space* space;
+  mutable TilePlan* tiles = nullptr;  // built on first use (sq_tiles.hip)
+  mutable bool tiles_built = false;
   std::vector<GateDev> gates;
   std::vector<Term> terms;
   GateDev* d_gates = nullptr;
@@ -131,6 +138,7 @@ enum ProfClass {
   PC_TILE_REV,
   PC_PERM,
   PC_RDM_GRAM,
+  PC_BATCH,
   PC_OTHER,
   kProfClasses
 };
@@ -171,6 +179,13 @@ void reduce_impl(const double* partial, int64_t n, double* out_dev, int slot, bo
                  hipStream_t st);
 void reverse_step_impl(const Space& s, const Program& p, int g, const double* theta_dev, double* psi, double* lam,
                        double* grad_dev, double* work, double* partial, hipStream_t st);
+
+// Implemented in sq_tiles.hip
+const TilePlan* tiles_for(const Space& s, const Program& p);  // nullptr when no tiled stage applies
+void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi,
+                   double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st);
+void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi, double* lam,
+                   double* sp_, double* sl_, double* gwork, double* partial, double* grad_dev, hipStream_t st);
 
 }  // namespace sq
 

@@ -13,6 +13,7 @@
 // Kernel 3 (bandwidth): sigma gather + <psi|sigma> partials.
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "sq_internal.h"
@@ -170,6 +171,107 @@ __global__ __launch_bounds__(512) void k_gemm_mfma(const double* __restrict__ At
   }
 }
 
+// Kernel 2' (fused): the same contraction with the B operand D[k][c]
+// generated in registers from psi by the single-excitation gathers of
+// k_dbuild, so D never touches memory.  Lane l computes D[k4 + l/16][c] for
+// its column c; row npair is psi itself, rows above it are zero.
+__constant__ uint8_t c_kp[160], c_kq[160];
+
+template <int MT>
+__global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ At, const double* __restrict__ psi,
+                                                    const int32_t* __restrict__ aexc, const int32_t* __restrict__ bexc,
+                                                    double* __restrict__ G, int npair, int K4, int kpad, int nsa,
+                                                    int nsb) {
+  extern __shared__ double sA[];
+  constexpr int W = MT * 16;
+  const int m0 = blockIdx.y * W;
+  for (int i = threadIdx.x; i < K4 * W; i += blockDim.x) {
+    int k = i / W, m = m0 + i % W;
+    sA[i] = m < kpad ? At[(size_t)k * kpad + m] : 0.0;
+  }
+  __syncthreads();
+  const int64_t dim = (int64_t)nsa * nsb;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t c = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 16 + (lane & 15);
+  const int kq = lane >> 4;
+  const bool valid = c < dim;
+  const int ia = valid ? (int)(c / nsb) : 0;
+  const int ib = valid ? (int)(c - (int64_t)ia * nsb) : 0;
+  const double* row = psi + (size_t)ia * nsb;
+  const double pc = valid ? psi[c] : 0.0;
+  auto dval = [&](int k) -> double {
+    if (!valid || k > npair) return 0.0;
+    if (k == npair) return pc;
+    const int32_t ea = aexc[(size_t)k * nsa + ia];
+    const int32_t eb = bexc[(size_t)k * nsb + ib];
+    if (c_kp[k] == c_kq[k]) return (double)(ea + eb) * pc;
+    double v = 0.0;
+    if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
+    if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
+    return v;
+  };
+  typedef double double4_t __attribute__((ext_vector_type(4)));
+  double4_t acc[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  double bnext = dval(kq);
+  for (int k4 = 0; k4 < K4; k4 += 4) {
+    const double b = bnext;
+    if (k4 + 4 < K4) bnext = dval(k4 + 4 + kq);
+    const double* arow = sA + (size_t)(k4 + kq) * W + (lane & 15);
+#pragma unroll
+    for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+  }
+  if (!valid) return;
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int m = m0 + t * 16 + kq + 4 * r;
+      if (m < npair) G[(size_t)m * dim + c] = acc[t][r];
+    }
+  }
+}
+
+static void upload_pair_tables(int n) {
+  static int done_for = -1;
+  if (done_for == n) return;
+  uint8_t kp[160] = {0}, kq[160] = {0};
+  for (int p = 0, k = 0; p < n; ++p)
+    for (int q = 0; q <= p; ++q, ++k) kp[k] = (uint8_t)p, kq[k] = (uint8_t)q;
+  SQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_kp), kp, sizeof(kp)));
+  SQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_kq), kq, sizeof(kq)));
+  done_for = n;
+}
+
+static void launch_gemm_fused(const Space& s, const SigmaDims& d, const double* At, const double* psi, double* G,
+                              hipStream_t st) {
+  upload_pair_tables(s.n);
+  const int threads = 512;
+  const int64_t cols_per_block = (threads / 64) * 16;
+  const int mt_total = d.kpad / 16;
+  int mt = mt_total;
+  while (mt > 1 && (size_t)d.K4 * mt * 16 * sizeof(double) > 65536) --mt;
+  const int groups = (mt_total + mt - 1) / mt;
+  dim3 grid((unsigned)((d.dim + cols_per_block - 1) / cols_per_block), groups);
+  size_t lds = sizeof(double) * (size_t)d.K4 * mt * 16;
+  // algorithmic traffic: psi read once per m-group (+ gathers hitting cache), G written
+  const double gbytes = 8.0 * (double)d.dim * (groups + d.npair);
+  const double gflops = 2.0 * (double)d.npair * (d.npair + 1) * (double)d.dim;
+  switch (mt) {
+#define SQ_GF(MT)                                                                                          \
+  case MT:                                                                                                 \
+    SQ_PROF(PC_GEMM, st, gbytes, gflops,                                                                   \
+            k_gemm_fused<MT><<<grid, threads, lds, st>>>(At, psi, s.d_aexc, s.d_bexc, G, d.npair, d.K4,    \
+                                                         d.kpad, s.nsa, s.nsb));                           \
+    break;
+    SQ_GF(1) SQ_GF(2) SQ_GF(3) SQ_GF(4) SQ_GF(5) SQ_GF(6) SQ_GF(7) SQ_GF(8) SQ_GF(9)
+#undef SQ_GF
+    default:
+      throw Error{SQ_ERR_ARG, "unsupported orbital count for the sigma GEMM"};
+  }
+}
+
 static void launch_gemm(const SigmaDims& d, const double* At, const double* D, double* G, hipStream_t st) {
   const int threads = 512;
   const int64_t cols_per_block = (threads / 64) * 16;
@@ -195,6 +297,15 @@ static void launch_gemm(const SigmaDims& d, const double* At, const double* D, d
   }
 }
 
+static bool fused_sigma() {
+  static int v = -1;
+  if (v < 0) {
+    const char* e = getenv("SQ_SIGMA_FUSED");
+    v = (e && e[0] == '0') ? 0 : 1;
+  }
+  return v == 1;
+}
+
 void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* energy_host,
                 double* work, hipStream_t st) {
   SigmaDims d = sigma_dims(s);
@@ -208,9 +319,13 @@ void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double*
   double* scal = partial + nb;
   dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
   const double d8 = 8.0 * (double)d.dim;
-  SQ_PROF(PC_DBUILD, st, d8 * (d.K4 + 1), 0.0,
-          k_dbuild<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, d.npair, d.K4, ae, be, psi, D));
-  launch_gemm(d, H.d_A, D, G, st);
+  if (fused_sigma()) {
+    launch_gemm_fused(s, d, H.d_A, psi, G, st);
+  } else {
+    SQ_PROF(PC_DBUILD, st, d8 * (d.K4 + 1), 0.0,
+            k_dbuild<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, d.npair, d.K4, ae, be, psi, D));
+    launch_gemm(d, H.d_A, D, G, st);
+  }
   SQ_PROF(PC_SIGMA_GATHER, st, d8 * (d.npair + 2), 0.0,
           k_sigma_gather<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
   if (energy_host) {
@@ -232,9 +347,13 @@ void sigma_energy_dev(const Space& s, const Hamiltonian& H, const double* psi, d
   int64_t nb = (int64_t)s.nsa * ((s.nsb + kB - 1) / kB);
   dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
   const double d8 = 8.0 * (double)d.dim;
-  SQ_PROF(PC_DBUILD, st, d8 * (d.K4 + 1), 0.0,
-          k_dbuild<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, d.npair, d.K4, ae, be, psi, D));
-  launch_gemm(d, H.d_A, D, G, st);
+  if (fused_sigma()) {
+    launch_gemm_fused(s, d, H.d_A, psi, G, st);
+  } else {
+    SQ_PROF(PC_DBUILD, st, d8 * (d.K4 + 1), 0.0,
+            k_dbuild<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, d.npair, d.K4, ae, be, psi, D));
+    launch_gemm(d, H.d_A, D, G, st);
+  }
   SQ_PROF(PC_SIGMA_GATHER, st, d8 * (d.npair + 2), 0.0,
           k_sigma_gather<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
   reduce_impl(partial, nb, e_dev, 0, false, 1.0, st);

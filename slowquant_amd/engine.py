@@ -249,3 +249,108 @@ def space_for(n_orb: int, n_alpha: int, n_beta: int) -> Space:
     if key not in _SPACES:
         _SPACES[key] = Space(*key)
     return _SPACES[key]
+
+
+def _ucc_work(space: Space, prog: Program) -> torch.Tensor:
+    nbytes = int(N.lib().sq_ucc_work_bytes(space.handle, prog.handle))
+    key = ("ucc", nbytes)
+    if key not in space._work:
+        space._work[key] = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device="cuda")
+    return space._work[key]
+
+
+def ucc_expm(prog: Program, theta, psi_in: torch.Tensor, t: float = 1.0) -> torch.Tensor:
+    """exp(t T(theta)) psi_in for a UCC program (new tensor)."""
+    th = prog.theta_tensor(theta)
+    out = torch.empty_like(psi_in)
+    N.check(N.lib().sq_ucc_expm(prog.space.handle, prog.handle, _ptr(th), float(t), _ptr(psi_in), _ptr(out),
+                                _ptr(_ucc_work(prog.space, prog)), _stream()))
+    return out
+
+
+def ucc_energy_grad(prog: Program, H: Hamiltonian, theta, psi0: torch.Tensor, want_grad: bool = True):
+    """E = <psi|H|psi>, psi = exp(T(theta)) psi0, and the analytic dE/dtheta."""
+    th = prog.theta_tensor(theta)
+    e = ctypes.c_double()
+    g = np.zeros(max(prog.n_gates, 1))
+    N.check(N.lib().sq_ucc_energy_grad(prog.space.handle, prog.handle, H.handle, _ptr(th), _ptr(psi0),
+                                       ctypes.byref(e), g.ctypes.data if want_grad else None,
+                                       _ptr(_ucc_work(prog.space, prog)), _stream()))
+    return e.value, (g[: prog.n_gates].copy() if want_grad else None)
+
+class Batch:
+    """Many independent instances (space, rotation program, Hamiltonian) whose
+    energies and adjoint gradients come from one persistent launch, started
+    from each space's HF determinant.  ``run(theta, graph=True)`` replays a
+    captured hipGraph of that launch."""
+
+    def __init__(self, instances, n_workgroups: int = 0) -> None:
+        self.instances = list(instances)
+        n = len(self.instances)
+        sp = (ctypes.c_void_p * n)(*[s.handle.value for s, _, _ in self.instances])
+        pr = (ctypes.c_void_p * n)(*[p.handle.value for _, p, _ in self.instances])
+        hh = (ctypes.c_void_p * n)(*[h.handle.value for _, _, h in self.instances])
+        b = ctypes.c_void_p()
+        N.check(N.lib().sq_batch_create(n, sp, pr, hh, int(n_workgroups), ctypes.byref(b)))
+        self._h = b
+        self.n_workgroups = int(n_workgroups)
+        self.n_params = int(N.lib().sq_batch_num_params(b))
+        self.offsets = np.cumsum([0] + [p.n_gates for _, p, _ in self.instances])
+
+    def __del__(self) -> None:
+        h = getattr(self, "_h", None)
+        if h is not None and N._lib is not None:
+            N.lib().sq_batch_destroy(h)
+            self._h = None
+
+    def set_theta(self, theta) -> None:
+        th = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64).ravel() for t in theta])
+                                  if isinstance(theta, (list, tuple)) else theta, dtype=np.float64)
+        if th.size != self.n_params:
+            raise ValueError(f"expected {self.n_params} parameters, got {th.size}")
+        self._theta_host = th  # kept alive until the asynchronous copy has run
+        N.check(N.lib().sq_batch_set_theta(self._h, th.ctypes.data, _stream()))
+
+    def launch(self, graph: bool = False) -> None:
+        fn = N.lib().sq_batch_run_graph if graph else N.lib().sq_batch_run
+        N.check(fn(self._h, _stream()))
+
+    def results(self) -> tuple[np.ndarray, list[np.ndarray]]:
+        e = np.zeros(len(self.instances))
+        g = np.zeros(max(self.n_params, 1))
+        N.check(N.lib().sq_batch_results(self._h, e.ctypes.data, g.ctypes.data, _stream()))
+        return e, [g[self.offsets[i]:self.offsets[i + 1]].copy() for i in range(len(self.instances))]
+
+    def run(self, theta, graph: bool = False) -> tuple[np.ndarray, list[np.ndarray]]:
+        self.set_theta(theta)
+        self.launch(graph)
+        return self.results()
+
+
+def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), reps: int = 3) -> Batch:
+    """Build one Batch per workgroup count in ``candidates`` (capped at the
+    number of instances), time ``reps`` graph replays of each and return the
+    fastest."""
+    best, best_t = None, float("inf")
+    seen = set()
+    for c in candidates:
+        c = min(int(c), len(instances))
+        if c in seen:
+            continue
+        seen.add(c)
+        b = Batch(instances, c)
+        b.set_theta(theta)
+        b.launch(graph=True)
+        torch.cuda.synchronize()
+        t0 = torch.cuda.Event(enable_timing=True)
+        t1 = torch.cuda.Event(enable_timing=True)
+        t0.record()
+        for _ in range(reps):
+            b.launch(graph=True)
+        t1.record()
+        t1.synchronize()
+        t = t0.elapsed_time(t1)
+        if t < best_t:
+            best, best_t = b, t
+    best.tuned_ms = best_t / reps
+    return best

@@ -107,12 +107,10 @@ def get_ucc_T(thetas, ucc_struct: UccStructure, offset: int = 0) -> FermionicOpe
 
 def construct_ucc_state(state, ci_info: CI_Info, thetas, ucc_struct: UccStructure, dagger: bool = False) -> np.ndarray:
     """exp(T(theta)) state, matrix-free on the GPU (operator_state_algebra.py:870-896)."""
-    from .ucc import expm_apply
-
     sp = _space(ci_info)
     prog = _ucc_program(sp, ucc_struct)
     th = np.asarray(thetas, dtype=np.float64)
-    out = expm_apply(prog, -th if dagger else th, _to_device(sp, state))
+    out = E.ucc_expm(prog, th, _to_device(sp, state), -1.0 if dagger else 1.0)
     return _to_host(sp, out)
 
 
@@ -133,9 +131,7 @@ def _apply_list(operators, psi: torch.Tensor, ci: CI_Info, sp: E.Space, thetas, 
             if isinstance(wf_struct, UpsStructure):
                 psi = _program(sp, wf_struct).state_prep(th, psi, dagger=(op == "Ud"))
             elif isinstance(wf_struct, UccStructure):
-                from .ucc import expm_apply
-
-                psi = expm_apply(_ucc_program(sp, wf_struct), -th if op == "Ud" else th, psi)
+                psi = E.ucc_expm(_ucc_program(sp, wf_struct), th, psi, -1.0 if op == "Ud" else 1.0)
             else:
                 raise TypeError(f"Got unknown wave function structure type, {type(wf_struct)}")
         else:

@@ -11,9 +11,7 @@ for the reference's product-rule sweep, or "plain").
 
 from __future__ import annotations
 
-import time
 from collections.abc import Sequence
-from functools import partial
 from typing import Any
 
 import numpy as np
@@ -21,7 +19,6 @@ import numpy as np
 from .density_matrix import get_electronic_energy, get_orbital_gradient
 from .fermionic_operator import FermionicOperator
 from .operators import hamiltonian_0i_0a
-from .optimizers import Optimizers
 from .util import UpsStructure
 from .wavefunction_base import WaveFunctionBase
 
@@ -248,90 +245,3 @@ class WaveFunctionUPS(WaveFunctionBase):
 
     def _calc_energy_rotosolve_optimization(self, parameters, theta_diffs, theta_idx):
         raise NotImplementedError("RotoSolve is not provided by the MI355X engine")
-
-    def run_wf_optimization_2step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
-                                  maxiter: int = 1000, is_silent_subiterations: bool = False) -> None:
-        print("### Parameters information:")
-        if orbital_optimization:
-            print(f"### Number kappa: {len(self.kappa)}")
-        print(f"### Number theta: {self.ups_layout.n_params}")
-        e_old = 1e12
-        e_new = None
-        print("Full optimization")
-        print("Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
-        for full_iter in range(int(maxiter)):
-            t0 = time.time()
-            if not is_silent_subiterations:
-                print("--------Ansatz optimization")
-                print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
-            opt = Optimizers(
-                partial(self._calc_energy_optimization, theta_optimization=True, kappa_optimization=False),
-                optimizer_name,
-                grad=partial(self._calc_gradient_optimization, theta_optimization=True, kappa_optimization=False),
-                maxiter=maxiter, tol=tol, is_silent=is_silent_subiterations,
-                energy_eval_callback=lambda: self.num_energy_evals,
-            )
-            self._old_opt_parameters = np.zeros_like(self.thetas) + 10**20
-            self._E_opt_old = 0.0
-            res = opt.minimize(self.thetas)
-            self.thetas = res.x.tolist()
-            if orbital_optimization and len(self.kappa) != 0:
-                if not is_silent_subiterations:
-                    print("--------Orbital optimization")
-                    print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
-                opt = Optimizers(
-                    partial(self._calc_energy_optimization, theta_optimization=False, kappa_optimization=True),
-                    "l-bfgs-b",
-                    grad=partial(self._calc_gradient_optimization, theta_optimization=False, kappa_optimization=True),
-                    maxiter=maxiter, tol=tol, is_silent=is_silent_subiterations,
-                    energy_eval_callback=lambda: self.num_energy_evals,
-                )
-                self._old_opt_parameters = np.zeros(len(self.kappa_idx)) + 10**20
-                self._E_opt_old = 0.0
-                res = opt.minimize([0.0] * len(self.kappa_idx))
-                for i in range(len(self.kappa)):
-                    self._kappa[i] = 0.0
-                    self._kappa_old[i] = 0.0
-            else:
-                e_new = res.fun
-                if orbital_optimization and len(self.kappa) == 0:
-                    print("WARNING: No orbital optimization performed, because there is no non-redundant orbital parameters.")
-                break
-            e_new = res.fun
-            print(f"{str(full_iter + 1).center(11)} | {f'{time.time() - t0:7.2f}'.center(18)} | "
-                  f"{f'{e_new:3.12f}'.center(27)} | {str(self.num_energy_evals).center(11)}")
-            if abs(e_new - e_old) < tol:
-                break
-            e_old = e_new
-        self._energy_elec = e_new
-
-    def run_wf_optimization_1step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
-                                  maxiter: int = 1000) -> None:
-        print("### Parameters information:")
-        if orbital_optimization:
-            print(f"### Number kappa: {len(self.kappa)}")
-        print(f"### Number theta: {self.ups_layout.n_params}")
-        if optimizer_name.lower() == "rotosolve" and orbital_optimization and len(self.kappa) != 0:
-            raise ValueError("Cannot use RotoSolve together with orbital optimization in the one-step solver.")
-        print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
-        theta_opt = not orbital_optimization or len(self.thetas) > 0
-        kappa_opt = bool(orbital_optimization)
-        energy = partial(self._calc_energy_optimization, theta_optimization=theta_opt, kappa_optimization=kappa_opt)
-        gradient = partial(self._calc_gradient_optimization, theta_optimization=theta_opt, kappa_optimization=kappa_opt)
-        if orbital_optimization:
-            parameters = self.kappa + self.thetas if len(self.thetas) > 0 else self.kappa
-        else:
-            parameters = self.thetas
-        opt = Optimizers(energy, optimizer_name, grad=gradient, maxiter=maxiter, tol=tol,
-                         energy_eval_callback=lambda: self.num_energy_evals)
-        self._old_opt_parameters = np.zeros_like(parameters) + 10**20
-        self._E_opt_old = 0.0
-        res = opt.minimize(parameters)
-        if orbital_optimization:
-            self.thetas = res.x[len(self.kappa):].tolist()
-            for i in range(len(self.kappa)):
-                self._kappa[i] = 0.0
-                self._kappa_old[i] = 0.0
-        else:
-            self.thetas = res.x.tolist()
-        self._energy_elec = res.fun

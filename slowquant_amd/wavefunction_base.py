@@ -15,7 +15,11 @@ from collections.abc import Sequence
 import numpy as np
 import scipy.linalg
 
+import time
+from functools import partial
+
 from .ci_spaces import get_indexing
+from .optimizers import Optimizers
 from .integrals import IntegralManager, one_electron_integral_transform, two_electron_integral_transform
 
 
@@ -168,3 +172,94 @@ class WaveFunctionBase:
             self._dev_ham = engine.Hamiltonian(self._space(), h_eff, g_act, e_core)
             self._dev_ham_key = key
         return self._dev_ham
+
+    # ---- optimisation drivers (ups_wavefunction.py:799-1017, ucc_wavefunction.py:723-965) --
+    def _print_parameter_info(self, orbital_optimization: bool) -> None:
+        print("### Parameters information:")
+        if orbital_optimization:
+            print(f"### Number kappa: {len(self.kappa)}")
+        print(f"### Number theta: {len(self._thetas)}")
+
+    def _reset_kappa_after_two_step(self) -> None:
+        for i in range(len(self.kappa)):
+            self._kappa[i] = 0.0
+            self._kappa_old[i] = 0.0
+
+    def run_wf_optimization_2step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
+                                  maxiter: int = 1000, is_silent_subiterations: bool = False) -> None:
+        self._print_parameter_info(orbital_optimization)
+        e_old = 1e12
+        e_new = None
+        print("Full optimization")
+        print("Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+        for full_iter in range(int(maxiter)):
+            t0 = time.time()
+            if not is_silent_subiterations:
+                print("--------Ansatz optimization")
+                print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+            opt = Optimizers(
+                partial(self._calc_energy_optimization, theta_optimization=True, kappa_optimization=False),
+                optimizer_name,
+                grad=partial(self._calc_gradient_optimization, theta_optimization=True, kappa_optimization=False),
+                maxiter=maxiter, tol=tol, is_silent=is_silent_subiterations,
+                energy_eval_callback=lambda: self.num_energy_evals,
+            )
+            self._old_opt_parameters = np.zeros_like(self.thetas) + 10**20
+            self._E_opt_old = 0.0
+            res = opt.minimize(self.thetas)
+            self.thetas = res.x.tolist()
+            if orbital_optimization and len(self.kappa) != 0:
+                if not is_silent_subiterations:
+                    print("--------Orbital optimization")
+                    print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+                opt = Optimizers(
+                    partial(self._calc_energy_optimization, theta_optimization=False, kappa_optimization=True),
+                    "l-bfgs-b",
+                    grad=partial(self._calc_gradient_optimization, theta_optimization=False, kappa_optimization=True),
+                    maxiter=maxiter, tol=tol, is_silent=is_silent_subiterations,
+                    energy_eval_callback=lambda: self.num_energy_evals,
+                )
+                self._old_opt_parameters = np.zeros(len(self.kappa_idx)) + 10**20
+                self._E_opt_old = 0.0
+                res = opt.minimize([0.0] * len(self.kappa_idx))
+                self._reset_kappa_after_two_step()
+            else:
+                e_new = res.fun
+                if orbital_optimization and len(self.kappa) == 0:
+                    print("WARNING: No orbital optimization performed, because there is no non-redundant orbital parameters.")
+                break
+            e_new = res.fun
+            print(f"{str(full_iter + 1).center(11)} | {f'{time.time() - t0:7.2f}'.center(18)} | "
+                  f"{f'{e_new:3.12f}'.center(27)} | {str(self.num_energy_evals).center(11)}")
+            if abs(e_new - e_old) < tol:
+                break
+            e_old = e_new
+        self._energy_elec = e_new
+
+    def run_wf_optimization_1step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
+                                  maxiter: int = 1000) -> None:
+        self._print_parameter_info(orbital_optimization)
+        if optimizer_name.lower() == "rotosolve" and orbital_optimization and len(self.kappa) != 0:
+            raise ValueError("Cannot use RotoSolve together with orbital optimization in the one-step solver.")
+        print("--------Iteration # | Iteration time [s] | Electronic energy [Hartree] | Energy measurement #")
+        theta_opt = not orbital_optimization or len(self.thetas) > 0
+        kappa_opt = bool(orbital_optimization)
+        energy = partial(self._calc_energy_optimization, theta_optimization=theta_opt, kappa_optimization=kappa_opt)
+        gradient = partial(self._calc_gradient_optimization, theta_optimization=theta_opt, kappa_optimization=kappa_opt)
+        if orbital_optimization:
+            parameters = self.kappa + self.thetas if len(self.thetas) > 0 else self.kappa
+        else:
+            parameters = self.thetas
+        opt = Optimizers(energy, optimizer_name, grad=gradient, maxiter=maxiter, tol=tol,
+                         energy_eval_callback=lambda: self.num_energy_evals)
+        self._old_opt_parameters = np.zeros_like(parameters) + 10**20
+        self._E_opt_old = 0.0
+        res = opt.minimize(parameters)
+        if orbital_optimization:
+            self.thetas = res.x[len(self.kappa):].tolist()
+            for i in range(len(self.kappa)):
+                self._kappa[i] = 0.0
+                self._kappa_old[i] = 0.0
+        else:
+            self.thetas = res.x.tolist()
+        self._energy_elec = res.fun

@@ -1,0 +1,112 @@
+"""Batched persistent launch (engine.Batch) against per-instance engine calls
+and, for the small spaces, the oracle.
+
+Instances: tUPS with 1-3 layers over (4e,4o), (6e,6o), (8e,8o), (10e,10o)
+with seeded synthetic Hamiltonians (rank 16, variance 0.3^2/n) and
+theta ~ N(0, 0.1^2).  Tolerances: energy 1e-10, gradient 1e-10 (analytic vs
+analytic); the oracle comparison uses the same bounds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _specs():
+    from slowquant_amd.batch import InstanceSpec
+
+    return [
+        InstanceSpec(4, 2, 2, 1, 11),
+        InstanceSpec(6, 3, 3, 2, 12),
+        InstanceSpec(8, 4, 4, 3, 13),
+        InstanceSpec(10, 5, 5, 1, 14),
+        InstanceSpec(4, 2, 2, 3, 15),
+        InstanceSpec(6, 3, 3, 1, 16),
+        InstanceSpec(8, 4, 4, 2, 17),
+        InstanceSpec(4, 2, 2, 2, 18),
+    ]
+
+
+def _close(e, g, e_ref, g_ref, tol=1e-10):
+    assert np.abs(np.asarray(e) - np.asarray(e_ref)).max() < tol
+    for a, b in zip(g, g_ref):
+        assert a.shape == b.shape
+        assert np.abs(a - b).max() < tol
+
+
+def test_batch_matches_instance_loop_and_oracle():
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, instance_problem, loop_energy_grad
+
+    specs = _specs()
+    inst, thetas = build_instances(specs)
+    e_loop, g_loop = loop_energy_grad(inst, thetas)
+    for nwg in (3, 8):  # several instances per workgroup, and one each
+        b = E.Batch(inst, nwg)
+        e, g = b.run(thetas)
+        _close(e, g, e_loop, g_loop)
+    # oracle for the (4e,4o) and (6e,6o) instances
+    from oracle import fermion as F
+    from oracle.apply import expectation_value
+    from oracle.ci import get_indexing
+    from oracle.ups import construct_ups_state, gradient_reference
+
+    for k, s in enumerate(specs):
+        if s.n_orb > 6:
+            continue
+        layout, h, gg, theta = instance_problem(s)
+        Hf = F.hamiltonian_0i_0a(h, gg, 0, s.n_orb)
+        ci = get_indexing(0, s.n_orb, 0, s.n_alpha, s.n_beta)
+        csf = np.zeros(len(ci.idx2det))
+        csf[0] = 1.0
+        psi = construct_ups_state(csf, ci, theta, layout)
+        e_ref = expectation_value(psi, [Hf], psi, ci, do_folding=False)
+        g_ref = gradient_reference(Hf, csf, ci, theta, layout)
+        assert abs(e[k] - e_ref) < 1e-10
+        assert np.abs(g[k] - g_ref).max() < 1e-10
+
+
+def test_batch_graph_replays():
+    """Replays of the captured graph with new theta and eager work between
+    them; every replay equals the eager per-instance result."""
+    import torch
+
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, loop_energy_grad
+
+    specs = _specs()[:5]
+    inst, thetas = build_instances(specs)
+    b = E.Batch(inst, 4)
+    rng = np.random.default_rng(5)
+    for rep in range(3):
+        th = [t + rng.normal(0.0, 0.02, t.shape) for t in thetas] if rep else thetas
+        e, g = b.run(th, graph=True)
+        sp, prog, H = inst[2]
+        H.sigma(E.hf_state(sp))  # unrelated eager work on the same stream
+        torch.cuda.synchronize()
+        e_loop, g_loop = loop_energy_grad(inst, th)
+        _close(e, g, e_loop, g_loop)
+
+
+def test_batch_autotune_and_errors():
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, loop_energy_grad
+
+    specs = _specs()[:4]
+    inst, thetas = build_instances(specs)
+    b = E.autotune_batch(inst, np.concatenate(thetas), candidates=(1, 2, 4))
+    assert b.tuned_ms > 0
+    e, g = b.run(thetas, graph=True)
+    e_loop, g_loop = loop_energy_grad(inst, thetas)
+    _close(e, g, e_loop, g_loop)
+    with pytest.raises(ValueError):
+        b.set_theta(np.zeros(b.n_params + 1))
+    # polynomial (sa_double_2) gates are not rotations: rejected with ValueError
+    sp = E.space_for(4, 2, 2)
+    prog = E.Program(sp, ["sa_double_2"], [(0, 1, 2, 3)])
+    H = inst[0][2]
+    with pytest.raises(ValueError):
+        E.Batch([(sp, prog, H)], 1)

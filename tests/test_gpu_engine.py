@@ -260,3 +260,43 @@ def test_rdm_parity(n, na, nb):
     e2 = rdm2(psi_ref, ci, e1)
     assert np.abs(r1 - e1).max() < 1e-12
     assert np.abs(r2 - e2).max() < 1e-12
+
+
+@pytest.mark.parametrize(
+    "n,na,opts",
+    [
+        (6, 3, {"n_layers": 2, "do_tups": True}),
+        (5, 2, {"n_layers": 2, "do_tups": True}),
+        (8, 4, {"n_layers": 2, "do_qnp": True}),
+        (10, 5, {"n_layers": 1, "do_tups": True, "skip_last_singles": True}),
+        (14, 7, {"n_layers": 1, "do_tups": True}),
+    ],
+)
+def test_tiled_stages_match_per_gate_kernels(n, na, opts, monkeypatch):
+    """Fused brick-column passes (default) against the per-gate rotation kernels
+    (SQ_TILES=0) for state preparation, dagger and all three gradients."""
+    from slowquant_amd.synthetic import synthetic_problem
+    from slowquant_amd.util import UpsStructure
+
+    E = _engine()
+    lay = UpsStructure()
+    lay.create_tiled(n, dict(opts))
+    h, g, theta = synthetic_problem(n, n, lay.n_params, 0.2, 5)
+    sp = E.Space(n, na, na)
+    tiled = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    monkeypatch.setenv("SQ_TILES", "0")
+    plain_prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    psi0 = E.hf_state(sp)
+    a = plain_prog.state_prep(theta, psi0).cpu().numpy()
+    monkeypatch.setenv("SQ_TILES", "1")
+    b = tiled.state_prep(theta, psi0).cpu().numpy()
+    assert np.abs(a - b).max() < 1e-12
+    back = tiled.state_prep(theta, tiled.state_prep(theta, psi0), dagger=True).cpu().numpy()
+    assert np.abs(back - psi0.cpu().numpy()).max() < 1e-12
+    if n <= 10:
+        H = E.Hamiltonian(sp, h, g)
+        e0, g0 = plain_prog.energy_grad(H, theta, psi0, "adjoint")
+        for method in ("adjoint", "plain", "reference"):
+            e1, g1 = tiled.energy_grad(H, theta, psi0, method)
+            assert abs(e1 - e0) < 1e-10, method
+            assert np.abs(g1 - g0).max() < 1e-10, method

@@ -1,0 +1,120 @@
+"""GPU checks of WaveFunctionUCC (matrix-free exp(T)) and of the module-level
+propagate_state / expectation_value API against the oracle."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _h4_rect(y=1.8, x=1.5):
+    from slowquant_amd.hchain import HydrogenSlowQuant
+
+    sq = HydrogenSlowQuant()
+    sq.set_molecule(f"H 0.0 0.0 0.0; H 0.0 {y} 0.0; H {x} 0.0 0.0; H {x} {y} 0.0", distance_unit="angstrom")
+    sq.set_basis_set("STO-3G")
+    sq.init_hartree_fock()
+    sq.hartree_fock.run_restricted_hartree_fock()
+    return sq
+
+
+@pytest.mark.parametrize("exc", ["SD", "SDTQ"])
+def test_ucc_state_and_gradient_vs_oracle(exc):
+    from oracle import fermion as F
+    from oracle.apply import expectation_value
+    from oracle.ci import get_indexing
+    from oracle.ucc import construct_ucc_state
+
+    from slowquant_amd.ucc_wavefunction import WaveFunctionUCC
+
+    sq = _h4_rect()
+    wf = WaveFunctionUCC((4, 4), sq.hartree_fock.mo_coeff, sq, exc)
+    theta = np.random.default_rng(2).normal(0, 0.1, wf.ucc_layout.n_params)
+    wf.thetas = theta.tolist()
+    ci = get_indexing(0, 4, 0, 2, 2)
+    ref = construct_ucc_state(wf.csf_coeffs, ci, theta, wf.ucc_layout)
+    assert np.abs(wf.ci_coeffs - ref).max() < 1e-12
+    H = F.hamiltonian_0i_0a(wf.h_mo, wf.g_mo, 0, 4)
+
+    def energy(th):
+        psi = construct_ucc_state(wf.csf_coeffs, ci, th, wf.ucc_layout)
+        return expectation_value(psi, [H], psi, ci, do_folding=False)
+
+    assert abs(wf.energy_elec - energy(theta)) < 1e-10
+    e, g = wf.energy_and_gradient()
+    h = 1e-5
+    fd = np.array([(energy(theta + h * v) - energy(theta - h * v)) / (2 * h) for v in np.eye(len(theta))])
+    assert np.abs(g - fd).max() < 1e-8
+    wf.gradient_method = "finite_difference"
+    g_fd = wf._calc_gradient_optimization(theta.tolist(), True, False)
+    assert np.abs(g_fd - g).max() < 1e-6
+
+
+def test_ucc_h4_sdtq_known_answer():
+    """tests/test_unitary_coupled_cluster.py:589-609: UCCSDTQ(4,4) on the H4
+    1.5 x 1.8 Angstrom rectangle, E_elec = -3.714153922167 (tol 1e-8)."""
+    from slowquant_amd.ucc_wavefunction import WaveFunctionUCC
+
+    sq = _h4_rect(1.8, 1.5)
+    wf = WaveFunctionUCC((4, 4), sq.hartree_fock.mo_coeff, sq, "SDTQ")
+    wf.run_wf_optimization_1step("BFGS", False)
+    assert abs(wf.energy_elec - (-3.714153922167)) < 1e-8
+
+
+def test_ucc_h4_dq_known_answer():
+    """tests/test_unitary_coupled_cluster.py:190-214: UCCDQ(4,4), 1.8 x 1.5
+    rectangle, E_elec + E_nuc = -1.968914822185857 (tol 1e-7)."""
+    from slowquant_amd.ucc_wavefunction import WaveFunctionUCC
+
+    sq = _h4_rect(1.5, 1.8)
+    wf = WaveFunctionUCC((4, 4), sq.hartree_fock.mo_coeff, sq, "DQ")
+    wf.run_wf_optimization_1step("BFGS", False)
+    assert abs(wf.energy_elec + sq.molecule.nuclear_repulsion - (-1.968914822185857)) < 1e-7
+
+
+def test_module_api_against_oracle():
+    from oracle import fermion as F
+    from oracle.apply import propagate_state as o_prop
+    from oracle.ci import get_indexing as o_idx
+    from oracle.ups import construct_ups_state as o_ups
+
+    from slowquant_amd import operator_state_algebra as OSA
+    from slowquant_amd import operators as OP
+    from slowquant_amd.ci_spaces import get_indexing
+    from slowquant_amd.util import UpsStructure
+
+    n = 4
+    ci = get_indexing(0, n, 0, 2, 2)
+    oci = o_idx(0, n, 0, 2, 2)
+    rng = np.random.default_rng(3)
+    psi = rng.normal(size=36)
+    psi /= np.linalg.norm(psi)
+    lay = UpsStructure()
+    lay.create_tiled(n, {"n_layers": 1, "do_tups": True})
+    th = rng.normal(0, 0.3, lay.n_params)
+    # E_pq products, G2_sa generators, "U"/"Ud"
+    ops = [OP.Epq(1, 2) * OP.Epq(2, 0), OP.G2_sa(0, 1, 2, 3, 4, True)]
+    oops = [F.Epq(1, 2) * F.Epq(2, 0), F.G2_sa(0, 1, 2, 3, 4, ah=True)]
+    got = OSA.propagate_state(ops, psi, ci)
+    exp = o_prop(oops, psi, oci)
+    assert np.abs(got - exp).max() < 1e-12
+    got = OSA.propagate_state(["Ud", ops[0], "U"], psi, ci, th, lay)
+    v = o_ups(psi, oci, th, lay)
+    v = o_prop([oops[0]], v, oci)
+    v = o_ups(v, oci, th, lay, dagger=True)
+    assert np.abs(got - v).max() < 1e-12
+    e = OSA.expectation_value(psi, [ops[0]], psi, ci)
+    assert abs(e - float(psi @ exp_single(oops[0], psi, oci))) < 1e-12
+    assert np.abs(OSA.construct_ups_state(psi, ci, th, lay) - o_ups(psi, oci, th, lay)).max() < 1e-12
+    M = OSA.build_operator_matrix(ops[1], ci)
+    assert np.abs(M + M.T).max() < 1e-13
+    with pytest.raises(KeyError):
+        OSA.propagate_state([OP.G1(0, 1)], psi, ci)  # alpha -> beta leaves the space
+
+
+def exp_single(op, psi, ci):
+    from oracle.apply import propagate_state
+
+    return propagate_state([op], psi, ci)
