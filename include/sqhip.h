@@ -141,6 +141,15 @@ int sq_energy_grad_reference_sweep(const sq_space* s, const sq_program* p, const
                                    const double* theta, const double* psi0, double* energy, double* grad,
                                    double* work, void* stream);
 
+/* Reverse half of the adjoint gradient, for a torch.autograd.Function whose
+ * forward ran sq_state_prep (psi = U(theta) psi0) and sq_sigma (lam = H psi):
+ * one reverse sweep un-applies every gate on psi and lam in place (both are
+ * consumed) and writes dE/dtheta_i = 2 <lam_i|T_i psi_i> to grad (host).
+ * Replaces the theta block of ups_wavefunction.py:1060-1142.
+ * work: sq_work_bytes(s, 2). */
+int sq_adjoint_reverse(const sq_space* s, const sq_program* p, const double* theta, double* psi, double* lam,
+                       double* grad, double* work, void* stream);
+
 /* ---- RDMs (ups_wavefunction.py:409-476, density_matrix.py:5-136) ---------
  * rdm1: host n*n, rdm2: host n^4 (p,q,r,s) = <E_pq E_rs> - delta_qr rdm1_ps. */
 int sq_rdm12(const sq_space* s, const double* psi, double* rdm1, double* rdm2, double* work, void* stream);

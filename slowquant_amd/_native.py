@@ -83,6 +83,7 @@ _SIGNATURES = {
     "sq_energy_grad_adjoint": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
     "sq_energy_grad_plain": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
     "sq_energy_grad_reference_sweep": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
+    "sq_adjoint_reverse": ([vp, vp, dp, dp, dp, vp, dp, vp], i32),
     "sq_rdm12": ([vp, dp, vp, vp, dp, vp], i32),
     "sq_debug_gemm": ([dp, dp, dp, i32, i32, i32, i64, vp], i32),
     "sq_operator_create": ([vp, ctypes.POINTER(OpString), i32, i32, ctypes.POINTER(vp)], i32),

@@ -27,7 +27,7 @@ class InstanceSpec:
     n_beta: int
     layers: int
     seed: int
-    rank: int = 16
+    rank: int = 0  # 0: 2 * n_orb
     sigma_theta: float = 0.1
 
 
@@ -37,22 +37,25 @@ def tups_layout(n_orb: int, layers: int) -> UpsStructure:
     return layout
 
 
-def draw_specs(n_inst: int, seed: int = 0, spaces=BATCH_SPACES, layers=(1, 2, 3)) -> list[InstanceSpec]:
-    """Spaces uniformly from ``spaces``, layer counts uniformly from ``layers``,
-    one synthetic-Hamiltonian seed per instance (seed*100000 + i)."""
+def draw_specs(n_inst: int = 512, seed: int = 7, spaces=BATCH_SPACES, layers=(1, 2, 3)) -> list[InstanceSpec]:
+    """The 512-instance workload: rng = default_rng(seed); per instance k the
+    space (uniform over ``spaces``), then the layer count (uniform over
+    ``layers``); instance k's Hamiltonian and theta use seed 1000 + k, rank
+    2n, variance 0.3^2/n, theta ~ N(0, 0.1^2)."""
     rng = np.random.default_rng(seed)
     out = []
-    for i in range(n_inst):
+    for k in range(n_inst):
         n, na, nb = spaces[int(rng.integers(len(spaces)))]
         L = int(layers[int(rng.integers(len(layers)))])
-        out.append(InstanceSpec(n, na, nb, L, seed * 100000 + i))
+        out.append(InstanceSpec(n, na, nb, L, 1000 + k))
     return out
 
 
 def instance_problem(spec: InstanceSpec):
     """(layout, h, g, theta) of one instance."""
     layout = tups_layout(spec.n_orb, spec.layers)
-    h, g, theta = synthetic_problem(spec.n_orb, spec.rank, layout.n_params, spec.sigma_theta, spec.seed)
+    rank = spec.rank if spec.rank > 0 else 2 * spec.n_orb
+    h, g, theta = synthetic_problem(spec.n_orb, rank, layout.n_params, spec.sigma_theta, spec.seed)
     return layout, h, g, theta
 
 

@@ -78,6 +78,22 @@ void energy_grad_adjoint(const Space& s, const Program& p, const Hamiltonian& H,
   finish(b, n, energy, grad, st);
 }
 
+// Reverse half of the adjoint on caller-owned psi = U psi0 and lam = H psi
+// (consumed in place); dE/dtheta to host.
+void adjoint_reverse(const Space& s, const Program& p, const double* theta, double* psi, double* lam, double* grad,
+                     double* work, hipStream_t st) {
+  const int n = (int)p.gates.size();
+  GradBufs b = carve(s, n, work);
+  const TilePlan* T = tiles_for(s, p);
+  if (T) {
+    tiles_reverse(s, p, *T, theta, psi, lam, b.s1, b.s2, b.t1, b.partial, b.grad, st);
+  } else {
+    for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, psi, lam, b.grad, b.t1, b.partial, st);
+  }
+  if (n > 0) SQ_HIP(hipMemcpyAsync(grad, b.grad, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+}
+
 void energy_grad_plain(const Space& s, const Program& p, const Hamiltonian& H, const double* theta,
                        const double* psi0, double* energy, double* grad, double* work, hipStream_t st) {
   const int n = (int)p.gates.size();
@@ -155,5 +171,15 @@ int64_t sq_work_bytes(const sq_space* sp, int what) {
 SQ_GRAD_ENTRY(sq_energy_grad_adjoint, energy_grad_adjoint)
 SQ_GRAD_ENTRY(sq_energy_grad_plain, energy_grad_plain)
 SQ_GRAD_ENTRY(sq_energy_grad_reference_sweep, energy_grad_reference_sweep)
+
+int sq_adjoint_reverse(const sq_space* sp, const sq_program* pr, const double* theta, double* psi, double* lam,
+                       double* grad, double* work, void* stream) {
+  return guarded([&] {
+    SQ_CHECK(sp && pr && psi && lam && work, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK((theta && grad) || pr->p.gates.empty(), SQ_ERR_ARG, "theta or grad is NULL");
+    SQ_CHECK((int)pr->p.gates.size() <= 65536, SQ_ERR_ARG, "too many gates");
+    adjoint_reverse(sp->s, pr->p, theta, psi, lam, grad, work, as_stream(stream));
+  });
+}
 
 }  // extern "C"

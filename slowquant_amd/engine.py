@@ -119,7 +119,10 @@ class Program:
             self._h = None
 
     def theta_tensor(self, theta) -> torch.Tensor:
-        t = torch.as_tensor(np.asarray(theta, dtype=np.float64)).to("cuda")
+        if isinstance(theta, torch.Tensor):
+            t = theta.detach().to(device="cuda", dtype=torch.float64)
+        else:
+            t = torch.as_tensor(np.asarray(theta, dtype=np.float64)).to("cuda")
         if t.numel() != self.n_gates:
             raise ValueError(f"Expected {self.n_gates} theta values got {t.numel()}")
         return t.contiguous()
@@ -159,6 +162,15 @@ class Program:
                _ptr(self.space.work(2)), _stream())
         )
         return e.value, g[: self.n_gates].copy()
+
+
+    def adjoint_reverse(self, theta, psi: torch.Tensor, lam: torch.Tensor) -> np.ndarray:
+        """dE/dtheta from psi = U(theta) psi0 and lam = H psi; both are consumed."""
+        th = self.theta_tensor(theta)
+        g = np.zeros(max(self.n_gates, 1))
+        N.check(N.lib().sq_adjoint_reverse(self.space.handle, self._h, _ptr(th), _ptr(psi), _ptr(lam), g.ctypes.data,
+                                           _ptr(self.space.work(2)), _stream()))
+        return g[: self.n_gates].copy()
 
 
 class Hamiltonian:
