@@ -170,9 +170,11 @@ def main() -> int:
 
     e_adj, g_adj = step()
     t_adj = time_steps(step, args.steps, args.warmup)
-    t_plain = time_steps(lambda: prog.energy_grad(H, th, psi0, "plain"), args.plain_steps, 0)
-    e_pl, g_pl = prog.energy_grad(H, th, psi0, "plain")
-    grad_diff = float(np.abs(g_adj - g_pl).max())
+    t_plain = grad_diff = None
+    if args.plain_steps > 0:
+        t_plain = time_steps(lambda: prog.energy_grad(H, th, psi0, "plain"), args.plain_steps, 0)
+        e_pl, g_pl = prog.energy_grad(H, th, psi0, "plain")
+        grad_diff = float(np.abs(g_adj - g_pl).max())
     prof = kernel_profile(sp, prog, H, th, psi0, reps=3)
     dom = prof["dominant"]
     roof = {
@@ -204,8 +206,9 @@ def main() -> int:
             "ci_dim": sp.dim,
             "n_params": lay.n_params,
         },
-        "plain": {"value": 1.0 / t_plain, "unit": "evals/s", "ms_per_step": 1e3 * t_plain, "steps": args.plain_steps},
-        "adjoint_over_plain": t_plain / t_adj,
+        "plain": None if t_plain is None else {
+            "value": 1.0 / t_plain, "unit": "evals/s", "ms_per_step": 1e3 * t_plain, "steps": args.plain_steps},
+        "adjoint_over_plain": None if t_plain is None else t_plain / t_adj,
         "max_grad_diff_adjoint_vs_plain": grad_diff,
         "energy": e_adj,
         "roofline": roof,

@@ -36,6 +36,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 #include <numeric>
 
 #include "sq_internal.h"
@@ -43,9 +44,10 @@
 namespace sq {
 
 constexpr int kMaxPairs = 6;        // hypercubes of at most 64 strings per spin
-constexpr int kTileMax = 4096;      // doubles per vector per tile
+constexpr int kTileMax = 4096;      // doubles per vector per tile (large tiles)
+constexpr int kSmallMax = 1024;     // largest tile run by the one-wave kernel
 constexpr int kTileMaxCols = 1024;  // columns per tile
-constexpr int kTB = 256;            // threads per tile block
+constexpr int kTB = 256;            // threads per block of the large-tile kernel and reductions
 constexpr int kMaxStageGates = 48;  // stage entries (sa_single counts twice)
 constexpr int kMaxPairEntries = 8;  // entries per pair within a stage
 constexpr int kTypes = 9;           // orbit types (alpha kind x beta kind)
@@ -63,18 +65,23 @@ struct StageEntry {
   uint32_t code[kTypes];
 };
 
+// Per pair j of a stage, a 5-bit field (bits 5j..5j+4) of a string's
+// descriptor: kind (2 bits: 0 pair empty, 1 pair doubly occupied, 2 singly
+// occupied with hypercube bit 0, 3 singly occupied with hypercube bit 1) and
+// the hypercube bit index (3 bits).
 struct StageDev {
   int npairs;
   int nentries;
-  int ntiles;
   int nsb;
+  int tile0;                 // first tile of this launch (small tiles, then large ones)
+  int xcd;                   // 1: XCD-contiguous tile ranges (SQ_XCD_REMAP=0 turns it off)
+  int emax, wmax;            // largest tile of this launch: elements, columns
   int pe[kMaxPairs + 1];     // entries of pair j: [pe[j], pe[j+1]) (entries sorted by pair)
   int64_t partial_off;       // this stage's block of the partial-dot buffer
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
   const int32_t* agoff;      // [ngroups+1]
-  const uint32_t* agocc;     // per alpha group: 2 bits per pair (local occupancy of non-single pairs)
-  const uint32_t* agbit;     // per alpha group: 4 bits per pair: hypercube bit, 0xF if not single
-  const uint64_t* bdesc;     // per beta stage position: bits(24) | occ(12) << 24 | offset in group << 36
+  const uint32_t* adesc;     // per alpha group: descriptor (kind 2 for every single pair)
+  const uint32_t* bdesc;     // per beta stage position: descriptor
   const int32_t* bcol;       // per beta stage position: canonical beta column
   const int4* tiles;         // (alpha group, col0, col1, -)
   const StageEntry* entries;
@@ -86,7 +93,8 @@ struct Stage {
   int npairs = 0;
   int pp[kMaxPairs];
   std::vector<StageEntry> entries;
-  StageDev dev{};
+  int ntiles = 0, nsmall = 0;  // tiles [0, nsmall) run one wave each, the rest 256 threads
+  StageDev small{}, large{};   // launch descriptors
   std::vector<void*> allocs;
 };
 
@@ -287,53 +295,110 @@ static T* upload(std::vector<void*>& allocs, const std::vector<T>& v) {
   return d;
 }
 
+// Elements per one-wave tile (SQ_TILE_TARGET overrides); tiles above it (a
+// single group pair larger than the target) run in the 256-thread kernel.
+static int tile_target() {
+  const char* env = getenv("SQ_TILE_TARGET");
+  int t = env ? atoi(env) : 512;
+  return std::min(std::max(t, 16), kSmallMax);
+}
+
+// 5-bit pair fields of a string given its local occupancies (2 bits per pair)
+// and hypercube bit indices (4 bits per pair, 0xF if the pair is not single).
+static uint32_t pair_desc(uint32_t occ, uint32_t bits, int npairs) {
+  uint32_t d = 0;
+  for (int j = 0; j < npairs; ++j) {
+    const uint32_t loc = (occ >> (2 * j)) & 3u;
+    const uint32_t b = (bits >> (4 * j)) & 0xFu;
+    uint32_t f;
+    if (b == 0xFu) f = loc == 0 ? 0u : 1u;
+    else f = (loc == 2u ? 3u : 2u) | (b << 2);
+    d |= f << (5 * j);
+  }
+  return d;
+}
+
 static void finalize_stage(const Space& sp, Stage& S) {
   // entries grouped by pair, program order kept within a pair
   std::stable_sort(S.entries.begin(), S.entries.end(),
                    [](const StageEntry& a, const StageEntry& b) { return a.pair < b.pair; });
+  int pe[kMaxPairs + 1];
   for (int j = 0, k = 0; j <= S.npairs; ++j) {
     while (k < (int)S.entries.size() && S.entries[k].pair < j) ++k;
-    S.dev.pe[j] = k;
+    pe[j] = k;
   }
   GroupInfo A = group_strings(sp.h_astr, S.npairs, S.pp);
   GroupInfo B = group_strings(sp.h_bstr, S.npairs, S.pp);
-  std::vector<int4> tiles;
+  // tiles: each alpha group x runs of whole beta groups, up to the target
+  // size; a beta group that alone exceeds the target gets its own tile
+  const int target = tile_target();
+  std::vector<int4> small, large;
   const int nag = (int)A.goff.size() - 1;
   const int nbg = (int)B.goff.size() - 1;
   for (int ga = 0; ga < nag; ++ga) {
     const int R = A.goff[ga + 1] - A.goff[ga];
-    int c0 = 0;
+    SQ_CHECK(R <= 64, SQ_ERR_ARG, "tile does not fit: alpha hypercube too large");
     int gb = 0;
     while (gb < nbg) {
+      const int c0 = B.goff[gb];
       int c1 = B.goff[gb + 1];
       ++gb;
-      while (gb < nbg) {
-        int nc1 = B.goff[gb + 1];
-        if ((int64_t)R * (nc1 - c0) > kTileMax || nc1 - c0 > kTileMaxCols) break;
-        c1 = nc1;
+      while (gb < nbg && (int64_t)R * (B.goff[gb + 1] - c0) <= target) {
+        c1 = B.goff[gb + 1];
         ++gb;
       }
-      SQ_CHECK((int64_t)R * (c1 - c0) <= kTileMax && c1 - c0 <= kTileMaxCols && R <= 64, SQ_ERR_ARG,
-               "tile does not fit: hypercube groups too large");
-      tiles.push_back(int4{ga, c0, c1, 0});
-      c0 = c1;
+      const int64_t E = (int64_t)R * (c1 - c0);
+      SQ_CHECK(E <= kTileMax && c1 - c0 <= kTileMaxCols, SQ_ERR_ARG, "tile does not fit: hypercube groups too large");
+      (E <= target ? small : large).push_back(int4{ga, c0, c1, 0});
     }
   }
-  std::vector<uint64_t> bdesc(sp.nsb);
-  for (int k = 0; k < sp.nsb; ++k)
-    bdesc[k] = (uint64_t)B.bit[k] | ((uint64_t)B.occ[k] << 24) | ((uint64_t)(k - B.start[k]) << 36);
-  S.dev.npairs = S.npairs;
-  S.dev.nentries = (int)S.entries.size();
-  S.dev.ntiles = (int)tiles.size();
-  S.dev.nsb = sp.nsb;
-  S.dev.arows = upload(S.allocs, A.order);
-  S.dev.agoff = upload(S.allocs, A.goff);
-  S.dev.agocc = upload(S.allocs, A.gocc);
-  S.dev.agbit = upload(S.allocs, A.gbit);
-  S.dev.bdesc = upload(S.allocs, bdesc);
-  S.dev.bcol = upload(S.allocs, B.order);
-  S.dev.tiles = upload(S.allocs, tiles);
-  S.dev.entries = upload(S.allocs, S.entries);
+  std::vector<int4> tiles(small);
+  tiles.insert(tiles.end(), large.begin(), large.end());
+  S.ntiles = (int)tiles.size();
+  S.nsmall = (int)small.size();
+  std::vector<uint32_t> adesc(nag), bdesc(sp.nsb);
+  for (int ga = 0; ga < nag; ++ga) {
+    // kind 2 (single, bit 0) for single pairs; the row's own bit is its hypercube index
+    uint32_t d = 0;
+    for (int j = 0; j < S.npairs; ++j) {
+      const uint32_t b = (A.gbit[ga] >> (4 * j)) & 0xFu;
+      const uint32_t loc = (A.gocc[ga] >> (2 * j)) & 3u;
+      const uint32_t f = b == 0xFu ? (loc == 0 ? 0u : 1u) : (2u | (b << 2));
+      d |= f << (5 * j);
+    }
+    adesc[ga] = d;
+  }
+  for (int k = 0; k < sp.nsb; ++k) bdesc[k] = pair_desc(B.occ[k], B.bit[k], S.npairs);
+  StageDev d{};
+  d.npairs = S.npairs;
+  d.nentries = (int)S.entries.size();
+  d.nsb = sp.nsb;
+  for (int j = 0; j <= kMaxPairs; ++j) d.pe[j] = j <= S.npairs ? pe[j] : pe[S.npairs];
+  d.arows = upload(S.allocs, A.order);
+  d.agoff = upload(S.allocs, A.goff);
+  d.adesc = upload(S.allocs, adesc);
+  d.bdesc = upload(S.allocs, bdesc);
+  d.bcol = upload(S.allocs, B.order);
+  d.tiles = upload(S.allocs, tiles);
+  d.entries = upload(S.allocs, S.entries);
+  const char* xenv = getenv("SQ_XCD_REMAP");
+  d.xcd = (xenv && xenv[0] == '0') ? 0 : 1;
+  S.small = d;
+  S.large = d;
+  S.small.tile0 = 0;
+  S.large.tile0 = S.nsmall;
+  auto extent = [&](int t0, int t1, StageDev& D) {
+    D.emax = 0;
+    D.wmax = 0;
+    for (int t = t0; t < t1; ++t) {
+      const int R = A.goff[tiles[t].x + 1] - A.goff[tiles[t].x];
+      const int W = tiles[t].z - tiles[t].y;
+      D.emax = std::max(D.emax, R * W);
+      D.wmax = std::max(D.wmax, W);
+    }
+  };
+  extent(0, S.nsmall, S.small);
+  extent(S.nsmall, S.ntiles, S.large);
 }
 
 TilePlan* build_tile_plan(const Space& sp, const Program& P) {
@@ -389,15 +454,16 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
     for (auto& S : T->stages) {
       if (!S.tiled) continue;
       finalize_stage(sp, S);
-      S.dev.partial_off = off;
-      const int ne = S.dev.nentries;
+      S.small.partial_off = off;
+      S.large.partial_off = off;
+      const int ne = S.small.nentries;
       for (int k = 0; k < ne;) {
         int k1 = k + 1;
         while (k1 < ne && S.entries[k1].gate == S.entries[k].gate) ++k1;
-        red.push_back(GateRed{off, S.dev.ntiles, ne, k, k1, S.entries[k].gate, 0});
+        red.push_back(GateRed{off, S.ntiles, ne, k, k1, S.entries[k].gate, 0});
         k = k1;
       }
-      off += (int64_t)S.dev.ntiles * ne;
+      off += (int64_t)S.ntiles * ne;
     }
     SQ_HIP(hipMalloc(&T->d_partial, sizeof(double) * std::max<int64_t>(off, 1)));
     T->partial_len = off;
@@ -437,47 +503,135 @@ const TilePlan* tiles_for(const Space& s, const Program& p) {
 
 // ---------------------------------------------------------------------------
 // Device kernels.
+//
+// Tile kernels are templated on the block size: NT = 64 (one wave, tiles of
+// up to the plan's target size, many resident blocks per CU) or NT = 256
+// (tiles of up to kTileMax).  LDS is dynamic, sized to the launch's largest
+// tile:
+//   X[emax] (| L[emax] in the reverse sweep) | cdesc[wmax] | ccol[wmax] |
+//   codes[ne*9] | pad | cs[3*ne] (cos, sin, scale) | rows[64] |
+//   red[ne * NT/64] (reverse, NT > 64)
+//
+// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8);
+// tiles are remapped so that each XCD runs a contiguous range of the tile
+// list.  Consecutive tiles share alpha rows, so the rows' cache lines are
+// fetched into one XCD's L2 and the scattered column accesses of the tiles
+// hit there.
 
-struct TileShared {
+struct TileView {
   int R, W, E, c0;
   float invW;
-  uint32_t gocc, gbit;
-  int32_t rows[64];
+  uint32_t adesc;
 };
 
-// Block-cooperative load of a tile's descriptors (row list, column table,
-// per-entry cos/sin and codes) into LDS.
-__device__ __forceinline__ void tile_setup(const StageDev& S, TileShared& t, uint64_t* cdesc, uint16_t* ccol,
-                                           const double* __restrict__ theta, double sgn_th, double* cs_c,
-                                           double* cs_s, uint32_t* codes) {
-  const int4 T = S.tiles[blockIdx.x];
+struct TileLds {
+  double* X;
+  double* L;
+  uint32_t* cdesc;
+  uint32_t* ccol;
+  uint32_t* codes;
+  double* cs;
+  int32_t* rows;
+  double* red;
+};
+
+static size_t lds_bytes(int nt, int emax, int wmax, int ne, bool rev) {
+  const size_t n32 = 2 * (size_t)wmax + (size_t)ne * kTypes;
+  size_t b = (size_t)emax * 8 * (rev ? 2 : 1) + (n32 + (n32 & 1)) * 4 + (size_t)ne * 24 + 64 * 4;
+  if (rev && nt > 64) b += (size_t)ne * (nt / 64) * 8;
+  return b;
+}
+
+__device__ __forceinline__ TileLds carve_lds(double* smem, const StageDev& S, bool rev) {
+  TileLds m;
+  const int ne = S.nentries;
+  m.X = smem;
+  m.L = rev ? m.X + S.emax : nullptr;
+  m.cdesc = reinterpret_cast<uint32_t*>(m.X + (rev ? 2 : 1) * S.emax);
+  m.ccol = m.cdesc + S.wmax;
+  m.codes = m.ccol + S.wmax;
+  const int n32 = 2 * S.wmax + ne * kTypes;
+  m.cs = reinterpret_cast<double*>(m.cdesc + n32 + (n32 & 1));
+  m.rows = reinterpret_cast<int32_t*>(m.cs + 3 * ne);
+  m.red = reinterpret_cast<double*>(m.rows + 64);
+  return m;
+}
+
+// Tile index of this block: each XCD gets a contiguous range of n tiles.
+__device__ __forceinline__ int xcd_tile(int b, int n, int on) {
+  if (!on) return b;
+  const int x = b & 7, q = b >> 3;
+  const int per = n >> 3, rem = n & 7;
+  return x * per + min(x, rem) + q;
+}
+
+// Tile descriptors into LDS: columns' pair descriptors and canonical indices,
+// alpha rows, entry codes and the entries' cos/sin/scale (identity rotation
+// for |theta| < 1e-28).
+template <int NT>
+__device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile,
+                                               const double* __restrict__ theta, double sgn_th) {
+  const int4 T = S.tiles[S.tile0 + tile];
   const int r0 = S.agoff[T.x];
-  const int R = S.agoff[T.x + 1] - r0;
-  const int W = T.z - T.y;
-  if (threadIdx.x == 0) {
-    t.c0 = T.y;
-    t.W = W;
-    t.R = R;
-    t.E = R * W;
-    t.invW = 1.0f / (float)W;
-    t.gocc = S.agocc[T.x];
-    t.gbit = S.agbit[T.x];
+  TileView t;
+  t.R = S.agoff[T.x + 1] - r0;
+  t.W = T.z - T.y;
+  t.E = t.R * t.W;
+  t.c0 = T.y;
+  t.invW = 1.0f / (float)t.W;
+  t.adesc = S.adesc[T.x];
+  for (int r = threadIdx.x; r < t.R; r += NT) m.rows[r] = S.arows[r0 + r];
+#pragma unroll 4
+  for (int c = threadIdx.x; c < t.W; c += NT) {
+    m.cdesc[c] = S.bdesc[T.y + c];
+    m.ccol[c] = (uint32_t)S.bcol[T.y + c];
   }
-  for (int r = threadIdx.x; r < R; r += blockDim.x) t.rows[r] = S.arows[r0 + r];
-  for (int c = threadIdx.x; c < W; c += blockDim.x) {
-    cdesc[c] = S.bdesc[T.y + c];
-    ccol[c] = (uint16_t)S.bcol[T.y + c];
-  }
-  for (int k = threadIdx.x; k < S.nentries; k += blockDim.x) {
+#pragma unroll 4
+  for (int i = threadIdx.x; i < S.nentries * kTypes; i += NT) m.codes[i] = S.entries[i / kTypes].code[i % kTypes];
+  for (int k = threadIdx.x; k < S.nentries; k += NT) {
     const StageEntry& en = S.entries[k];
     const double th = theta[en.gate];
-    double s = 0.0, c = 1.0;
-    if (fabs(th) >= 1e-28) sincos(sgn_th * en.scale * th, &s, &c);
-    cs_c[k] = c;
-    cs_s[k] = s;
+    double sn = 0.0, c = 1.0;
+    if (fabs(th) >= 1e-28) sincos(sgn_th * en.scale * th, &sn, &c);
+    m.cs[3 * k] = c;
+    m.cs[3 * k + 1] = sn;
+    m.cs[3 * k + 2] = en.scale;
   }
-  for (int i = threadIdx.x; i < S.nentries * kTypes; i += blockDim.x) codes[i] = S.entries[i / kTypes].code[i % kTypes];
   __syncthreads();
+  return t;
+}
+
+// Global <-> LDS copies of a tile (gather through the row and column tables),
+// eight loads in flight per thread.
+template <int NT>
+__device__ __forceinline__ void tile_load(const StageDev& S, const TileView& t, const TileLds& m,
+                                          const double* __restrict__ g, double* X) {
+  constexpr int U = 8;
+  for (int e0 = threadIdx.x; e0 < t.E; e0 += NT * U) {
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NT;
+      if (e < t.E) {
+        const int r = (int)(((float)e + 0.5f) * t.invW);
+        v[u] = g[(size_t)m.rows[r] * S.nsb + m.ccol[e - r * t.W]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + u * NT;
+      if (e < t.E) X[e] = v[u];
+    }
+  }
+}
+
+template <int NT>
+__device__ __forceinline__ void tile_store(const StageDev& S, const TileView& t, const TileLds& m, double* g,
+                                           const double* X) {
+  for (int e = threadIdx.x; e < t.E; e += NT) {
+    const int r = (int)(((float)e + 0.5f) * t.invW);
+    g[(size_t)m.rows[r] * S.nsb + m.ccol[e - r * t.W]] = X[e];
+  }
 }
 
 // Orbit of pair j owned by work item i: member element offsets in the tile
@@ -488,23 +642,24 @@ struct Orbit {
   int type;
 };
 
-__device__ __forceinline__ bool orbit_of(const TileShared& t, const uint64_t* cdesc, int j, int i, Orbit& ob) {
-  const uint32_t abit = (t.gbit >> (4 * j)) & 0xFu;
-  const bool asingle = abit != 0xFu;
+__device__ __forceinline__ bool orbit_of(const TileView& t, const uint32_t* cdesc, int j, int i, Orbit& ob) {
+  const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
+  const bool asingle = (fa & 3u) >= 2u;
+  const int abit = (int)(fa >> 2);
   const int rr = (int)(((float)i + 0.5f) * t.invW);
   const int c = i - rr * t.W;
   const int r0 = asingle ? (((rr >> abit) << (abit + 1)) | (rr & ((1 << abit) - 1))) : rr;
-  const uint64_t d = cdesc[c];
-  const uint32_t bb = (uint32_t)(d >> (4 * j)) & 0xFu;
-  const bool bsingle = bb != 0xFu;
-  if (bsingle && ((d >> (36 + bb)) & 1u)) return false;
-  const int ta = asingle ? 2 : (((t.gocc >> (2 * j)) & 3u) == 0 ? 0 : 1);
-  const int tb = bsingle ? 2 : ((((uint32_t)(d >> (24 + 2 * j))) & 3u) == 0 ? 0 : 1);
+  const uint32_t fb = (cdesc[c] >> (5 * j)) & 31u;
+  const uint32_t kb = fb & 3u;
+  if (kb == 3u) return false;  // upper member of a beta pair
+  const bool bsingle = kb == 2u;
+  const int ta = asingle ? 2 : (int)(fa & 3u);
+  const int tb = bsingle ? 2 : (int)kb;
   ob.type = ta * 3 + tb;
   const int W = t.W;
   ob.m0 = r0 * W + c;
   const int dr = asingle ? (W << abit) : 0;
-  const int dc = bsingle ? (1 << bb) : 0;
+  const int dc = bsingle ? (1 << (fb >> 2)) : 0;
   ob.m1 = asingle ? ob.m0 + dr : -1;
   ob.m2 = bsingle ? ob.m0 + dc : -1;
   ob.m3 = (asingle && bsingle) ? ob.m0 + dr + dc : -1;
@@ -526,7 +681,7 @@ __device__ __forceinline__ void apply_code(uint32_t code, double c, double s, do
   }
 }
 
-// out = T v for one entry's code (T|a> = sg|b>, T|b> = -sg|a>), scaled.
+// out += scale * T v for one entry's code (T|a> = sg|b>, T|b> = -sg|a>).
 __device__ __forceinline__ void t_code(uint32_t code, double scale, const double (&v)[4], double (&out)[4]) {
 #pragma unroll
   for (int sl = 0; sl < 6; ++sl) {
@@ -567,32 +722,30 @@ __device__ __forceinline__ void store_orbit(double* X, const Orbit& ob, const do
   if (ob.m3 >= 0) X[ob.m3] = v[3];
 }
 
+constexpr int kBatch = 4;     // orbits per thread in flight (forward)
+constexpr int kBatchRev = 2;  // same, reverse sweep (two vectors)
+
 // Gates of one stage on one vector (canonical order, in place).
 // dagger: U^+ (pair entries reversed, -theta).  deriv_gate: insert T_g after gate g.
-__global__ __launch_bounds__(kTB) void k_stage_fwd(StageDev S, const double* __restrict__ theta, int dagger,
-                                                   double* __restrict__ psi, int deriv_gate) {
-  __shared__ double X[kTileMax];
-  __shared__ uint64_t cdesc[kTileMaxCols];
-  __shared__ uint16_t ccol[kTileMaxCols];
-  __shared__ double cs_c[kMaxStageGates], cs_s[kMaxStageGates];
-  __shared__ uint32_t codes[kMaxStageGates * kTypes];
-  __shared__ TileShared t;
-  tile_setup(S, t, cdesc, ccol, theta, dagger ? -1.0 : 1.0, cs_c, cs_s, codes);
-  const int W = t.W, E = t.E, nsb = S.nsb;
-  const float invW = t.invW;
-  for (int e = threadIdx.x; e < E; e += kTB) {
-    const int r = (int)(((float)e + 0.5f) * invW);
-    X[e] = psi[(size_t)t.rows[r] * nsb + ccol[e - r * W]];
-  }
+template <int NT>
+__global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ theta, int dagger,
+                                                  double* __restrict__ psi, int deriv_gate) {
+  extern __shared__ double smem[];
+  const TileLds m = carve_lds(smem, S, false);
+  double* X = m.X;
+  const uint32_t* cdesc = m.cdesc;
+  const uint32_t* codes = m.codes;
+  const double* cs = m.cs;
+  const TileView t = tile_setup<NT>(S, m, xcd_tile(blockIdx.x, gridDim.x, S.xcd), theta, dagger ? -1.0 : 1.0);
+  tile_load<NT>(S, t, m, psi, X);
   __syncthreads();
   for (int j = 0; j < S.npairs; ++j) {
     const int k0 = S.pe[j], k1 = S.pe[j + 1];
     const int nent = k1 - k0;
     if (nent == 0) continue;
-    const bool asingle = ((t.gbit >> (4 * j)) & 0xFu) != 0xFu;
-    const int items = asingle ? E >> 1 : E;
-    // derivative insertion: after the last entry of deriv_gate on this pair
-    int dq0 = -1, dq1 = -1;
+    const bool asingle = ((t.adesc >> (5 * j)) & 3u) >= 2u;
+    const int items = asingle ? t.E >> 1 : t.E;
+    int dq0 = -1, dq1 = -1;  // entries of deriv_gate on this pair
     if (deriv_gate >= 0) {
       for (int k = k0; k < k1; ++k)
         if (S.entries[k].gate == deriv_gate) {
@@ -600,110 +753,139 @@ __global__ __launch_bounds__(kTB) void k_stage_fwd(StageDev S, const double* __r
           dq1 = k + 1;
         }
     }
-    for (int i = threadIdx.x; i < items; i += kTB) {
-      Orbit ob;
-      if (!orbit_of(t, cdesc, j, i, ob)) continue;
-      double v[4];
-      load_orbit(X, ob, v);
+    for (int i0 = threadIdx.x; i0 < items; i0 += NT * kBatch) {
+      Orbit ob[kBatch];
+      bool ok[kBatch];
+      double v[kBatch][4];
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b) {
+        const int i = i0 + b * NT;
+        ok[b] = i < items && orbit_of(t, cdesc, j, i, ob[b]);
+        if (ok[b]) load_orbit(X, ob[b], v[b]);
+      }
 #pragma unroll
       for (int kk = 0; kk < kMaxPairEntries; ++kk) {
         if (kk < nent) {
           const int k = dagger ? k1 - 1 - kk : k0 + kk;
-          apply_code(codes[k * kTypes + ob.type], cs_c[k], cs_s[k], v);
-          if (k + 1 == dq1) {
-            double o[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int q = dq0; q < dq1; ++q) t_code(codes[q * kTypes + ob.type], S.entries[q].scale, v, o);
+          const double c = cs[3 * k], sn = cs[3 * k + 1];
 #pragma unroll
-            for (int m = 0; m < 4; ++m) v[m] = o[m];
+          for (int b = 0; b < kBatch; ++b)
+            if (ok[b]) apply_code(codes[k * kTypes + ob[b].type], c, sn, v[b]);
+          if (k + 1 == dq1) {
+#pragma unroll
+            for (int b = 0; b < kBatch; ++b) {
+              if (!ok[b]) continue;
+              double o[4] = {0.0, 0.0, 0.0, 0.0};
+              for (int q = dq0; q < dq1; ++q) t_code(codes[q * kTypes + ob[b].type], cs[3 * q + 2], v[b], o);
+#pragma unroll
+              for (int m = 0; m < 4; ++m) v[b][m] = o[m];
+            }
           }
         }
       }
-      store_orbit(X, ob, v);
+#pragma unroll
+      for (int b = 0; b < kBatch; ++b)
+        if (ok[b]) store_orbit(X, ob[b], v[b]);
     }
     __syncthreads();
   }
-  for (int e = threadIdx.x; e < E; e += kTB) {
-    const int r = (int)(((float)e + 0.5f) * invW);
-    psi[(size_t)t.rows[r] * nsb + ccol[e - r * W]] = X[e];
-  }
+  tile_store<NT>(S, t, m, psi, X);
 }
 
 // Reverse sweep of one stage on (psi, lam), in place, with per-entry partial
 // dots written to partial[partial_off + tile * nentries + k].
-__global__ __launch_bounds__(kTB) void k_stage_rev(StageDev S, const double* __restrict__ theta,
-                                                   double* __restrict__ psi, double* __restrict__ lam,
-                                                   double* __restrict__ partial, int write_back) {
-  __shared__ double X[kTileMax];
-  __shared__ double L[kTileMax];
-  __shared__ uint64_t cdesc[kTileMaxCols];
-  __shared__ uint16_t ccol[kTileMaxCols];
-  __shared__ double cs_c[kMaxStageGates], cs_s[kMaxStageGates];
-  __shared__ uint32_t codes[kMaxStageGates * kTypes];
-  __shared__ double red[kMaxStageGates][kTB / 64];
-  __shared__ TileShared t;
-  tile_setup(S, t, cdesc, ccol, theta, -1.0, cs_c, cs_s, codes);
-  const int W = t.W, E = t.E, nsb = S.nsb;
-  const float invW = t.invW;
-  for (int e = threadIdx.x; e < E; e += kTB) {
-    const int r = (int)(((float)e + 0.5f) * invW);
-    const size_t gi = (size_t)t.rows[r] * nsb + ccol[e - r * W];
-    X[e] = psi[gi];
-    L[e] = lam[gi];
-  }
+template <int NT>
+__global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __restrict__ theta,
+                                                  double* __restrict__ psi, double* __restrict__ lam,
+                                                  double* __restrict__ partial, int write_back) {
+  extern __shared__ double smem[];
+  const int ne = S.nentries;
+  const TileLds m = carve_lds(smem, S, true);
+  double* X = m.X;
+  double* L = m.L;
+  const uint32_t* cdesc = m.cdesc;
+  const uint32_t* codes = m.codes;
+  const double* cs = m.cs;
+  double* red = m.red;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x, S.xcd);
+  const TileView t = tile_setup<NT>(S, m, tile, theta, -1.0);
+  tile_load<NT>(S, t, m, psi, X);
+  tile_load<NT>(S, t, m, lam, L);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const size_t pbase = S.partial_off + (size_t)(S.tile0 + tile) * ne;
   for (int j = 0; j < S.npairs; ++j) {
     const int k0 = S.pe[j], k1 = S.pe[j + 1];
     const int nent = k1 - k0;
     if (nent == 0) continue;
-    const bool asingle = ((t.gbit >> (4 * j)) & 0xFu) != 0xFu;
-    const int items = asingle ? E >> 1 : E;
+    const bool asingle = ((t.adesc >> (5 * j)) & 3u) >= 2u;
+    const int items = asingle ? t.E >> 1 : t.E;
     double dots[kMaxPairEntries];
 #pragma unroll
     for (int kk = 0; kk < kMaxPairEntries; ++kk) dots[kk] = 0.0;
-    for (int i = threadIdx.x; i < items; i += kTB) {
-      Orbit ob;
-      if (!orbit_of(t, cdesc, j, i, ob)) continue;
-      double v[4], w[4];
-      load_orbit(X, ob, v);
-      load_orbit(L, ob, w);
+    for (int i0 = threadIdx.x; i0 < items; i0 += NT * kBatchRev) {
+      Orbit ob[kBatchRev];
+      bool ok[kBatchRev];
+      double v[kBatchRev][4], w[kBatchRev][4];
+#pragma unroll
+      for (int b = 0; b < kBatchRev; ++b) {
+        const int i = i0 + b * NT;
+        ok[b] = i < items && orbit_of(t, cdesc, j, i, ob[b]);
+        if (ok[b]) {
+          load_orbit(X, ob[b], v[b]);
+          load_orbit(L, ob[b], w[b]);
+        }
+      }
 #pragma unroll
       for (int kk = 0; kk < kMaxPairEntries; ++kk) {
         if (kk < nent) {
           const int k = k1 - 1 - kk;
-          const uint32_t code = codes[k * kTypes + ob.type];
-          dots[kk] += dot_code(code, v, w);
-          apply_code(code, cs_c[k], cs_s[k], v);
-          apply_code(code, cs_c[k], cs_s[k], w);
+          const double c = cs[3 * k], sn = cs[3 * k + 1];
+#pragma unroll
+          for (int b = 0; b < kBatchRev; ++b) {
+            if (!ok[b]) continue;
+            const uint32_t code = codes[k * kTypes + ob[b].type];
+            dots[kk] += dot_code(code, v[b], w[b]);
+            apply_code(code, c, sn, v[b]);
+            apply_code(code, c, sn, w[b]);
+          }
         }
       }
       if (write_back) {
-        store_orbit(X, ob, v);
-        store_orbit(L, ob, w);
+#pragma unroll
+        for (int b = 0; b < kBatchRev; ++b) {
+          if (!ok[b]) continue;
+          store_orbit(X, ob[b], v[b]);
+          store_orbit(L, ob[b], w[b]);
+        }
       }
     }
 #pragma unroll
     for (int kk = 0; kk < kMaxPairEntries; ++kk) {
       if (kk < nent) {
+        const int k = k1 - 1 - kk;
         double d = dots[kk];
         for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
-        if (lane == 0) red[k1 - 1 - kk][wave] = d * S.entries[k1 - 1 - kk].scale;
+        d *= cs[3 * k + 2];
+        if (NT == 64) {
+          if (lane == 0) partial[pbase + k] = d;
+        } else if (lane == 0) {
+          red[k * (NT / 64) + wave] = d;
+        }
       }
     }
     __syncthreads();
   }
-  for (int k = threadIdx.x; k < S.nentries; k += kTB) {
-    double v = 0.0;
-    for (int w = 0; w < kTB / 64; ++w) v += red[k][w];
-    partial[S.partial_off + (size_t)blockIdx.x * S.nentries + k] = v;
+  if (NT > 64) {
+    for (int k = threadIdx.x; k < ne; k += NT) {
+      double v = 0.0;
+      for (int w = 0; w < NT / 64; ++w) v += red[k * (NT / 64) + w];
+      partial[pbase + k] = v;
+    }
   }
   if (!write_back) return;  // the first stage of the program: the un-applied states are not needed
-  for (int e = threadIdx.x; e < E; e += kTB) {
-    const int r = (int)(((float)e + 0.5f) * invW);
-    const size_t gi = (size_t)t.rows[r] * nsb + ccol[e - r * W];
-    psi[gi] = X[e];
-    lam[gi] = L[e];
-  }
+  tile_store<NT>(S, t, m, psi, X);
+  tile_store<NT>(S, t, m, lam, L);
 }
 
 // grad[gate] = 2 * sum over the stage's tiles of the gate's entry partials;
@@ -730,6 +912,92 @@ __global__ void k_tile_grad(const double* __restrict__ partial, const GateRed* _
 
 static double stage_bytes(const Space& s, int vectors) { return 16.0 * (double)s.dim * vectors; }
 
+// Large tiles run on a side stream concurrently with the small-tile launch
+// (they touch disjoint elements): fork/join through events, so the few
+// large blocks overlap the bulk instead of forming a tail.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
+};
+
+static SideStream& side_stream() {
+  static std::mutex mu;
+  static SideStream per_dev[64];
+  int dev = 0;
+  SQ_HIP(hipGetDevice(&dev));
+  SQ_CHECK(dev >= 0 && dev < 64, SQ_ERR_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(mu);
+  SideStream& S = per_dev[dev];
+  if (!S.s) {
+    SQ_HIP(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
+    SQ_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
+    SQ_HIP(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
+  }
+  return S;
+}
+
+template <class SmallL, class LargeL>
+static void fork_join(const Stage& S, hipStream_t st, SmallL small, LargeL large) {
+  if (S.ntiles > S.nsmall && S.nsmall > 0) {
+    SideStream& side = side_stream();
+    SQ_HIP(hipEventRecord(side.fork, st));
+    SQ_HIP(hipStreamWaitEvent(side.s, side.fork, 0));
+    large(side.s);
+    SQ_HIP(hipGetLastError());
+    SQ_HIP(hipEventRecord(side.join, side.s));
+    small(st);
+    SQ_HIP(hipStreamWaitEvent(st, side.join, 0));
+  } else if (S.nsmall > 0) {
+    small(st);
+  } else if (S.ntiles > 0) {
+    large(st);
+  }
+}
+
+// Dynamic LDS above 64 KB (large-tile reverse sweep) must be enabled per kernel.
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+  static std::mutex mu;
+  static size_t allowed = 64 * 1024;
+  SQ_CHECK(bytes <= 160 * 1024, SQ_ERR_ARG, "tile needs more LDS than a CU has");
+  std::lock_guard<std::mutex> lock(mu);
+  if (bytes > allowed) {
+    SQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               160 * 1024));
+    allowed = 160 * 1024;
+  }
+}
+
+static void launch_fwd(const Stage& S, const double* theta, bool dagger, double* psi, int dg, hipStream_t st) {
+  const int ne = S.small.nentries;
+  fork_join(
+      S, st,
+      [&](hipStream_t q) {
+        k_stage_fwd<64><<<S.nsmall, 64, lds_bytes(64, S.small.emax, S.small.wmax, ne, false), q>>>(
+            S.small, theta, dagger ? 1 : 0, psi, dg);
+      },
+      [&](hipStream_t q) {
+        k_stage_fwd<kTB><<<S.ntiles - S.nsmall, kTB, lds_bytes(kTB, S.large.emax, S.large.wmax, ne, false), q>>>(
+            S.large, theta, dagger ? 1 : 0, psi, dg);
+      });
+}
+
+static void launch_rev(const Stage& S, const double* theta, double* psi, double* lam, double* partial, int wb,
+                       hipStream_t st) {
+  const int ne = S.small.nentries;
+  fork_join(
+      S, st,
+      [&](hipStream_t q) {
+        k_stage_rev<64><<<S.nsmall, 64, lds_bytes(64, S.small.emax, S.small.wmax, ne, true), q>>>(
+            S.small, theta, psi, lam, partial, wb);
+      },
+      [&](hipStream_t q) {
+        const size_t lds = lds_bytes(kTB, S.large.emax, S.large.wmax, ne, true);
+        allow_lds(k_stage_rev<kTB>, lds);
+        k_stage_rev<kTB><<<S.ntiles - S.nsmall, kTB, lds, q>>>(S.large, theta, psi, lam, partial, wb);
+      });
+}
+
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
 // program, tiles where possible, in place on psi (canonical order).
 // scratch >= dim doubles (derivative insertion on per-gate stages);
@@ -750,8 +1018,7 @@ void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const do
       continue;
     }
     const int dg = (deriv_gate >= S.g0 && deriv_gate < S.g1) ? deriv_gate : -1;
-    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0,
-            k_stage_fwd<<<S.dev.ntiles, kTB, 0, st>>>(S.dev, theta, dagger ? 1 : 0, psi, dg));
+    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, theta, dagger, psi, dg, st));
   }
 }
 
@@ -768,8 +1035,7 @@ void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const do
       continue;
     }
     const int wb = i > 0 ? 1 : 0;
-    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0,
-            k_stage_rev<<<S.dev.ntiles, kTB, 0, st>>>(S.dev, theta, psi, lam, T.d_partial, wb));
+    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(S, theta, psi, lam, T.d_partial, wb, st));
   }
   if (T.nred > 0)
     SQ_PROF(PC_REDUCE, st, 8.0 * (double)T.partial_len, 0.0,
