@@ -66,6 +66,24 @@ def time_steps(fn, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "tups12_traffic.txt")
+
+
+def measured_traffic(kernel_class):
+    """HBM bytes per launch of a kernel class, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (lines
+    'traffic <class> <bytes>'), or None when it has not been measured."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                parts = line.split()
+                if len(parts) >= 3 and parts[0] == "traffic" and parts[1] == kernel_class:
+                    return float(parts[2])
+    except OSError:
+        pass
+    return None
+
+
 def kernel_profile(sp, prog, H, theta, psi0, reps):
     """Average per-launch duration of each kernel class via HIP events."""
     from slowquant_amd import profiling
@@ -144,6 +162,51 @@ def cpu_baseline(h, g, theta, lay, budget_s=20.0):
     }
 
 
+def bench_batch(args) -> dict:
+    """512 independent tUPS instances over (4e,4o)..(10e,10o), energy + adjoint
+    gradient each: one persistent launch replayed from a hipGraph vs a loop of
+    per-instance engine calls."""
+    import torch
+
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, draw_specs, loop_energy_grad
+
+    specs = draw_specs(args.instances)
+    inst, thetas = build_instances(specs)
+    th_all = np.concatenate(thetas)
+    b = E.autotune_batch(inst, th_all)
+    e_b, g_b = b.run(thetas, graph=True)
+    e_l, g_l = loop_energy_grad(inst, thetas)
+    diff = max(float(np.abs(e_b - e_l).max()), max(float(np.abs(x - y).max()) for x, y in zip(g_b, g_l)))
+
+    def batch_step():
+        b.launch(graph=True)
+
+    t_batch = time_steps(batch_step, args.steps, args.warmup)
+    t_loop = time_steps(lambda: loop_energy_grad(inst, thetas), max(1, args.steps // 5), 1)
+    return {
+        "metric": "batched energy+gradient instances/s, 512 tUPS instances (4e,4o)..(10e,10o) FP64",
+        "value": len(inst) / t_batch,
+        "unit": "instances/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_batch,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{len(inst)} tUPS instances, spaces uniform over (4e,4o),(6e,6o),(8e,8o),(10e,10o), "
+                               "1-3 layers, rank-2n synthetic h/g, theta~N(0,0.1^2), one hipGraph-replayed launch",
+                   "mean_ci_dim": float(np.mean([s.dim for s, _, _ in inst])),
+                   "workgroups": b.n_workgroups},
+        "loop": {"value": len(inst) / t_loop, "unit": "instances/s", "ms_per_step": 1e3 * t_loop},
+        "batch_over_loop": t_loop / t_batch,
+        "max_diff_batch_vs_loop": diff,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -152,6 +215,9 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--workload", choices=("tups12", "batch512"), default="tups12",
+                    help="tups12: the headline (12e,12o) tUPS metric; batch512: 512 batched instances")
+    ap.add_argument("--instances", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -162,6 +228,9 @@ def main() -> int:
         return 2
     import torch
 
+    if args.workload == "batch512":
+        print(json.dumps(bench_batch(args)), flush=True)
+        return 0
     lay, h, g, theta, sp, prog, H, psi0 = build_workload()
     th = np.asarray(theta)
 
@@ -183,7 +252,7 @@ def main() -> int:
         "peak": dom["peak"],
         "unit": dom["unit"],
         "frac": dom["achieved"] / dom["peak"],
-        "traffic": None,
+        "traffic": measured_traffic(dom["name"]),
         "kernel": dom["name"],
     }
     out = {

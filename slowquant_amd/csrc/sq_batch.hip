@@ -45,17 +45,23 @@ struct BatchInstance {
 
 // One rotation half: string G = sgn * Ga * Gb with distinct indices; the
 // layout gate's theta index and angle scale.
+// A rotation string touches exactly the (alpha, beta) pairs where both spin
+// parts apply: precomputed lists of (string, image | parity << 31) per spin
+// (shared between rotations with the same spin part on the same space); a
+// spin part without operators acts on every string (list offset -1).
 struct BatchRot {
-  SpinOps a, b;
   double sgn, scale;
   int theta_idx;    // index relative to the instance's theta0
   int grad_idx;     // same, for the gradient accumulation
+  int aoff, na;     // alpha list [aoff, aoff + na), or aoff = -1: every alpha string
+  int boff, nb;     // beta list, likewise
 };
 
 struct BatchDev {
   const BatchSpace* spaces;
   const BatchInstance* inst;
   const BatchRot* rots;
+  const int2* lists;       // (string, image | parity << 31) entries of the spin-part lists
   const double* theta;
   const double* A;
   const int32_t* wg_list;  // concatenated instance lists per workgroup
@@ -67,19 +73,31 @@ struct BatchDev {
   double* grad;            // concatenated gradients
 };
 
-__device__ __forceinline__ bool bapply(const SpinOps& o, uint32_t& m, int& par) {
-  for (int j = o.n - 1; j >= 0; --j) {
-    uint32_t bit = 1u << o.orb[j];
-    par += __popc(m & (bit - 1u));
-    if (o.cre[j]) {
-      if (m & bit) return false;
-      m |= bit;
-    } else {
-      if (!(m & bit)) return false;
-      m &= ~bit;
-    }
-  }
-  return true;
+// Visit every (alpha row, beta column) of a space with 32-bit index math:
+// a block-wide 2-D walk, rows_per_step rows of min(nsb, blockDim) columns.
+template <class F>
+__device__ __forceinline__ void for_each_det(int nsa, int nsb, F&& f) {
+  const int cols = nsb < (int)blockDim.x ? nsb : (int)blockDim.x;
+  const int rps = (int)blockDim.x / cols;
+  const int r0 = (int)threadIdx.x / cols, c0 = (int)threadIdx.x - r0 * cols;
+  if (r0 >= rps) return;
+  for (int ia = r0; ia < nsa; ia += rps)
+    for (int ib = c0; ib < nsb; ib += cols) f(ia, ib, ia * nsb + ib);
+}
+
+// Visit every (alpha entry, beta entry) of a rotation's lists.
+template <class F>
+__device__ __forceinline__ void for_each_pair(int na, int nb, F&& f) {
+  const int cols = nb < (int)blockDim.x ? (nb > 0 ? nb : 1) : (int)blockDim.x;
+  const int rps = (int)blockDim.x / cols;
+  const int r0 = (int)threadIdx.x / cols, c0 = (int)threadIdx.x - r0 * cols;
+  if (r0 >= rps) return;
+  for (int ai = r0; ai < na; ai += rps)
+    for (int bi = c0; bi < nb; bi += cols) f(ai, bi);
+}
+
+__device__ __forceinline__ int2 list_entry(const int2* lists, int off, int i) {
+  return off < 0 ? int2{i, i} : lists[off + i];
 }
 
 __device__ double block_reduce(double v, double* red) {
@@ -125,23 +143,20 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       if (fabs(t0) >= 1e-28) {
         double s, c;
         sincos(R.scale * t0, &s, &c);
-        for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
-          const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
-          uint32_t a = S.astr[ia], b = S.bstr[ib];
-          int par = 0;
-          if (!bapply(R.a, a, par) || !bapply(R.b, b, par)) continue;
-          const double sg = ((par & 1) ? -1.0 : 1.0) * R.sgn;
-          const size_t J = (size_t)S.arank[a] * nsb + S.brank[b];
+        for_each_pair(R.na, R.nb, [&](int ai, int bi) {
+          const int2 A = list_entry(B.lists, R.aoff, ai), Bq = list_entry(B.lists, R.boff, bi);
+          const int x = A.x * nsb + Bq.x;
+          const int J = (A.y & 0x7fffffff) * nsb + (Bq.y & 0x7fffffff);
+          const double sg = (((uint32_t)(A.y ^ Bq.y) >> 31) ? -1.0 : 1.0) * R.sgn;
           const double pd = psi[x], pj = psi[J];
           psi[x] = c * pd - sg * s * pj;
           psi[J] = sg * s * pd + c * pj;
-        }
+        });
       }
       __syncthreads();
     }
     // D build (rows k < npair: E+_k psi; row npair: psi; pad rows: 0)
-    for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
-      const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
+    for_each_det(S.nsa, nsb, [&](int ia, int ib, int x) {
       const double px = psi[x];
       int k = 0;
       for (int p = 0; p < S.n; ++p) {
@@ -161,7 +176,7 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       }
       D[(size_t)S.npair * dim + x] = px;
       for (int k2 = S.npair + 1; k2 < S.K4; ++k2) D[(size_t)k2 * dim + x] = 0.0;
-    }
+    });
     // stage A (K4 x kpad, transposed coefficient matrix) in LDS
     const double* Ag = B.A + I.A_off;
     for (int i = threadIdx.x; i < S.K4 * S.kpad; i += blockDim.x) sA[i] = Ag[i];
@@ -194,8 +209,7 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
     __syncthreads();
     // sigma gather into lam, energy
     double e = 0.0;
-    for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
-      const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
+    for_each_det(S.nsa, nsb, [&](int ia, int ib, int x) {
       double v = I.e_core * psi[x];
       int k = 0;
       for (int p = 0; p < S.n; ++p) {
@@ -213,7 +227,7 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       }
       lam[x] = v;
       e += v * psi[x];
-    }
+    });
     const double energy = block_reduce(e, red);
     if (threadIdx.x == 0) B.energy[B.wg_list[li]] = energy;
     // reverse sweep with gradient dots
@@ -226,13 +240,11 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       double s, c;
       sincos(-R.scale * t0, &s, &c);
       double dot = 0.0;
-      for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) {
-        const int ia = (int)(x / nsb), ib = (int)(x - (int64_t)ia * nsb);
-        uint32_t a = S.astr[ia], b = S.bstr[ib];
-        int par = 0;
-        if (!bapply(R.a, a, par) || !bapply(R.b, b, par)) continue;
-        const double sg = ((par & 1) ? -1.0 : 1.0) * R.sgn;
-        const size_t J = (size_t)S.arank[a] * nsb + S.brank[b];
+      for_each_pair(R.na, R.nb, [&](int ai, int bi) {
+        const int2 A = list_entry(B.lists, R.aoff, ai), Bq = list_entry(B.lists, R.boff, bi);
+        const int x = A.x * nsb + Bq.x;
+        const int J = (A.y & 0x7fffffff) * nsb + (Bq.y & 0x7fffffff);
+        const double sg = (((uint32_t)(A.y ^ Bq.y) >> 31) ? -1.0 : 1.0) * R.sgn;
         const double pd = psi[x], pj = psi[J], ld = lam[x], lj = lam[J];
         dot += R.scale * sg * (lj * pd - ld * pj);
         if (!skip) {
@@ -241,7 +253,7 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
           lam[x] = c * ld - sg * s * lj;
           lam[J] = sg * s * ld + c * lj;
         }
-      }
+      });
       const double tot = block_reduce(dot, red);
       if (threadIdx.x == 0) B.grad[I.out0 + R.grad_idx] += 2.0 * tot;
       __syncthreads();
@@ -295,6 +307,48 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
     Batch& B = bo->b;
     try {
       std::map<const sq_space*, int> sp_index;
+      std::vector<int2> lists;
+      std::map<std::pair<const void*, std::vector<int>>, std::pair<int, int>> list_cache;
+      // (string, image | parity << 31) of every string a spin part applies to
+      auto spin_list = [&](const std::vector<uint32_t>& strs, const SpinOps& o, int& off, int& len) {
+        if (o.n == 0) {
+          off = -1;
+          len = (int)strs.size();
+          return;
+        }
+        std::vector<int> key{o.n};
+        for (int j = 0; j < o.n; ++j) key.push_back(o.orb[j] * 2 + (o.cre[j] ? 1 : 0));
+        auto ck = std::make_pair((const void*)&strs, key);
+        auto it = list_cache.find(ck);
+        if (it != list_cache.end()) {
+          off = it->second.first;
+          len = it->second.second;
+          return;
+        }
+        std::map<uint32_t, int> rank;
+        for (int i = 0; i < (int)strs.size(); ++i) rank[strs[i]] = i;
+        off = (int)lists.size();
+        for (int i = 0; i < (int)strs.size(); ++i) {
+          uint32_t m = strs[i];
+          int par = 0;
+          bool ok = true;
+          for (int j = o.n - 1; j >= 0 && ok; --j) {
+            const uint32_t bit = 1u << o.orb[j];
+            par += __builtin_popcount(m & (bit - 1u));
+            if (o.cre[j]) {
+              if (m & bit) ok = false;
+              m |= bit;
+            } else {
+              if (!(m & bit)) ok = false;
+              m &= ~bit;
+            }
+          }
+          if (!ok) continue;
+          lists.push_back(int2{i, rank.at(m) | (int)((uint32_t)(par & 1) << 31)});
+        }
+        len = (int)lists.size() - off;
+        list_cache[ck] = {off, len};
+      };
       std::vector<BatchSpace> bsp;
       std::vector<BatchRot> rots;
       std::vector<double> Astore;
@@ -305,6 +359,7 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
         const Program& p = programs[i]->p;
         const Hamiltonian& H = hams[i]->h;
         SQ_CHECK(p.space == &s && H.space == &s, SQ_ERR_ARG, "program / Hamiltonian built for another space");
+        SQ_CHECK(s.dim < (int64_t)1 << 30, SQ_ERR_ARG, "batched spaces must have fewer than 2^30 determinants");
         if (!sp_index.count(spaces[i])) {
           sp_index[spaces[i]] = (int)bsp.size();
           BatchSpace b{};
@@ -334,8 +389,8 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
           for (int h = 0; h < halves; ++h) {
             const Term& T = p.terms[gd.term_begin + 2 * h];
             BatchRot R{};
-            R.a = T.a;
-            R.b = T.b;
+            spin_list(s.h_astr, T.a, R.aoff, R.na);
+            spin_list(s.h_bstr, T.b, R.boff, R.nb);
             R.sgn = T.coef < 0 ? -1.0 : 1.0;
             R.scale = halves == 2 ? 1.0 : gd.scale;
             R.theta_idx = g;
@@ -370,15 +425,15 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       std::vector<int> order(n_inst);
       std::iota(order.begin(), order.end(), 0);
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-      std::vector<std::vector<int32_t>> lists(nwg);
+      std::vector<std::vector<int32_t>> wg_inst(nwg);
       std::vector<int64_t> load(nwg, 0);
       for (int i : order) {
         int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        lists[w].push_back(i);
+        wg_inst[w].push_back(i);
         load[w] += cost[i];
       }
       std::vector<int32_t> wl, wo{0};
-      for (auto& l : lists) {
+      for (auto& l : wg_inst) {
         wl.insert(wl.end(), l.begin(), l.end());
         wo.push_back((int32_t)wl.size());
       }
@@ -386,6 +441,7 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       B.dev.spaces = bupload(B, bsp);
       B.dev.inst = bupload(B, B.inst);
       B.dev.rots = bupload(B, rots);
+      B.dev.lists = bupload(B, lists);
       B.dev.A = bupload(B, Astore);
       B.dev.wg_list = bupload(B, wl);
       B.dev.wg_off = bupload(B, wo);

@@ -184,6 +184,7 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
                                                     int nsb) {
   extern __shared__ double sA[];
   constexpr int W = MT * 16;
+  constexpr int DEPTH = 1;  // B operands in flight ahead of the MFMA step that uses them (4 measured slower)
   const int m0 = blockIdx.y * W;
   for (int i = threadIdx.x; i < K4 * W; i += blockDim.x) {
     int k = i / W, m = m0 + i % W;
@@ -214,13 +215,21 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
   double4_t acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-  double bnext = dval(kq);
-  for (int k4 = 0; k4 < K4; k4 += 4) {
-    const double b = bnext;
-    if (k4 + 4 < K4) bnext = dval(k4 + 4 + kq);
-    const double* arow = sA + (size_t)(k4 + kq) * W + (lane & 15);
+  double q[DEPTH];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+  for (int d = 0; d < DEPTH; ++d) q[d] = dval(4 * d + kq);
+  for (int k4 = 0; k4 < K4; k4 += 4 * DEPTH) {
+#pragma unroll
+    for (int d = 0; d < DEPTH; ++d) {
+      const int ks = k4 + 4 * d;
+      if (ks < K4) {
+        const double b = q[d];
+        q[d] = dval(ks + 4 * DEPTH + kq);
+        const double* arow = sA + (size_t)(ks + kq) * W + (lane & 15);
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+      }
+    }
   }
   if (!valid) return;
 #pragma unroll

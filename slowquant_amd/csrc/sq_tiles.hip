@@ -65,25 +65,53 @@ struct StageEntry {
   uint32_t code[kTypes];
 };
 
+// Per pair of a stage, the gates act on orbits of <= 4 amplitudes (member
+// o = alpha bit | beta bit << 1).  Their combined action is one matrix per
+// orbit type, stored compactly in kPairVals doubles:
+//   [0..15]  type (alpha single, beta single): 4x4, row-major M[a][b]
+//   [16..19] type (alpha single, beta empty):  2x2 on members {0, 1}
+//   [20..23] type (alpha single, beta double): 2x2 on members {0, 1}
+//   [24..27] type (alpha empty, beta single):  2x2 on members {0, 2}
+//   [28..31] type (alpha double, beta single): 2x2 on members {0, 2}
+// (types without a single pair are left alone by every gate).  A gate's
+// gradient is sum over orbits of <lam|W T W^T|psi> with W the product of the
+// pair's later gates; W T W^T is antisymmetric, so the reverse sweep only
+// accumulates Q[a<b] = sum (lam_a psi_b - lam_b psi_a):
+//   Q[0..5]  type (single, single): pairs (0,1),(0,2),(0,3),(1,2),(1,3),(2,3)
+//   Q[6..9]  the four two-member types, (member 0, second member)
+constexpr int kPairVals = 32;
+constexpr int kQVals = 10;
+// Pair matrices carry one more value, [32]: the factor for orbits of types
+// without a single pair (1, or 0 when a derivative T_g is inserted on the
+// pair: T_g annihilates them); padded to 36.
+constexpr int kMatVals = 36;
+
 // Per pair j of a stage, a 5-bit field (bits 5j..5j+4) of a string's
 // descriptor: kind (2 bits: 0 pair empty, 1 pair doubly occupied, 2 singly
 // occupied with hypercube bit 0, 3 singly occupied with hypercube bit 1) and
 // the hypercube bit index (3 bits).
 struct StageDev {
   int npairs;
-  int nentries;
   int nsb;
   int tile0;                 // first tile of this launch (small tiles, then large ones)
   int xcd;                   // 1: XCD-contiguous tile ranges (SQ_XCD_REMAP=0 turns it off)
   int emax, wmax;            // largest tile of this launch: elements, columns
-  int pe[kMaxPairs + 1];     // entries of pair j: [pe[j], pe[j+1]) (entries sorted by pair)
-  int64_t partial_off;       // this stage's block of the partial-dot buffer
+  int ntiles_stage;          // tiles of the whole stage (small and large)
+  int64_t poff;              // this stage's block of the gradient partials
+  const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
   const int32_t* agoff;      // [ngroups+1]
   const uint32_t* adesc;     // per alpha group: descriptor (kind 2 for every single pair)
   const uint32_t* bdesc;     // per beta stage position: descriptor
   const int32_t* bcol;       // per beta stage position: canonical beta column
   const int4* tiles;         // (alpha group, col0, col1, -)
+};
+
+// Per tiled stage, what the matrix and gradient kernels need.
+struct StageInfo {
+  int npairs, nentries, ntiles, pad;
+  int pe[kMaxPairs + 1];     // entries of pair j: [pe[j], pe[j+1]) (entries sorted by pair)
+  int64_t poff;              // gradient partials: [npairs][kQVals][ntiles]
   const StageEntry* entries;
 };
 
@@ -93,27 +121,27 @@ struct Stage {
   int npairs = 0;
   int pp[kMaxPairs];
   std::vector<StageEntry> entries;
+  int pe[kMaxPairs + 1];
   int ntiles = 0, nsmall = 0;  // tiles [0, nsmall) run one wave each, the rest 256 threads
+  int index = -1;              // position among the tiled stages
   StageDev small{}, large{};   // launch descriptors
+  const StageEntry* d_entries = nullptr;
   std::vector<void*> allocs;
-};
-
-struct GateRed {
-  int64_t off;  // partial_off of the gate's stage
-  int32_t ntiles, stride, k0, k1, gate, pad;
 };
 
 struct TilePlan {
   std::vector<Stage> stages;
-  double* d_partial = nullptr;
+  int ntiled = 0;
+  double* d_partial = nullptr;  // gradient partials of all tiled stages
   int64_t partial_len = 0;
-  GateRed* d_red = nullptr;
-  int nred = 0;
+  double* d_mats = nullptr;     // [ntiled][kMaxPairs][kMatVals]
+  StageInfo* d_info = nullptr;  // [ntiled]
   ~TilePlan() {
     for (auto& s : stages)
       for (void* p : s.allocs) (void)hipFree(p);
     if (d_partial) (void)hipFree(d_partial);
-    if (d_red) (void)hipFree(d_red);
+    if (d_mats) (void)hipFree(d_mats);
+    if (d_info) (void)hipFree(d_info);
   }
 };
 
@@ -322,8 +350,8 @@ static void finalize_stage(const Space& sp, Stage& S) {
   // entries grouped by pair, program order kept within a pair
   std::stable_sort(S.entries.begin(), S.entries.end(),
                    [](const StageEntry& a, const StageEntry& b) { return a.pair < b.pair; });
-  int pe[kMaxPairs + 1];
-  for (int j = 0, k = 0; j <= S.npairs; ++j) {
+  int* pe = S.pe;
+  for (int j = 0, k = 0; j <= kMaxPairs; ++j) {
     while (k < (int)S.entries.size() && S.entries[k].pair < j) ++k;
     pe[j] = k;
   }
@@ -371,22 +399,22 @@ static void finalize_stage(const Space& sp, Stage& S) {
   for (int k = 0; k < sp.nsb; ++k) bdesc[k] = pair_desc(B.occ[k], B.bit[k], S.npairs);
   StageDev d{};
   d.npairs = S.npairs;
-  d.nentries = (int)S.entries.size();
   d.nsb = sp.nsb;
-  for (int j = 0; j <= kMaxPairs; ++j) d.pe[j] = j <= S.npairs ? pe[j] : pe[S.npairs];
   d.arows = upload(S.allocs, A.order);
   d.agoff = upload(S.allocs, A.goff);
   d.adesc = upload(S.allocs, adesc);
   d.bdesc = upload(S.allocs, bdesc);
   d.bcol = upload(S.allocs, B.order);
   d.tiles = upload(S.allocs, tiles);
-  d.entries = upload(S.allocs, S.entries);
+  S.d_entries = upload(S.allocs, S.entries);
   const char* xenv = getenv("SQ_XCD_REMAP");
   d.xcd = (xenv && xenv[0] == '0') ? 0 : 1;
   S.small = d;
   S.large = d;
   S.small.tile0 = 0;
   S.large.tile0 = S.nsmall;
+  S.small.ntiles_stage = S.ntiles;
+  S.large.ntiles_stage = S.ntiles;
   auto extent = [&](int t0, int t1, StageDev& D) {
     D.emax = 0;
     D.wmax = 0;
@@ -450,26 +478,34 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
       T->stages.push_back(std::move(S));
     }
     int64_t off = 0;
-    std::vector<GateRed> red;
+    std::vector<StageInfo> info;
     for (auto& S : T->stages) {
       if (!S.tiled) continue;
       finalize_stage(sp, S);
-      S.small.partial_off = off;
-      S.large.partial_off = off;
-      const int ne = S.small.nentries;
-      for (int k = 0; k < ne;) {
-        int k1 = k + 1;
-        while (k1 < ne && S.entries[k1].gate == S.entries[k].gate) ++k1;
-        red.push_back(GateRed{off, S.ntiles, ne, k, k1, S.entries[k].gate, 0});
-        k = k1;
-      }
-      off += (int64_t)S.ntiles * ne;
+      S.index = (int)info.size();
+      StageInfo I{};
+      I.npairs = S.npairs;
+      I.nentries = (int)S.entries.size();
+      I.ntiles = S.ntiles;
+      for (int j = 0; j <= kMaxPairs; ++j) I.pe[j] = S.pe[j];
+      I.poff = off;
+      I.entries = S.d_entries;
+      info.push_back(I);
+      S.small.poff = off;
+      S.large.poff = off;
+      off += (int64_t)S.ntiles * S.npairs * kQVals;
     }
+    T->ntiled = (int)info.size();
     SQ_HIP(hipMalloc(&T->d_partial, sizeof(double) * std::max<int64_t>(off, 1)));
     T->partial_len = off;
+    SQ_HIP(hipMalloc(&T->d_mats, sizeof(double) * std::max<int64_t>((int64_t)T->ntiled * kMaxPairs * kMatVals, 1)));
+    for (auto& S : T->stages) {
+      if (!S.tiled) continue;
+      S.small.mats = T->d_mats + (size_t)S.index * kMaxPairs * kMatVals;
+      S.large.mats = S.small.mats;
+    }
     std::vector<void*> tmp;
-    T->d_red = upload(tmp, red);
-    T->nred = (int)red.size();
+    T->d_info = upload(tmp, info);
   } catch (...) {
     delete T;
     throw;
@@ -504,19 +540,193 @@ const TilePlan* tiles_for(const Space& s, const Program& p) {
 // ---------------------------------------------------------------------------
 // Device kernels.
 //
+// k_stage_mats   per tiled stage and pair, the product of the pair's gates
+//                (optionally with T_g inserted after gate g) per orbit type
+// k_stage_fwd    one stage on one vector: per pair, orbit matrix-vector
+//                products (transposed for U^+)
+// k_stage_rev    reverse stage on (psi, lam): per pair, accumulate the orbit
+//                outer products, then un-apply the pair matrix on both
+// k_stage_grad   per stage and pair, sum the outer products over tiles and
+//                contract them with each gate's conjugated generator
+//
 // Tile kernels are templated on the block size: NT = 64 (one wave, tiles of
 // up to the plan's target size, many resident blocks per CU) or NT = 256
 // (tiles of up to kTileMax).  LDS is dynamic, sized to the launch's largest
-// tile:
-//   X[emax] (| L[emax] in the reverse sweep) | cdesc[wmax] | ccol[wmax] |
-//   codes[ne*9] | pad | cs[3*ne] (cos, sin, scale) | rows[64] |
-//   red[ne * NT/64] (reverse, NT > 64)
+// tile: X[emax] (| L[emax] in the reverse sweep) | mats[npairs*32] |
+// cdesc[wmax] | ccol[wmax] | rows[64].
 //
 // Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8);
 // tiles are remapped so that each XCD runs a contiguous range of the tile
-// list.  Consecutive tiles share alpha rows, so the rows' cache lines are
-// fetched into one XCD's L2 and the scattered column accesses of the tiles
-// hit there.
+// list (consecutive tiles share alpha rows).
+
+// ---- 4x4 orbit algebra (matrix kernels) ------------------------------------
+
+// R M for the rotation of one entry code (disjoint member planes).
+__device__ __forceinline__ void rot_rows(uint32_t code, double c, double s, double (&M)[16]) {
+  for (int sl = 0; sl < 6; ++sl) {
+    const uint32_t f = (code >> (2 * sl)) & 3u;
+    if (!(f & 1u)) continue;
+    const int a = kSlotA[sl], b = kSlotB[sl];
+    const double sg = (f & 2u) ? -s : s;
+    for (int q = 0; q < 4; ++q) {
+      const double x = M[a * 4 + q], y = M[b * 4 + q];
+      M[a * 4 + q] = c * x - sg * y;
+      M[b * 4 + q] = sg * x + c * y;
+    }
+  }
+}
+
+// M R^T (columns), the other half of a conjugation.
+__device__ __forceinline__ void rot_cols(uint32_t code, double c, double s, double (&M)[16]) {
+  for (int sl = 0; sl < 6; ++sl) {
+    const uint32_t f = (code >> (2 * sl)) & 3u;
+    if (!(f & 1u)) continue;
+    const int a = kSlotA[sl], b = kSlotB[sl];
+    const double sg = (f & 2u) ? -s : s;
+    for (int q = 0; q < 4; ++q) {
+      const double x = M[q * 4 + a], y = M[q * 4 + b];
+      M[q * 4 + a] = c * x - sg * y;
+      M[q * 4 + b] = sg * x + c * y;
+    }
+  }
+}
+
+// out += scale * T M (rows), T|a> = sg|b>, T|b> = -sg|a> per active plane.
+__device__ __forceinline__ void t_rows(uint32_t code, double scale, const double (&M)[16], double (&out)[16]) {
+  for (int sl = 0; sl < 6; ++sl) {
+    const uint32_t f = (code >> (2 * sl)) & 3u;
+    if (!(f & 1u)) continue;
+    const int a = kSlotA[sl], b = kSlotB[sl];
+    const double sg = (f & 2u) ? -scale : scale;
+    for (int q = 0; q < 4; ++q) {
+      out[b * 4 + q] += sg * M[a * 4 + q];
+      out[a * 4 + q] -= sg * M[b * 4 + q];
+    }
+  }
+}
+
+// Orbit-type classes of the compact layout: type index (alpha kind * 3 +
+// beta kind), offset, second member.
+__device__ __constant__ int c_cls_type[5] = {8, 6, 7, 2, 5};
+__device__ __constant__ int c_cls_off[5] = {0, 16, 20, 24, 28};
+__device__ __constant__ int c_cls_m1[5] = {0, 1, 1, 2, 2};
+
+__device__ __forceinline__ void entry_cs(const StageEntry& en, const double* __restrict__ theta, double& c,
+                                         double& s) {
+  const double th = theta[en.gate];
+  s = 0.0;
+  c = 1.0;
+  if (fabs(th) >= 1e-28) sincos(en.scale * th, &s, &c);
+}
+
+// One thread per (stage, pair, class): the pair's gate product in forward
+// order, with T_g inserted after gate g when deriv_gate is on this pair.
+__global__ void k_stage_mats(const StageInfo* __restrict__ info, const double* __restrict__ theta, int deriv_gate,
+                             double* __restrict__ mats) {
+  const StageInfo I = info[blockIdx.x];
+  const int j = threadIdx.x / 5, cls = threadIdx.x % 5;
+  if (j >= I.npairs) return;
+  const int t = c_cls_type[cls];
+  double M[16];
+  for (int q = 0; q < 16; ++q) M[q] = (q % 5 == 0) ? 1.0 : 0.0;
+  const int k0 = I.pe[j], k1 = I.pe[j + 1];
+  bool has_deriv = false;
+  for (int k = k0; k < k1; ++k) {
+    const StageEntry& en = I.entries[k];
+    double c, s;
+    entry_cs(en, theta, c, s);
+    rot_rows(en.code[t], c, s, M);
+    if (en.gate == deriv_gate && (k + 1 == k1 || I.entries[k + 1].gate != deriv_gate)) {
+      has_deriv = true;
+      double o[16];
+      for (int q = 0; q < 16; ++q) o[q] = 0.0;
+      for (int q = k; q >= k0 && I.entries[q].gate == deriv_gate; --q) t_rows(I.entries[q].code[t], I.entries[q].scale, M, o);
+      for (int q = 0; q < 16; ++q) M[q] = o[q];
+    }
+  }
+  double* base = mats + ((size_t)blockIdx.x * kMaxPairs + j) * kMatVals;
+  if (cls == 0) base[kPairVals] = has_deriv ? 0.0 : 1.0;
+  double* out = base + c_cls_off[cls];
+  if (cls == 0) {
+    for (int q = 0; q < 16; ++q) out[q] = M[q];
+  } else {
+    const int m1 = c_cls_m1[cls];
+    out[0] = M[0];
+    out[1] = M[m1];
+    out[2] = M[m1 * 4];
+    out[3] = M[m1 * 4 + m1];
+  }
+}
+
+// One block per (stage, pair): Q summed over the stage's tiles, then for every
+// entry k of the pair dot_k = sum_{a<b} (W T_k W^T)[a][b] Q[a<b] per class, W
+// the product of the pair's entries after k; grad[gate] = 2 sum over its
+// entries of scale * dot.
+__device__ __constant__ int c_qa[6] = {0, 0, 0, 1, 1, 2};
+__device__ __constant__ int c_qb[6] = {1, 2, 3, 2, 3, 3};
+
+__global__ void k_stage_grad(const StageInfo* __restrict__ info, const double* __restrict__ theta,
+                             const double* __restrict__ partial, double* __restrict__ grad) {
+  __shared__ double Q[kQVals];
+  __shared__ double dots[kMaxPairEntries * 5];
+  __shared__ double red[kTB / 64];
+  const StageInfo I = info[blockIdx.x / kMaxPairs];
+  const int j = blockIdx.x % kMaxPairs;
+  if (j >= I.npairs) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int v = 0; v < kQVals; ++v) {
+    const double* src = partial + I.poff + ((size_t)j * kQVals + v) * I.ntiles;
+    double acc = 0.0;
+    for (int tile = threadIdx.x; tile < I.ntiles; tile += blockDim.x) acc += src[tile];
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (lane == 0) red[wave] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
+      Q[v] = tot;
+    }
+    __syncthreads();
+  }
+  const int k0 = I.pe[j], k1 = I.pe[j + 1];
+  const int nent = k1 - k0;
+  if (threadIdx.x < nent * 5) {
+    const int k = k0 + threadIdx.x / 5, cls = threadIdx.x % 5;
+    const int t = c_cls_type[cls];
+    double M[16];
+    for (int q = 0; q < 16; ++q) M[q] = 0.0;
+    double I4[16];
+    for (int q = 0; q < 16; ++q) I4[q] = (q % 5 == 0) ? 1.0 : 0.0;
+    t_rows(I.entries[k].code[t], 1.0, I4, M);  // M = T_k
+    for (int e = k + 1; e < k1; ++e) {
+      double c, s;
+      entry_cs(I.entries[e], theta, c, s);
+      rot_rows(I.entries[e].code[t], c, s, M);
+      rot_cols(I.entries[e].code[t], c, s, M);
+    }
+    double d = 0.0;
+    if (cls == 0) {
+      for (int q = 0; q < 6; ++q) d += M[c_qa[q] * 4 + c_qb[q]] * Q[q];
+    } else {
+      d = M[c_cls_m1[cls]] * Q[5 + cls];
+    }
+    dots[threadIdx.x] = d * I.entries[k].scale;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 0; q < nent;) {
+      const int g = I.entries[k0 + q].gate;
+      double tot = 0.0;
+      while (q < nent && I.entries[k0 + q].gate == g) {
+        for (int cls = 0; cls < 5; ++cls) tot += dots[q * 5 + cls];
+        ++q;
+      }
+      grad[g] = 2.0 * tot;
+    }
+  }
+}
+
+// ---- tile kernels -----------------------------------------------------------
 
 struct TileView {
   int R, W, E, c0;
@@ -527,33 +737,24 @@ struct TileView {
 struct TileLds {
   double* X;
   double* L;
+  double* mats;
   uint32_t* cdesc;
   uint32_t* ccol;
-  uint32_t* codes;
-  double* cs;
   int32_t* rows;
-  double* red;
 };
 
-static size_t lds_bytes(int nt, int emax, int wmax, int ne, bool rev) {
-  const size_t n32 = 2 * (size_t)wmax + (size_t)ne * kTypes;
-  size_t b = (size_t)emax * 8 * (rev ? 2 : 1) + (n32 + (n32 & 1)) * 4 + (size_t)ne * 24 + 64 * 4;
-  if (rev && nt > 64) b += (size_t)ne * (nt / 64) * 8;
-  return b;
+static size_t lds_bytes(int emax, int wmax, int npairs, bool rev) {
+  return (size_t)emax * 8 * (rev ? 2 : 1) + (size_t)npairs * kMatVals * 8 + (size_t)wmax * 8 + 64 * 4;
 }
 
 __device__ __forceinline__ TileLds carve_lds(double* smem, const StageDev& S, bool rev) {
   TileLds m;
-  const int ne = S.nentries;
   m.X = smem;
   m.L = rev ? m.X + S.emax : nullptr;
-  m.cdesc = reinterpret_cast<uint32_t*>(m.X + (rev ? 2 : 1) * S.emax);
+  m.mats = m.X + (rev ? 2 : 1) * S.emax;
+  m.cdesc = reinterpret_cast<uint32_t*>(m.mats + S.npairs * kMatVals);
   m.ccol = m.cdesc + S.wmax;
-  m.codes = m.ccol + S.wmax;
-  const int n32 = 2 * S.wmax + ne * kTypes;
-  m.cs = reinterpret_cast<double*>(m.cdesc + n32 + (n32 & 1));
-  m.rows = reinterpret_cast<int32_t*>(m.cs + 3 * ne);
-  m.red = reinterpret_cast<double*>(m.rows + 64);
+  m.rows = reinterpret_cast<int32_t*>(m.ccol + S.wmax);
   return m;
 }
 
@@ -565,12 +766,10 @@ __device__ __forceinline__ int xcd_tile(int b, int n, int on) {
   return x * per + min(x, rem) + q;
 }
 
-// Tile descriptors into LDS: columns' pair descriptors and canonical indices,
-// alpha rows, entry codes and the entries' cos/sin/scale (identity rotation
-// for |theta| < 1e-28).
+// Tile descriptors into LDS: alpha rows, the columns' pair descriptors and
+// canonical indices, and the stage's pair matrices.
 template <int NT>
-__device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile,
-                                               const double* __restrict__ theta, double sgn_th) {
+__device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile) {
   const int4 T = S.tiles[S.tile0 + tile];
   const int r0 = S.agoff[T.x];
   TileView t;
@@ -586,17 +785,7 @@ __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds&
     m.cdesc[c] = S.bdesc[T.y + c];
     m.ccol[c] = (uint32_t)S.bcol[T.y + c];
   }
-#pragma unroll 4
-  for (int i = threadIdx.x; i < S.nentries * kTypes; i += NT) m.codes[i] = S.entries[i / kTypes].code[i % kTypes];
-  for (int k = threadIdx.x; k < S.nentries; k += NT) {
-    const StageEntry& en = S.entries[k];
-    const double th = theta[en.gate];
-    double sn = 0.0, c = 1.0;
-    if (fabs(th) >= 1e-28) sincos(sgn_th * en.scale * th, &sn, &c);
-    m.cs[3 * k] = c;
-    m.cs[3 * k + 1] = sn;
-    m.cs[3 * k + 2] = en.scale;
-  }
+  for (int i = threadIdx.x; i < S.npairs * kMatVals; i += NT) m.mats[i] = S.mats[i];
   __syncthreads();
   return t;
 }
@@ -634,15 +823,16 @@ __device__ __forceinline__ void tile_store(const StageDev& S, const TileView& t,
   }
 }
 
-// Orbit of pair j owned by work item i: member element offsets in the tile
-// (-1 where a member does not exist) and the orbit type; false when item i is
-// not an orbit representative.
+// Orbit of pair j owned by work item i: element offsets of members 0..3 in
+// the tile and the class of its type in the compact layout (-1: item i is
+// not an orbit representative; -2: a one-member orbit no gate acts on, m0 set).
 struct Orbit {
   int m0, m1, m2, m3;
-  int type;
+  int cls;
 };
 
-__device__ __forceinline__ bool orbit_of(const TileView& t, const uint32_t* cdesc, int j, int i, Orbit& ob) {
+__device__ __forceinline__ void orbit_of(const TileView& t, const uint32_t* cdesc, int j, int i, Orbit& ob) {
+  ob.cls = -1;
   const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
   const bool asingle = (fa & 3u) >= 2u;
   const int abit = (int)(fa >> 2);
@@ -651,260 +841,156 @@ __device__ __forceinline__ bool orbit_of(const TileView& t, const uint32_t* cdes
   const int r0 = asingle ? (((rr >> abit) << (abit + 1)) | (rr & ((1 << abit) - 1))) : rr;
   const uint32_t fb = (cdesc[c] >> (5 * j)) & 31u;
   const uint32_t kb = fb & 3u;
-  if (kb == 3u) return false;  // upper member of a beta pair
+  if (kb == 3u) return;  // upper member of a beta pair
   const bool bsingle = kb == 2u;
-  const int ta = asingle ? 2 : (int)(fa & 3u);
-  const int tb = bsingle ? 2 : (int)kb;
-  ob.type = ta * 3 + tb;
   const int W = t.W;
   ob.m0 = r0 * W + c;
+  if (!asingle && !bsingle) {  // no gate of the pair acts
+    ob.cls = -2;
+    return;
+  }
   const int dr = asingle ? (W << abit) : 0;
   const int dc = bsingle ? (1 << (fb >> 2)) : 0;
-  ob.m1 = asingle ? ob.m0 + dr : -1;
-  ob.m2 = bsingle ? ob.m0 + dc : -1;
-  ob.m3 = (asingle && bsingle) ? ob.m0 + dr + dc : -1;
-  return true;
+  ob.m1 = ob.m0 + dr;
+  ob.m2 = ob.m0 + dc;
+  ob.m3 = ob.m0 + dr + dc;
+  if (asingle && bsingle) ob.cls = 0;
+  else if (asingle) ob.cls = kb == 0u ? 1 : 2;
+  else ob.cls = (fa & 3u) == 0u ? 3 : 4;
 }
 
-__device__ __forceinline__ void rot(double& a, double& b, double c, double s) {
-  const double x = a, y = b;
-  a = c * x - s * y;
-  b = s * x + c * y;
-}
+constexpr int kBatch = 4;  // orbits per thread in flight
 
-// Rotation by (c, s) of every member pair that a code marks (sign per pair).
-__device__ __forceinline__ void apply_code(uint32_t code, double c, double s, double (&v)[4]) {
+// v = M v (or M^T v) for one orbit, in place in X.
+__device__ __forceinline__ void orbit_apply(double* X, const Orbit& ob, const double* Mp, bool tr) {
+  if (ob.cls == 0) {
+    const double v0 = X[ob.m0], v1 = X[ob.m1], v2 = X[ob.m2], v3 = X[ob.m3];
+    double o[4];
 #pragma unroll
-  for (int sl = 0; sl < 6; ++sl) {
-    const uint32_t f = (code >> (2 * sl)) & 3u;
-    if (f & 1u) rot(v[kSlotA[sl]], v[kSlotB[sl]], c, (f & 2u) ? -s : s);
-  }
-}
-
-// out += scale * T v for one entry's code (T|a> = sg|b>, T|b> = -sg|a>).
-__device__ __forceinline__ void t_code(uint32_t code, double scale, const double (&v)[4], double (&out)[4]) {
-#pragma unroll
-  for (int sl = 0; sl < 6; ++sl) {
-    const uint32_t f = (code >> (2 * sl)) & 3u;
-    if (f & 1u) {
-      const double sg = (f & 2u) ? -scale : scale;
-      out[kSlotB[sl]] += sg * v[kSlotA[sl]];
-      out[kSlotA[sl]] -= sg * v[kSlotB[sl]];
+    for (int a = 0; a < 4; ++a) {
+      const double* r = tr ? Mp + a : Mp + 4 * a;
+      const int st = tr ? 4 : 1;
+      o[a] = r[0] * v0 + r[st] * v1 + r[2 * st] * v2 + r[3 * st] * v3;
     }
+    X[ob.m0] = o[0];
+    X[ob.m1] = o[1];
+    X[ob.m2] = o[2];
+    X[ob.m3] = o[3];
+  } else {
+    const int m1 = ob.cls <= 2 ? ob.m1 : ob.m2;
+    const double* B = Mp + c_cls_off[ob.cls];
+    const double v0 = X[ob.m0], v1 = X[m1];
+    const double b01 = tr ? B[2] : B[1], b10 = tr ? B[1] : B[2];
+    X[ob.m0] = B[0] * v0 + b01 * v1;
+    X[m1] = b10 * v0 + B[3] * v1;
   }
 }
-
-// Contribution of one entry's code to <lam|T psi>.
-__device__ __forceinline__ double dot_code(uint32_t code, const double (&p)[4], const double (&l)[4]) {
-  double d = 0.0;
-#pragma unroll
-  for (int sl = 0; sl < 6; ++sl) {
-    const uint32_t f = (code >> (2 * sl)) & 3u;
-    if (f & 1u) {
-      const double x = l[kSlotB[sl]] * p[kSlotA[sl]] - l[kSlotA[sl]] * p[kSlotB[sl]];
-      d += (f & 2u) ? -x : x;
-    }
-  }
-  return d;
-}
-
-__device__ __forceinline__ void load_orbit(const double* X, const Orbit& ob, double (&v)[4]) {
-  v[0] = X[ob.m0];
-  v[1] = ob.m1 >= 0 ? X[ob.m1] : 0.0;
-  v[2] = ob.m2 >= 0 ? X[ob.m2] : 0.0;
-  v[3] = ob.m3 >= 0 ? X[ob.m3] : 0.0;
-}
-
-__device__ __forceinline__ void store_orbit(double* X, const Orbit& ob, const double (&v)[4]) {
-  X[ob.m0] = v[0];
-  if (ob.m1 >= 0) X[ob.m1] = v[1];
-  if (ob.m2 >= 0) X[ob.m2] = v[2];
-  if (ob.m3 >= 0) X[ob.m3] = v[3];
-}
-
-constexpr int kBatch = 4;     // orbits per thread in flight (forward)
-constexpr int kBatchRev = 2;  // same, reverse sweep (two vectors)
 
 // Gates of one stage on one vector (canonical order, in place).
-// dagger: U^+ (pair entries reversed, -theta).  deriv_gate: insert T_g after gate g.
+// dagger: U^+ (transposed pair matrices).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ theta, int dagger,
-                                                  double* __restrict__ psi, int deriv_gate) {
+__global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, int dagger, double* __restrict__ psi) {
   extern __shared__ double smem[];
   const TileLds m = carve_lds(smem, S, false);
-  double* X = m.X;
-  const uint32_t* cdesc = m.cdesc;
-  const uint32_t* codes = m.codes;
-  const double* cs = m.cs;
-  const TileView t = tile_setup<NT>(S, m, xcd_tile(blockIdx.x, gridDim.x, S.xcd), theta, dagger ? -1.0 : 1.0);
-  tile_load<NT>(S, t, m, psi, X);
+  const TileView t = tile_setup<NT>(S, m, xcd_tile(blockIdx.x, gridDim.x, S.xcd));
+  tile_load<NT>(S, t, m, psi, m.X);
   __syncthreads();
   for (int j = 0; j < S.npairs; ++j) {
-    const int k0 = S.pe[j], k1 = S.pe[j + 1];
-    const int nent = k1 - k0;
-    if (nent == 0) continue;
     const bool asingle = ((t.adesc >> (5 * j)) & 3u) >= 2u;
     const int items = asingle ? t.E >> 1 : t.E;
-    int dq0 = -1, dq1 = -1;  // entries of deriv_gate on this pair
-    if (deriv_gate >= 0) {
-      for (int k = k0; k < k1; ++k)
-        if (S.entries[k].gate == deriv_gate) {
-          if (dq0 < 0) dq0 = k;
-          dq1 = k + 1;
-        }
-    }
+    const double* Mp = m.mats + j * kMatVals;
+    const bool zero_rest = Mp[kPairVals] == 0.0;  // derivative pair: T_g annihilates the other types
     for (int i0 = threadIdx.x; i0 < items; i0 += NT * kBatch) {
       Orbit ob[kBatch];
-      bool ok[kBatch];
-      double v[kBatch][4];
 #pragma unroll
       for (int b = 0; b < kBatch; ++b) {
         const int i = i0 + b * NT;
-        ok[b] = i < items && orbit_of(t, cdesc, j, i, ob[b]);
-        if (ok[b]) load_orbit(X, ob[b], v[b]);
+        ob[b].cls = -1;
+        if (i < items) orbit_of(t, m.cdesc, j, i, ob[b]);
       }
 #pragma unroll
-      for (int kk = 0; kk < kMaxPairEntries; ++kk) {
-        if (kk < nent) {
-          const int k = dagger ? k1 - 1 - kk : k0 + kk;
-          const double c = cs[3 * k], sn = cs[3 * k + 1];
-#pragma unroll
-          for (int b = 0; b < kBatch; ++b)
-            if (ok[b]) apply_code(codes[k * kTypes + ob[b].type], c, sn, v[b]);
-          if (k + 1 == dq1) {
-#pragma unroll
-            for (int b = 0; b < kBatch; ++b) {
-              if (!ok[b]) continue;
-              double o[4] = {0.0, 0.0, 0.0, 0.0};
-              for (int q = dq0; q < dq1; ++q) t_code(codes[q * kTypes + ob[b].type], cs[3 * q + 2], v[b], o);
-#pragma unroll
-              for (int m = 0; m < 4; ++m) v[b][m] = o[m];
-            }
-          }
-        }
+      for (int b = 0; b < kBatch; ++b) {
+        if (ob[b].cls >= 0) orbit_apply(m.X, ob[b], Mp, dagger != 0);
+        else if (ob[b].cls == -2 && zero_rest) m.X[ob[b].m0] = 0.0;
       }
-#pragma unroll
-      for (int b = 0; b < kBatch; ++b)
-        if (ok[b]) store_orbit(X, ob[b], v[b]);
     }
     __syncthreads();
   }
-  tile_store<NT>(S, t, m, psi, X);
+  tile_store<NT>(S, t, m, psi, m.X);
 }
 
-// Reverse sweep of one stage on (psi, lam), in place, with per-entry partial
-// dots written to partial[partial_off + tile * nentries + k].
+// Reverse stage on (psi, lam), in place: per pair, the antisymmetric orbit
+// products Q[a<b] += lam_a psi_b - lam_b psi_a before un-applying the pair
+// matrix (M^T) on both; per-tile sums go to partial[poff + (j*10 + v)*ntiles + tile].
 template <int NT>
-__global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __restrict__ theta,
-                                                  double* __restrict__ psi, double* __restrict__ lam,
+__global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, double* __restrict__ psi, double* __restrict__ lam,
                                                   double* __restrict__ partial, int write_back) {
   extern __shared__ double smem[];
-  const int ne = S.nentries;
+  __shared__ double red[kQVals][NT / 64];
   const TileLds m = carve_lds(smem, S, true);
-  double* X = m.X;
-  double* L = m.L;
-  const uint32_t* cdesc = m.cdesc;
-  const uint32_t* codes = m.codes;
-  const double* cs = m.cs;
-  double* red = m.red;
   const int tile = xcd_tile(blockIdx.x, gridDim.x, S.xcd);
-  const TileView t = tile_setup<NT>(S, m, tile, theta, -1.0);
-  tile_load<NT>(S, t, m, psi, X);
-  tile_load<NT>(S, t, m, lam, L);
+  const TileView t = tile_setup<NT>(S, m, tile);
+  tile_load<NT>(S, t, m, psi, m.X);
+  tile_load<NT>(S, t, m, lam, m.L);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const size_t pbase = S.partial_off + (size_t)(S.tile0 + tile) * ne;
+  const int ntiles_stage = S.ntiles_stage;
+  const int my_tile = S.tile0 + tile;
   for (int j = 0; j < S.npairs; ++j) {
-    const int k0 = S.pe[j], k1 = S.pe[j + 1];
-    const int nent = k1 - k0;
-    if (nent == 0) continue;
     const bool asingle = ((t.adesc >> (5 * j)) & 3u) >= 2u;
     const int items = asingle ? t.E >> 1 : t.E;
-    double dots[kMaxPairEntries];
+    const double* Mp = m.mats + j * kMatVals;
+    double Q[kQVals];
 #pragma unroll
-    for (int kk = 0; kk < kMaxPairEntries; ++kk) dots[kk] = 0.0;
-    for (int i0 = threadIdx.x; i0 < items; i0 += NT * kBatchRev) {
-      Orbit ob[kBatchRev];
-      bool ok[kBatchRev];
-      double v[kBatchRev][4], w[kBatchRev][4];
+    for (int q = 0; q < kQVals; ++q) Q[q] = 0.0;
+    for (int i = threadIdx.x; i < items; i += NT) {
+      Orbit ob;
+      orbit_of(t, m.cdesc, j, i, ob);
+      if (ob.cls < 0) continue;
+      if (ob.cls == 0) {
+        const double v0 = m.X[ob.m0], v1 = m.X[ob.m1], v2 = m.X[ob.m2], v3 = m.X[ob.m3];
+        const double w0 = m.L[ob.m0], w1 = m.L[ob.m1], w2 = m.L[ob.m2], w3 = m.L[ob.m3];
+        Q[0] += w0 * v1 - w1 * v0;
+        Q[1] += w0 * v2 - w2 * v0;
+        Q[2] += w0 * v3 - w3 * v0;
+        Q[3] += w1 * v2 - w2 * v1;
+        Q[4] += w1 * v3 - w3 * v1;
+        Q[5] += w2 * v3 - w3 * v2;
+      } else {
+        const int m1 = ob.cls <= 2 ? ob.m1 : ob.m2;
+        const double x = m.L[ob.m0] * m.X[m1] - m.L[m1] * m.X[ob.m0];
 #pragma unroll
-      for (int b = 0; b < kBatchRev; ++b) {
-        const int i = i0 + b * NT;
-        ok[b] = i < items && orbit_of(t, cdesc, j, i, ob[b]);
-        if (ok[b]) {
-          load_orbit(X, ob[b], v[b]);
-          load_orbit(L, ob[b], w[b]);
-        }
-      }
-#pragma unroll
-      for (int kk = 0; kk < kMaxPairEntries; ++kk) {
-        if (kk < nent) {
-          const int k = k1 - 1 - kk;
-          const double c = cs[3 * k], sn = cs[3 * k + 1];
-#pragma unroll
-          for (int b = 0; b < kBatchRev; ++b) {
-            if (!ok[b]) continue;
-            const uint32_t code = codes[k * kTypes + ob[b].type];
-            dots[kk] += dot_code(code, v[b], w[b]);
-            apply_code(code, c, sn, v[b]);
-            apply_code(code, c, sn, w[b]);
-          }
-        }
+        for (int cls = 1; cls < 5; ++cls)
+          if (ob.cls == cls) Q[5 + cls] += x;
       }
       if (write_back) {
-#pragma unroll
-        for (int b = 0; b < kBatchRev; ++b) {
-          if (!ok[b]) continue;
-          store_orbit(X, ob[b], v[b]);
-          store_orbit(L, ob[b], w[b]);
-        }
+        orbit_apply(m.X, ob, Mp, true);
+        orbit_apply(m.L, ob, Mp, true);
       }
     }
 #pragma unroll
-    for (int kk = 0; kk < kMaxPairEntries; ++kk) {
-      if (kk < nent) {
-        const int k = k1 - 1 - kk;
-        double d = dots[kk];
-        for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
-        d *= cs[3 * k + 2];
-        if (NT == 64) {
-          if (lane == 0) partial[pbase + k] = d;
-        } else if (lane == 0) {
-          red[k * (NT / 64) + wave] = d;
-        }
+    for (int q = 0; q < kQVals; ++q) {
+      double d = Q[q];
+      for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
+      if (NT == 64) {
+        if (lane == 0) partial[S.poff + ((size_t)j * kQVals + q) * ntiles_stage + my_tile] = d;
+      } else if (lane == 0) {
+        red[q][wave] = d;
+      }
+    }
+    if (NT > 64) {
+      __syncthreads();
+      if (threadIdx.x < kQVals) {
+        double v = 0.0;
+        for (int w = 0; w < NT / 64; ++w) v += red[threadIdx.x][w];
+        partial[S.poff + ((size_t)j * kQVals + threadIdx.x) * ntiles_stage + my_tile] = v;
       }
     }
     __syncthreads();
   }
-  if (NT > 64) {
-    for (int k = threadIdx.x; k < ne; k += NT) {
-      double v = 0.0;
-      for (int w = 0; w < NT / 64; ++w) v += red[k * (NT / 64) + w];
-      partial[pbase + k] = v;
-    }
-  }
   if (!write_back) return;  // the first stage of the program: the un-applied states are not needed
-  tile_store<NT>(S, t, m, psi, X);
-  tile_store<NT>(S, t, m, lam, L);
-}
-
-// grad[gate] = 2 * sum over the stage's tiles of the gate's entry partials;
-// one block per tiled gate.
-__global__ void k_tile_grad(const double* __restrict__ partial, const GateRed* __restrict__ red,
-                            double* __restrict__ grad) {
-  __shared__ double rs[kTB / 64];
-  const GateRed R = red[blockIdx.x];
-  double v = 0.0;
-  for (int i = threadIdx.x; i < R.ntiles; i += blockDim.x)
-    for (int q = R.k0; q < R.k1; ++q) v += partial[R.off + (size_t)i * R.stride + q];
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-  if ((threadIdx.x & 63) == 0) rs[threadIdx.x >> 6] = v;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double tot = 0.0;
-    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += rs[w];
-    grad[R.gate] = 2.0 * tot;
-  }
+  tile_store<NT>(S, t, m, psi, m.X);
+  tile_store<NT>(S, t, m, lam, m.L);
 }
 
 // ---------------------------------------------------------------------------
@@ -954,47 +1040,49 @@ static void fork_join(const Stage& S, hipStream_t st, SmallL small, LargeL large
   }
 }
 
-// Dynamic LDS above 64 KB (large-tile reverse sweep) must be enabled per kernel.
+// Dynamic LDS above 64 KB (large-tile reverse sweep) must be enabled per
+// kernel; the request covers exactly what the launch needs.
 template <class K>
 static void allow_lds(K kernel, size_t bytes) {
   static std::mutex mu;
   static size_t allowed = 64 * 1024;
-  SQ_CHECK(bytes <= 160 * 1024, SQ_ERR_ARG, "tile needs more LDS than a CU has");
   std::lock_guard<std::mutex> lock(mu);
   if (bytes > allowed) {
     SQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               160 * 1024));
-    allowed = 160 * 1024;
+                               (int)bytes));
+    allowed = bytes;
   }
 }
 
-static void launch_fwd(const Stage& S, const double* theta, bool dagger, double* psi, int dg, hipStream_t st) {
-  const int ne = S.small.nentries;
+static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, hipStream_t st) {
+  SQ_PROF(PC_OTHER, st, 0.0, 0.0,
+          k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
+}
+
+static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st) {
   fork_join(
       S, st,
       [&](hipStream_t q) {
-        k_stage_fwd<64><<<S.nsmall, 64, lds_bytes(64, S.small.emax, S.small.wmax, ne, false), q>>>(
-            S.small, theta, dagger ? 1 : 0, psi, dg);
+        k_stage_fwd<64><<<S.nsmall, 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), q>>>(
+            S.small, dagger ? 1 : 0, psi);
       },
       [&](hipStream_t q) {
-        k_stage_fwd<kTB><<<S.ntiles - S.nsmall, kTB, lds_bytes(kTB, S.large.emax, S.large.wmax, ne, false), q>>>(
-            S.large, theta, dagger ? 1 : 0, psi, dg);
+        k_stage_fwd<kTB><<<S.ntiles - S.nsmall, kTB, lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), q>>>(
+            S.large, dagger ? 1 : 0, psi);
       });
 }
 
-static void launch_rev(const Stage& S, const double* theta, double* psi, double* lam, double* partial, int wb,
-                       hipStream_t st) {
-  const int ne = S.small.nentries;
+static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
   fork_join(
       S, st,
       [&](hipStream_t q) {
-        k_stage_rev<64><<<S.nsmall, 64, lds_bytes(64, S.small.emax, S.small.wmax, ne, true), q>>>(
-            S.small, theta, psi, lam, partial, wb);
+        k_stage_rev<64><<<S.nsmall, 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, true), q>>>(
+            S.small, psi, lam, partial, wb);
       },
       [&](hipStream_t q) {
-        const size_t lds = lds_bytes(kTB, S.large.emax, S.large.wmax, ne, true);
+        const size_t lds = lds_bytes(S.large.emax, S.large.wmax, S.npairs, true);
         allow_lds(k_stage_rev<kTB>, lds);
-        k_stage_rev<kTB><<<S.ntiles - S.nsmall, kTB, lds, q>>>(S.large, theta, psi, lam, partial, wb);
+        k_stage_rev<kTB><<<S.ntiles - S.nsmall, kTB, lds, q>>>(S.large, psi, lam, partial, wb);
       });
 }
 
@@ -1005,6 +1093,7 @@ static void launch_rev(const Stage& S, const double* theta, double* psi, double*
 // deriv_gate >= 0 inserts T_g right after gate g (plain gradient; forward only).
 void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi,
                    double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st) {
+  launch_mats(T, theta, dagger ? -1 : deriv_gate, st);
   const int ns = (int)T.stages.size();
   for (int ii = 0; ii < ns; ++ii) {
     const int i = dagger ? ns - 1 - ii : ii;
@@ -1017,8 +1106,7 @@ void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const do
       }
       continue;
     }
-    const int dg = (deriv_gate >= S.g0 && deriv_gate < S.g1) ? deriv_gate : -1;
-    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, theta, dagger, psi, dg, st));
+    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, dagger, psi, st));
   }
 }
 
@@ -1028,6 +1116,7 @@ void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const do
                    double* sp_, double* sl_, double* gwork, double* partial, double* grad_dev, hipStream_t st) {
   (void)sp_;
   (void)sl_;
+  launch_mats(T, theta, -1, st);
   for (int i = (int)T.stages.size() - 1; i >= 0; --i) {
     const Stage& S = T.stages[i];
     if (!S.tiled) {
@@ -1035,11 +1124,11 @@ void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const do
       continue;
     }
     const int wb = i > 0 ? 1 : 0;
-    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(S, theta, psi, lam, T.d_partial, wb, st));
+    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(S, psi, lam, T.d_partial, wb, st));
   }
-  if (T.nred > 0)
+  if (T.ntiled > 0)
     SQ_PROF(PC_REDUCE, st, 8.0 * (double)T.partial_len, 0.0,
-            k_tile_grad<<<T.nred, kTB, 0, st>>>(T.d_partial, T.d_red, grad_dev));
+            k_stage_grad<<<T.ntiled * kMaxPairs, kTB, 0, st>>>(T.d_info, theta, T.d_partial, grad_dev));
 }
 
 }  // namespace sq
