@@ -14,24 +14,23 @@
 //
 // Gates on different pairs commute, so the stage runs pair by pair.  For one
 // pair j the gates only mix the <= 4 members of an "orbit" {alpha bit of j}
-// x {beta bit of j} (other indices fixed); a thread owns an orbit, keeps its
-// members in registers and applies all of pair j's gates (and, in the
-// reverse sweep, their gradient dots) without synchronising; one barrier per
-// pair.  Signs come from the pair-local bits only: the environment parity
-// cancels for number-conserving spin parts.
+// x {beta bit of j} (other indices fixed), and their whole product is one
+// small matrix per orbit type; a thread owns an orbit and applies it.  Signs
+// come from the pair-local bits only: the environment parity cancels for
+// number-conserving spin parts.  One barrier per pair.
 //
-// Per entry (one rotation string) and orbit type (alpha part of the pair
-// empty / doubly occupied / single, same for beta) the host precomputes a
-// 12-bit code: for each of the 6 member pairs {0,1},{2,3},{0,2},{1,3},{0,3},
-// {1,2} whether the string rotates it and with which sign, so the device
-// applies rotations with register indices fixed at compile time.
+// Per rotation string and orbit type (alpha part of the pair empty / doubly
+// occupied / single, same for beta) the host precomputes a 12-bit code: for
+// each of the 6 member pairs {0,1},{2,3},{0,2},{1,3},{0,3},{1,2} whether the
+// string rotates it and with which sign.
 //
 // Kernels:
-//   k_stage_fwd     forward gates of a stage on one vector; dagger runs U^+;
-//                   optional derivative insertion T_g after gate g
-//   k_stage_rev     reverse sweep on (psi, lambda): per-entry partial dots
-//                   <lambda|T psi>, then un-apply, for every gate of the stage
-//   k_tile_grad     grad[g] = 2 * sum over tiles of the gate's partial dots
+//   k_stage_mats    per stage, pair and orbit type: the product of the pair's
+//                   gates (optionally with T_g inserted) from theta
+//   k_stage_fwd     one stage on one vector; dagger runs U^+
+//   k_stage_rev     reverse stage on (psi, lambda): antisymmetric orbit
+//                   products, then un-apply the pair matrices
+//   k_stage_grad    grad[g] from the orbit products summed over tiles
 
 #include <algorithm>
 #include <cmath>
@@ -47,7 +46,8 @@ constexpr int kMaxPairs = 6;        // hypercubes of at most 64 strings per spin
 constexpr int kTileMax = 4096;      // doubles per vector per tile (large tiles)
 constexpr int kSmallMax = 1024;     // largest tile run by the one-wave kernel
 constexpr int kTileMaxCols = 1024;  // columns per tile
-constexpr int kTB = 256;            // threads per block of the large-tile kernel and reductions
+constexpr int kTB = 256;            // threads per block of the reductions
+constexpr int kLargeNT = 1024;      // threads per block of the large-tile kernels
 constexpr int kMaxStageGates = 48;  // stage entries (sa_single counts twice)
 constexpr int kMaxPairEntries = 8;  // entries per pair within a stage
 constexpr int kTypes = 9;           // orbit types (alpha kind x beta kind)
@@ -90,6 +90,14 @@ constexpr int kMatVals = 36;
 // descriptor: kind (2 bits: 0 pair empty, 1 pair doubly occupied, 2 singly
 // occupied with hypercube bit 0, 3 singly occupied with hypercube bit 1) and
 // the hypercube bit index (3 bits).
+// One tile: alpha rows arows[r0, r0+R), beta stage positions [c0, c1), and
+// the alpha group's pair descriptor.
+struct TileRec {
+  int32_t r0, R, c0, c1;
+  uint32_t adesc;
+  int32_t pad[3];
+};
+
 struct StageDev {
   int npairs;
   int nsb;
@@ -100,11 +108,9 @@ struct StageDev {
   int64_t poff;              // this stage's block of the gradient partials
   const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
-  const int32_t* agoff;      // [ngroups+1]
-  const uint32_t* adesc;     // per alpha group: descriptor (kind 2 for every single pair)
   const uint32_t* bdesc;     // per beta stage position: descriptor
   const int32_t* bcol;       // per beta stage position: canonical beta column
-  const int4* tiles;         // (alpha group, col0, col1, -)
+  const struct TileRec* tiles;
 };
 
 // Per tiled stage, what the matrix and gradient kernels need.
@@ -397,15 +403,18 @@ static void finalize_stage(const Space& sp, Stage& S) {
     adesc[ga] = d;
   }
   for (int k = 0; k < sp.nsb; ++k) bdesc[k] = pair_desc(B.occ[k], B.bit[k], S.npairs);
+  std::vector<TileRec> recs(tiles.size());
+  for (size_t t = 0; t < tiles.size(); ++t) {
+    const int ga = tiles[t].x;
+    recs[t] = TileRec{A.goff[ga], A.goff[ga + 1] - A.goff[ga], tiles[t].y, tiles[t].z, adesc[ga], {0, 0, 0}};
+  }
   StageDev d{};
   d.npairs = S.npairs;
   d.nsb = sp.nsb;
   d.arows = upload(S.allocs, A.order);
-  d.agoff = upload(S.allocs, A.goff);
-  d.adesc = upload(S.allocs, adesc);
   d.bdesc = upload(S.allocs, bdesc);
   d.bcol = upload(S.allocs, B.order);
-  d.tiles = upload(S.allocs, tiles);
+  d.tiles = upload(S.allocs, recs);
   S.d_entries = upload(S.allocs, S.entries);
   const char* xenv = getenv("SQ_XCD_REMAP");
   d.xcd = (xenv && xenv[0] == '0') ? 0 : 1;
@@ -563,11 +572,13 @@ const TilePlan* tiles_for(const Space& s, const Program& p) {
 
 // R M for the rotation of one entry code (disjoint member planes).
 __device__ __forceinline__ void rot_rows(uint32_t code, double c, double s, double (&M)[16]) {
+#pragma unroll
   for (int sl = 0; sl < 6; ++sl) {
     const uint32_t f = (code >> (2 * sl)) & 3u;
     if (!(f & 1u)) continue;
     const int a = kSlotA[sl], b = kSlotB[sl];
     const double sg = (f & 2u) ? -s : s;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double x = M[a * 4 + q], y = M[b * 4 + q];
       M[a * 4 + q] = c * x - sg * y;
@@ -578,11 +589,13 @@ __device__ __forceinline__ void rot_rows(uint32_t code, double c, double s, doub
 
 // M R^T (columns), the other half of a conjugation.
 __device__ __forceinline__ void rot_cols(uint32_t code, double c, double s, double (&M)[16]) {
+#pragma unroll
   for (int sl = 0; sl < 6; ++sl) {
     const uint32_t f = (code >> (2 * sl)) & 3u;
     if (!(f & 1u)) continue;
     const int a = kSlotA[sl], b = kSlotB[sl];
     const double sg = (f & 2u) ? -s : s;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double x = M[q * 4 + a], y = M[q * 4 + b];
       M[q * 4 + a] = c * x - sg * y;
@@ -593,11 +606,13 @@ __device__ __forceinline__ void rot_cols(uint32_t code, double c, double s, doub
 
 // out += scale * T M (rows), T|a> = sg|b>, T|b> = -sg|a> per active plane.
 __device__ __forceinline__ void t_rows(uint32_t code, double scale, const double (&M)[16], double (&out)[16]) {
+#pragma unroll
   for (int sl = 0; sl < 6; ++sl) {
     const uint32_t f = (code >> (2 * sl)) & 3u;
     if (!(f & 1u)) continue;
     const int a = kSlotA[sl], b = kSlotB[sl];
     const double sg = (f & 2u) ? -scale : scale;
+#pragma unroll
     for (int q = 0; q < 4; ++q) {
       out[b * 4 + q] += sg * M[a * 4 + q];
       out[a * 4 + q] -= sg * M[b * 4 + q];
@@ -609,7 +624,6 @@ __device__ __forceinline__ void t_rows(uint32_t code, double scale, const double
 // beta kind), offset, second member.
 __device__ __constant__ int c_cls_type[5] = {8, 6, 7, 2, 5};
 __device__ __constant__ int c_cls_off[5] = {0, 16, 20, 24, 28};
-__device__ __constant__ int c_cls_m1[5] = {0, 1, 1, 2, 2};
 
 __device__ __forceinline__ void entry_cs(const StageEntry& en, const double* __restrict__ theta, double& c,
                                          double& s) {
@@ -621,13 +635,14 @@ __device__ __forceinline__ void entry_cs(const StageEntry& en, const double* __r
 
 // One thread per (stage, pair, class): the pair's gate product in forward
 // order, with T_g inserted after gate g when deriv_gate is on this pair.
-__global__ void k_stage_mats(const StageInfo* __restrict__ info, const double* __restrict__ theta, int deriv_gate,
+__global__ __launch_bounds__(64) void k_stage_mats(const StageInfo* __restrict__ info, const double* __restrict__ theta, int deriv_gate,
                              double* __restrict__ mats) {
   const StageInfo I = info[blockIdx.x];
   const int j = threadIdx.x / 5, cls = threadIdx.x % 5;
   if (j >= I.npairs) return;
   const int t = c_cls_type[cls];
   double M[16];
+  #pragma unroll
   for (int q = 0; q < 16; ++q) M[q] = (q % 5 == 0) ? 1.0 : 0.0;
   const int k0 = I.pe[j], k1 = I.pe[j + 1];
   bool has_deriv = false;
@@ -639,8 +654,10 @@ __global__ void k_stage_mats(const StageInfo* __restrict__ info, const double* _
     if (en.gate == deriv_gate && (k + 1 == k1 || I.entries[k + 1].gate != deriv_gate)) {
       has_deriv = true;
       double o[16];
+      #pragma unroll
       for (int q = 0; q < 16; ++q) o[q] = 0.0;
       for (int q = k; q >= k0 && I.entries[q].gate == deriv_gate; --q) t_rows(I.entries[q].code[t], I.entries[q].scale, M, o);
+      #pragma unroll
       for (int q = 0; q < 16; ++q) M[q] = o[q];
     }
   }
@@ -648,13 +665,14 @@ __global__ void k_stage_mats(const StageInfo* __restrict__ info, const double* _
   if (cls == 0) base[kPairVals] = has_deriv ? 0.0 : 1.0;
   double* out = base + c_cls_off[cls];
   if (cls == 0) {
+#pragma unroll
     for (int q = 0; q < 16; ++q) out[q] = M[q];
   } else {
-    const int m1 = c_cls_m1[cls];
+    const bool a = cls <= 2;  // second member 1 (alpha pair) or 2 (beta pair)
     out[0] = M[0];
-    out[1] = M[m1];
-    out[2] = M[m1 * 4];
-    out[3] = M[m1 * 4 + m1];
+    out[1] = a ? M[1] : M[2];
+    out[2] = a ? M[4] : M[8];
+    out[3] = a ? M[5] : M[10];
   }
 }
 
@@ -662,29 +680,38 @@ __global__ void k_stage_mats(const StageInfo* __restrict__ info, const double* _
 // entry k of the pair dot_k = sum_{a<b} (W T_k W^T)[a][b] Q[a<b] per class, W
 // the product of the pair's entries after k; grad[gate] = 2 sum over its
 // entries of scale * dot.
-__device__ __constant__ int c_qa[6] = {0, 0, 0, 1, 1, 2};
-__device__ __constant__ int c_qb[6] = {1, 2, 3, 2, 3, 3};
+constexpr int kQa[6] = {0, 0, 0, 1, 1, 2};
+constexpr int kQb[6] = {1, 2, 3, 2, 3, 3};
 
-__global__ void k_stage_grad(const StageInfo* __restrict__ info, const double* __restrict__ theta,
+__global__ __launch_bounds__(kTB) void k_stage_grad(const StageInfo* __restrict__ info, const double* __restrict__ theta,
                              const double* __restrict__ partial, double* __restrict__ grad) {
   __shared__ double Q[kQVals];
   __shared__ double dots[kMaxPairEntries * 5];
-  __shared__ double red[kTB / 64];
+  __shared__ double red[kQVals * (kTB / 64)];
   const StageInfo I = info[blockIdx.x / kMaxPairs];
   const int j = blockIdx.x % kMaxPairs;
   if (j >= I.npairs) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int v = 0; v < kQVals; ++v) {
-    const double* src = partial + I.poff + ((size_t)j * kQVals + v) * I.ntiles;
-    double acc = 0.0;
-    for (int tile = threadIdx.x; tile < I.ntiles; tile += blockDim.x) acc += src[tile];
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (lane == 0) red[wave] = acc;
+  {
+    const double* src = partial + I.poff + (size_t)j * kQVals * I.ntiles;
+    double acc[kQVals];
+#pragma unroll
+    for (int v = 0; v < kQVals; ++v) acc[v] = 0.0;
+    for (int tile = threadIdx.x; tile < I.ntiles; tile += blockDim.x) {
+#pragma unroll
+      for (int v = 0; v < kQVals; ++v) acc[v] += src[(size_t)v * I.ntiles + tile];
+    }
+#pragma unroll
+    for (int v = 0; v < kQVals; ++v) {
+      double a = acc[v];
+      for (int off = 32; off > 0; off >>= 1) a += __shfl_down(a, off, 64);
+      if (lane == 0) red[v * (kTB / 64) + wave] = a;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x < kQVals) {
       double tot = 0.0;
-      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
-      Q[v] = tot;
+      for (int w = 0; w < kTB / 64; ++w) tot += red[threadIdx.x * (kTB / 64) + w];
+      Q[threadIdx.x] = tot;
     }
     __syncthreads();
   }
@@ -694,8 +721,10 @@ __global__ void k_stage_grad(const StageInfo* __restrict__ info, const double* _
     const int k = k0 + threadIdx.x / 5, cls = threadIdx.x % 5;
     const int t = c_cls_type[cls];
     double M[16];
+    #pragma unroll
     for (int q = 0; q < 16; ++q) M[q] = 0.0;
     double I4[16];
+    #pragma unroll
     for (int q = 0; q < 16; ++q) I4[q] = (q % 5 == 0) ? 1.0 : 0.0;
     t_rows(I.entries[k].code[t], 1.0, I4, M);  // M = T_k
     for (int e = k + 1; e < k1; ++e) {
@@ -706,9 +735,10 @@ __global__ void k_stage_grad(const StageInfo* __restrict__ info, const double* _
     }
     double d = 0.0;
     if (cls == 0) {
-      for (int q = 0; q < 6; ++q) d += M[c_qa[q] * 4 + c_qb[q]] * Q[q];
+#pragma unroll
+      for (int q = 0; q < 6; ++q) d += M[kQa[q] * 4 + kQb[q]] * Q[q];
     } else {
-      d = M[c_cls_m1[cls]] * Q[5 + cls];
+      d = (cls <= 2 ? M[1] : M[2]) * Q[5 + cls];  // M[0][m1], m1 = 1 (alpha pair) or 2 (beta pair)
     }
     dots[threadIdx.x] = d * I.entries[k].scale;
   }
@@ -770,20 +800,19 @@ __device__ __forceinline__ int xcd_tile(int b, int n, int on) {
 // canonical indices, and the stage's pair matrices.
 template <int NT>
 __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile) {
-  const int4 T = S.tiles[S.tile0 + tile];
-  const int r0 = S.agoff[T.x];
+  const TileRec T = S.tiles[S.tile0 + tile];
   TileView t;
-  t.R = S.agoff[T.x + 1] - r0;
-  t.W = T.z - T.y;
+  t.R = T.R;
+  t.W = T.c1 - T.c0;
   t.E = t.R * t.W;
-  t.c0 = T.y;
+  t.c0 = T.c0;
   t.invW = 1.0f / (float)t.W;
-  t.adesc = S.adesc[T.x];
-  for (int r = threadIdx.x; r < t.R; r += NT) m.rows[r] = S.arows[r0 + r];
+  t.adesc = T.adesc;
+  for (int r = threadIdx.x; r < t.R; r += NT) m.rows[r] = S.arows[T.r0 + r];
 #pragma unroll 4
   for (int c = threadIdx.x; c < t.W; c += NT) {
-    m.cdesc[c] = S.bdesc[T.y + c];
-    m.ccol[c] = (uint32_t)S.bcol[T.y + c];
+    m.cdesc[c] = S.bdesc[T.c0 + c];
+    m.ccol[c] = (uint32_t)S.bcol[T.c0 + c];
   }
   for (int i = threadIdx.x; i < S.npairs * kMatVals; i += NT) m.mats[i] = S.mats[i];
   __syncthreads();
@@ -1067,7 +1096,7 @@ static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st)
             S.small, dagger ? 1 : 0, psi);
       },
       [&](hipStream_t q) {
-        k_stage_fwd<kTB><<<S.ntiles - S.nsmall, kTB, lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), q>>>(
+        k_stage_fwd<kLargeNT><<<S.ntiles - S.nsmall, kLargeNT, lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), q>>>(
             S.large, dagger ? 1 : 0, psi);
       });
 }
@@ -1081,8 +1110,8 @@ static void launch_rev(const Stage& S, double* psi, double* lam, double* partial
       },
       [&](hipStream_t q) {
         const size_t lds = lds_bytes(S.large.emax, S.large.wmax, S.npairs, true);
-        allow_lds(k_stage_rev<kTB>, lds);
-        k_stage_rev<kTB><<<S.ntiles - S.nsmall, kTB, lds, q>>>(S.large, psi, lam, partial, wb);
+        allow_lds(k_stage_rev<kLargeNT>, lds);
+        k_stage_rev<kLargeNT><<<S.ntiles - S.nsmall, kLargeNT, lds, q>>>(S.large, psi, lam, partial, wb);
       });
 }
 

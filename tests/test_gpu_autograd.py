@@ -85,3 +85,33 @@ def test_autograd_6o_against_oracle_and_chain_rule():
     assert np.abs(th.grad.cpu().numpy() - 3.0 * g_ref).max() < 1e-10
     with pytest.raises(ValueError):
         ups_energy(th[:-1], prog, H, psi0)
+
+
+def test_14o_tups6_energy_gradient_rdms():
+    """tUPS with 6 layers on the synthetic (14e,14o) Hamiltonian (rank 28,
+    variance 0.3^2/14, theta ~ N(0, 0.03^2), seed 0; CI dimension 11,778,624,
+    234 parameters): adjoint gradient against the reference sweep (1e-10) and
+    central differences on a few parameters (h = 1e-5, 1e-8); RDM identities
+    (trace, partial trace, E(Gamma) = <H>, 1e-10)."""
+    from slowquant_amd import engine as E
+
+    lay, h, g, theta, sp, prog, H, psi0 = _problem(14, 6, 28, 0.03, 0)
+    assert sp.dim == 11778624 and lay.n_params == 234
+    e_a, g_a = prog.energy_grad(H, theta, psi0, "adjoint")
+    e_r, g_r = prog.energy_grad(H, theta, psi0, "reference")
+    assert abs(e_a - e_r) < 1e-10
+    assert np.abs(g_a - g_r).max() < 1e-10
+    hstep = 1e-5
+    for i in (0, 117, 233):
+        tp = theta.copy()
+        tm = theta.copy()
+        tp[i] += hstep
+        tm[i] -= hstep
+        fd = (_energy(prog, H, tp, psi0) - _energy(prog, H, tm, psi0)) / (2 * hstep)
+        assert abs(fd - g_a[i]) < 1e-8
+    psi = prog.state_prep(theta, psi0)
+    r1, r2 = E.rdm12(sp, psi)
+    assert abs(np.trace(r1) - 14.0) < 1e-10
+    assert np.abs(np.einsum("pqrr->pq", r2) - 13.0 * r1).max() < 1e-9
+    e_rdm = np.einsum("pq,pq->", h, r1) + 0.5 * np.einsum("pqrs,pqrs->", g, r2)
+    assert abs(e_rdm - e_a) < 1e-10
