@@ -21,7 +21,11 @@
 
 namespace sq {
 
-constexpr int kBT = 512;  // threads per batch workgroup (8 waves)
+constexpr int kBT = 512;    // threads per batch workgroup (8 waves)
+constexpr int kBatchMT = 5;  // row tiles of the sigma GEMM (kpad <= 80: up to 12 orbitals)
+
+// k -> whether packed pair k = p(p+1)/2 + q is diagonal (p == q), k < 91
+__constant__ uint8_t c_bdiag[91];
 
 struct BatchSpace {
   int n, nsa, nsb, npair, K4, kpad;
@@ -117,13 +121,12 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
   double* slot = B.scratch + (size_t)blockIdx.x * B.slot_doubles;
   double* psi = slot;
   double* lam = psi + B.dmax;
-  double* D = lam + B.dmax;
   for (int li = B.wg_off[blockIdx.x]; li < B.wg_off[blockIdx.x + 1]; ++li) {
     const BatchInstance I = B.inst[B.wg_list[li]];
     const BatchSpace S = B.spaces[I.space];
     const int nsb = S.nsb;
-    const int64_t dim = (int64_t)S.nsa * nsb;
-    double* G = D + (size_t)S.K4 * dim;
+    const int dim = S.nsa * nsb;  // < 2^30, checked at creation
+    double* G = lam + B.dmax;
     const double* th = B.theta + I.theta0;
     // psi = HF determinant (alpha string 0, beta string 0) in the engine's
     // sign convention: (-1)^{sum_{p in alpha} #{q in beta : q < p}}.
@@ -155,52 +158,54 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       }
       __syncthreads();
     }
-    // D build (rows k < npair: E+_k psi; row npair: psi; pad rows: 0)
-    for_each_det(S.nsa, nsb, [&](int ia, int ib, int x) {
-      const double px = psi[x];
-      int k = 0;
-      for (int p = 0; p < S.n; ++p) {
-        for (int q = 0; q <= p; ++q, ++k) {
-          const int32_t ea = S.aexc[(size_t)k * S.nsa + ia];
-          const int32_t eb = S.bexc[(size_t)k * nsb + ib];
-          double v;
-          if (p == q) {
-            v = (double)(ea + eb) * px;
-          } else {
-            v = 0.0;
-            if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
-            if (eb) v += (eb > 0 ? 1.0 : -1.0) * psi[(size_t)ia * nsb + abs(eb) - 1];
-          }
-          D[(size_t)k * dim + x] = v;
-        }
-      }
-      D[(size_t)S.npair * dim + x] = px;
-      for (int k2 = S.npair + 1; k2 < S.K4; ++k2) D[(size_t)k2 * dim + x] = 0.0;
-    });
     // stage A (K4 x kpad, transposed coefficient matrix) in LDS
     const double* Ag = B.A + I.A_off;
     for (int i = threadIdx.x; i < S.K4 * S.kpad; i += blockDim.x) sA[i] = Ag[i];
     __syncthreads();
-    // G = A . D with v_mfma_f64_16x16x4f64; each wave walks 16-column tiles
+    // G = A . D with v_mfma_f64_16x16x4f64, the B operand D[k][c] = (E+_k psi)[c]
+    // (row npair: psi) generated in registers: each wave walks 16-column tiles
+    // and keeps all kpad/16 row tiles in flight, so D is never stored.
     {
       typedef double double4_t __attribute__((ext_vector_type(4)));
       const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, li = lane & 15;
       const int nwaves = blockDim.x >> 6;
       const int MT = S.kpad / 16;
-      for (int64_t c0 = (int64_t)wave * 16; c0 < dim; c0 += (int64_t)nwaves * 16) {
-        const int64_t c = c0 + li;
+      for (int c0 = wave * 16; c0 < dim; c0 += nwaves * 16) {
+        const int c = c0 + li;
         const bool ok = c < dim;
-        for (int mt = 0; mt < MT; ++mt) {
-          double4_t acc = double4_t{0.0, 0.0, 0.0, 0.0};
-          for (int k4 = 0; k4 < S.K4; k4 += 4) {
-            const double b = ok ? D[(size_t)(k4 + kq) * dim + c] : 0.0;
-            const double a = sA[(size_t)(k4 + kq) * S.kpad + mt * 16 + li];
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-          }
-          if (ok) {
+        const int ia = ok ? c / nsb : 0, ib = ok ? c - ia * nsb : 0;
+        const double pc = ok ? psi[c] : 0.0;
+        auto dval = [&](int k) -> double {
+          if (!ok || k > S.npair) return 0.0;
+          if (k == S.npair) return pc;
+          const int32_t ea = S.aexc[k * S.nsa + ia];
+          const int32_t eb = S.bexc[k * nsb + ib];
+          if (c_bdiag[k]) return (double)(ea + eb) * pc;
+          double v = 0.0;
+          if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(abs(ea) - 1) * nsb + ib];
+          if (eb) v += (eb > 0 ? 1.0 : -1.0) * psi[ia * nsb + abs(eb) - 1];
+          return v;
+        };
+        double4_t acc[kBatchMT];
+#pragma unroll
+        for (int t = 0; t < kBatchMT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+        double bnext = dval(kq);
+        for (int k4 = 0; k4 < S.K4; k4 += 4) {
+          const double bv = bnext;
+          if (k4 + 4 < S.K4) bnext = dval(k4 + 4 + kq);
+          const double* arow = sA + (k4 + kq) * S.kpad + li;
+#pragma unroll
+          for (int t = 0; t < kBatchMT; ++t)
+            if (t < MT) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], bv, acc[t], 0, 0, 0);
+        }
+        if (ok) {
+#pragma unroll
+          for (int t = 0; t < kBatchMT; ++t) {
+            if (t >= MT) continue;
+#pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int m = mt * 16 + kq + 4 * r;
-              if (m < S.npair) G[(size_t)m * dim + c] = acc[r];
+              const int m = t * 16 + kq + 4 * r;
+              if (m < S.npair) G[(size_t)m * dim + c] = acc[t][r];
             }
           }
         }
@@ -303,6 +308,11 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
                     const sq_hamiltonian* const* hams, int n_workgroups, sq_batch** out) {
   return guarded([&] {
     SQ_CHECK(n_inst > 0 && spaces && programs && hams && out, SQ_ERR_ARG, "invalid arguments");
+    {
+      uint8_t diag[91] = {0};
+      for (int p = 0; p < 13; ++p) diag[p * (p + 1) / 2 + p] = 1;
+      SQ_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_bdiag), diag, sizeof(diag)));
+    }
     auto* bo = new sq_batch();
     Batch& B = bo->b;
     try {
@@ -413,11 +423,13 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
         I.e_core = H.e_core;
         B.inst.push_back(I);
         B.dmax = std::max<int64_t>(B.dmax, s.dim);
-        B.slot = std::max<int64_t>(B.slot, (int64_t)(2 + bs.K4 + bs.npair) * s.dim);
+        B.slot = std::max<int64_t>(B.slot, (int64_t)(2 + bs.npair) * s.dim);
         B.lds = std::max<size_t>(B.lds, sizeof(double) * bs.K4 * bs.kpad);
         cost[i] = s.dim * (int64_t)(2 * I.ngate + 4 + 2 * bs.npair + (bs.npair * bs.K4) / 8);
       }
       SQ_CHECK(B.lds <= 65536, SQ_ERR_ARG, "batched spaces limited to 12 orbitals (LDS)");
+      for (const auto& bs : bsp)
+        SQ_CHECK(bs.kpad <= 16 * kBatchMT && bs.npair < 91, SQ_ERR_ARG, "batched spaces limited to 12 orbitals");
       B.ngrad = grad_off;
       // longest processing time first, each to the least-loaded workgroup
       int nwg = n_workgroups > 0 ? n_workgroups : 256;

@@ -182,8 +182,6 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
                                                     const int32_t* __restrict__ aexc, const int32_t* __restrict__ bexc,
                                                     double* __restrict__ G, int npair, int K4, int kpad, int nsa,
                                                     int nsb) {
-  // Each block stages its slice of At in LDS; waves walk 16-determinant
-  // column tiles with a grid-wide stride.
   extern __shared__ double sA[];
   constexpr int W = MT * 16;
   const int m0 = blockIdx.y * W;
@@ -194,48 +192,43 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
   __syncthreads();
   const int64_t dim = (int64_t)nsa * nsb;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t c = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 16 + (lane & 15);
   const int kq = lane >> 4;
-  const int64_t ntile = (dim + 15) / 16;
-  const int64_t wstride = (int64_t)gridDim.x * (blockDim.x >> 6);
+  const bool valid = c < dim;
+  const int ia = valid ? (int)(c / nsb) : 0;
+  const int ib = valid ? (int)(c - (int64_t)ia * nsb) : 0;
+  const double* row = psi + (size_t)ia * nsb;
+  const double pc = valid ? psi[c] : 0.0;
+  auto dval = [&](int k) -> double {
+    if (!valid || k > npair) return 0.0;
+    if (k == npair) return pc;
+    const int32_t ea = aexc[(size_t)k * nsa + ia];
+    const int32_t eb = bexc[(size_t)k * nsb + ib];
+    if (c_kp[k] == c_kq[k]) return (double)(ea + eb) * pc;
+    double v = 0.0;
+    if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
+    if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
+    return v;
+  };
   typedef double double4_t __attribute__((ext_vector_type(4)));
-  for (int64_t tile = (int64_t)blockIdx.x * (blockDim.x >> 6) + wave; tile < ntile; tile += wstride) {
-    const int64_t c = tile * 16 + (lane & 15);
-    const bool valid = c < dim;
-    const int ia = valid ? (int)(c / nsb) : 0;
-    const int ib = valid ? (int)(c - (int64_t)ia * nsb) : 0;
-    const double* row = psi + (size_t)ia * nsb;
-    const double pc = valid ? psi[c] : 0.0;
-    auto dval = [&](int k) -> double {
-      if (!valid || k > npair) return 0.0;
-      if (k == npair) return pc;
-      const int32_t ea = aexc[(size_t)k * nsa + ia];
-      const int32_t eb = bexc[(size_t)k * nsb + ib];
-      if (c_kp[k] == c_kq[k]) return (double)(ea + eb) * pc;
-      double v = 0.0;
-      if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
-      if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
-      return v;
-    };
-    double4_t acc[MT];
+  double4_t acc[MT];
 #pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-    double bnext = dval(kq);
-    for (int k4 = 0; k4 < K4; k4 += 4) {
-      const double b = bnext;
-      if (k4 + 4 < K4) bnext = dval(k4 + 4 + kq);
-      const double* arow = sA + (size_t)(k4 + kq) * W + (lane & 15);
+  for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  double bnext = dval(kq);
+  for (int k4 = 0; k4 < K4; k4 += 4) {
+    const double b = bnext;
+    if (k4 + 4 < K4) bnext = dval(k4 + 4 + kq);
+    const double* arow = sA + (size_t)(k4 + kq) * W + (lane & 15);
 #pragma unroll
-      for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
-    }
-    if (valid) {
+    for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+  }
+  if (!valid) return;
 #pragma unroll
-      for (int t = 0; t < MT; ++t) {
+  for (int t = 0; t < MT; ++t) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          int m = m0 + t * 16 + kq + 4 * r;
-          if (m < npair) G[(size_t)m * dim + c] = acc[t][r];
-        }
-      }
+    for (int r = 0; r < 4; ++r) {
+      int m = m0 + t * 16 + kq + 4 * r;
+      if (m < npair) G[(size_t)m * dim + c] = acc[t][r];
     }
   }
 }
@@ -260,10 +253,8 @@ static void launch_gemm_fused(const Space& s, const SigmaDims& d, const double* 
   int mt = mt_total;
   while (mt > 1 && (size_t)d.K4 * mt * 16 * sizeof(double) > 65536) --mt;
   const int groups = (mt_total + mt - 1) / mt;
+  dim3 grid((unsigned)((d.dim + cols_per_block - 1) / cols_per_block), groups);
   size_t lds = sizeof(double) * (size_t)d.K4 * mt * 16;
-  // one 16-column tile per wave (a grid-stride loop over fewer blocks measured slower)
-  const int64_t need = (d.dim + cols_per_block - 1) / cols_per_block;
-  dim3 grid((unsigned)need, groups);
   // algorithmic traffic: psi read once per m-group (+ gathers hitting cache), G written
   const double gbytes = 8.0 * (double)d.dim * (groups + d.npair);
   const double gflops = 2.0 * (double)d.npair * (d.npair + 1) * (double)d.dim;

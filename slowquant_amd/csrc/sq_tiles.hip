@@ -105,6 +105,7 @@ struct StageDev {
   int xcd;                   // 1: XCD-contiguous tile ranges (SQ_XCD_REMAP=0 turns it off)
   int emax, wmax;            // largest tile of this launch: elements, columns
   int ntiles_stage;          // tiles of the whole stage (small and large)
+  int nt;                    // tiles of this launch
   int64_t poff;              // this stage's block of the gradient partials
   const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
@@ -424,6 +425,8 @@ static void finalize_stage(const Space& sp, Stage& S) {
   S.large.tile0 = S.nsmall;
   S.small.ntiles_stage = S.ntiles;
   S.large.ntiles_stage = S.ntiles;
+  S.small.nt = S.nsmall;
+  S.large.nt = S.ntiles - S.nsmall;
   auto extent = [&](int t0, int t1, StageDev& D) {
     D.emax = 0;
     D.wmax = 0;
@@ -797,7 +800,13 @@ __device__ __forceinline__ int xcd_tile(int b, int n, int on) {
 }
 
 // Tile descriptors into LDS: alpha rows, the columns' pair descriptors and
-// canonical indices, and the stage's pair matrices.
+// canonical indices.  The stage's pair matrices are loaded once per block
+// (load_mats).
+template <int NT>
+__device__ __forceinline__ void load_mats(const StageDev& S, const TileLds& m) {
+  for (int i = threadIdx.x; i < S.npairs * kMatVals; i += NT) m.mats[i] = S.mats[i];
+}
+
 template <int NT>
 __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile) {
   const TileRec T = S.tiles[S.tile0 + tile];
@@ -814,7 +823,6 @@ __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds&
     m.cdesc[c] = S.bdesc[T.c0 + c];
     m.ccol[c] = (uint32_t)S.bcol[T.c0 + c];
   }
-  for (int i = threadIdx.x; i < S.npairs * kMatVals; i += NT) m.mats[i] = S.mats[i];
   __syncthreads();
   return t;
 }
@@ -915,13 +923,37 @@ __device__ __forceinline__ void orbit_apply(double* X, const Orbit& ob, const do
   }
 }
 
+// Sum of a double over the 64 lanes of a wave with DPP moves (quad_perm
+// swaps, row_shr 4/8, row_bcast 15/31; no LDS traffic); the total lands in
+// lane 63.  Every lane of the wave must be active.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_move(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROWS, 0xf, true);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]: quad sums
+  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_move<0x118, 0xf>(v);  // row_shr:8: row sums in lanes 15 of each row
+  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
 // Gates of one stage on one vector (canonical order, in place).
 // dagger: U^+ (transposed pair matrices).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, int dagger, double* __restrict__ psi) {
   extern __shared__ double smem[];
   const TileLds m = carve_lds(smem, S, false);
-  const TileView t = tile_setup<NT>(S, m, xcd_tile(blockIdx.x, gridDim.x, S.xcd));
+  load_mats<NT>(S, m);
+  // persistent blocks: block b walks tiles b, b + grid, ... (grid is a
+  // multiple of 8, so every tile of a block maps into its XCD's range)
+  for (int ti = blockIdx.x; ti < S.nt; ti += gridDim.x) {
+  const TileView t = tile_setup<NT>(S, m, xcd_tile(ti, S.nt, S.xcd));
   tile_load<NT>(S, t, m, psi, m.X);
   __syncthreads();
   for (int j = 0; j < S.npairs; ++j) {
@@ -946,6 +978,8 @@ __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, int dagger, double
     __syncthreads();
   }
   tile_store<NT>(S, t, m, psi, m.X);
+  __syncthreads();  // the next tile overwrites the LDS tables
+  }
 }
 
 // Reverse stage on (psi, lam), in place: per pair, the antisymmetric orbit
@@ -957,7 +991,9 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, double* __restrict
   extern __shared__ double smem[];
   __shared__ double red[kQVals][NT / 64];
   const TileLds m = carve_lds(smem, S, true);
-  const int tile = xcd_tile(blockIdx.x, gridDim.x, S.xcd);
+  load_mats<NT>(S, m);
+  for (int ti = blockIdx.x; ti < S.nt; ti += gridDim.x) {
+  const int tile = xcd_tile(ti, S.nt, S.xcd);
   const TileView t = tile_setup<NT>(S, m, tile);
   tile_load<NT>(S, t, m, psi, m.X);
   tile_load<NT>(S, t, m, lam, m.L);
@@ -999,11 +1035,10 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, double* __restrict
     }
 #pragma unroll
     for (int q = 0; q < kQVals; ++q) {
-      double d = Q[q];
-      for (int off = 32; off > 0; off >>= 1) d += __shfl_down(d, off, 64);
+      const double d = wave_sum_lane63(Q[q]);
       if (NT == 64) {
-        if (lane == 0) partial[S.poff + ((size_t)j * kQVals + q) * ntiles_stage + my_tile] = d;
-      } else if (lane == 0) {
+        if (lane == 63) partial[S.poff + ((size_t)j * kQVals + q) * ntiles_stage + my_tile] = d;
+      } else if (lane == 63) {
         red[q][wave] = d;
       }
     }
@@ -1017,9 +1052,12 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, double* __restrict
     }
     __syncthreads();
   }
-  if (!write_back) return;  // the first stage of the program: the un-applied states are not needed
-  tile_store<NT>(S, t, m, psi, m.X);
-  tile_store<NT>(S, t, m, lam, m.L);
+  if (write_back) {  // not for the first stage of the program: the un-applied states are not needed
+    tile_store<NT>(S, t, m, psi, m.X);
+    tile_store<NT>(S, t, m, lam, m.L);
+  }
+  __syncthreads();  // the next tile overwrites the LDS tables
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1027,46 +1065,19 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, double* __restrict
 
 static double stage_bytes(const Space& s, int vectors) { return 16.0 * (double)s.dim * vectors; }
 
-// Large tiles run on a side stream concurrently with the small-tile launch
-// (they touch disjoint elements): fork/join through events, so the few
-// large blocks overlap the bulk instead of forming a tail.
-struct SideStream {
-  hipStream_t s = nullptr;
-  hipEvent_t fork = nullptr, join = nullptr;
-};
-
-static SideStream& side_stream() {
-  static std::mutex mu;
-  static SideStream per_dev[64];
-  int dev = 0;
-  SQ_HIP(hipGetDevice(&dev));
-  SQ_CHECK(dev >= 0 && dev < 64, SQ_ERR_ARG, "device index out of range");
-  std::lock_guard<std::mutex> lock(mu);
-  SideStream& S = per_dev[dev];
-  if (!S.s) {
-    SQ_HIP(hipStreamCreateWithFlags(&S.s, hipStreamNonBlocking));
-    SQ_HIP(hipEventCreateWithFlags(&S.fork, hipEventDisableTiming));
-    SQ_HIP(hipEventCreateWithFlags(&S.join, hipEventDisableTiming));
-  }
-  return S;
-}
-
-template <class SmallL, class LargeL>
-static void fork_join(const Stage& S, hipStream_t st, SmallL small, LargeL large) {
-  if (S.ntiles > S.nsmall && S.nsmall > 0) {
-    SideStream& side = side_stream();
-    SQ_HIP(hipEventRecord(side.fork, st));
-    SQ_HIP(hipStreamWaitEvent(side.s, side.fork, 0));
-    large(side.s);
-    SQ_HIP(hipGetLastError());
-    SQ_HIP(hipEventRecord(side.join, side.s));
-    small(st);
-    SQ_HIP(hipStreamWaitEvent(st, side.join, 0));
-  } else if (S.nsmall > 0) {
-    small(st);
-  } else if (S.ntiles > 0) {
-    large(st);
-  }
+// Persistent grids: a multiple of 8 blocks (one contiguous tile range per
+// XCD), at most blocks_per_cu blocks per CU (SQ_TILE_BPC overrides it for
+// the one-wave kernels), never more than the tiles.
+static unsigned persistent_grid(int ntiles, int blocks_per_cu) {
+  static const int env_bpc = [] {
+    const char* e = getenv("SQ_TILE_BPC");
+    return e ? std::max(1, atoi(e)) : 0;
+  }();
+  if (env_bpc > 0 && blocks_per_cu > 1) blocks_per_cu = env_bpc;
+  const int cap = 256 * blocks_per_cu;
+  int g = std::min(ntiles, cap);
+  g = (g + 7) / 8 * 8;
+  return (unsigned)std::max(g, 8);
 }
 
 // Dynamic LDS above 64 KB (large-tile reverse sweep) must be enabled per
@@ -1088,31 +1099,26 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
           k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
 }
 
+// Large tiles (whole 1024-thread blocks) run first, then the one-wave tiles.
 static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st) {
-  fork_join(
-      S, st,
-      [&](hipStream_t q) {
-        k_stage_fwd<64><<<S.nsmall, 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), q>>>(
-            S.small, dagger ? 1 : 0, psi);
-      },
-      [&](hipStream_t q) {
-        k_stage_fwd<kLargeNT><<<S.ntiles - S.nsmall, kLargeNT, lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), q>>>(
-            S.large, dagger ? 1 : 0, psi);
-      });
+  if (S.ntiles > S.nsmall)
+    k_stage_fwd<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT,
+                            lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), st>>>(S.large, dagger ? 1 : 0, psi);
+  if (S.nsmall > 0)
+    k_stage_fwd<64><<<persistent_grid(S.nsmall, 8), 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), st>>>(
+        S.small, dagger ? 1 : 0, psi);
 }
 
 static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
-  fork_join(
-      S, st,
-      [&](hipStream_t q) {
-        k_stage_rev<64><<<S.nsmall, 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, true), q>>>(
-            S.small, psi, lam, partial, wb);
-      },
-      [&](hipStream_t q) {
-        const size_t lds = lds_bytes(S.large.emax, S.large.wmax, S.npairs, true);
-        allow_lds(k_stage_rev<kLargeNT>, lds);
-        k_stage_rev<kLargeNT><<<S.ntiles - S.nsmall, kLargeNT, lds, q>>>(S.large, psi, lam, partial, wb);
-      });
+  if (S.ntiles > S.nsmall) {
+    const size_t lds = lds_bytes(S.large.emax, S.large.wmax, S.npairs, true);
+    allow_lds(k_stage_rev<kLargeNT>, lds);
+    k_stage_rev<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT, lds, st>>>(S.large, psi, lam, partial,
+                                                                                          wb);
+  }
+  if (S.nsmall > 0)
+    k_stage_rev<64><<<persistent_grid(S.nsmall, 8), 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, true), st>>>(
+        S.small, psi, lam, partial, wb);
 }
 
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
