@@ -87,7 +87,7 @@ __global__ void k_sigma_gather(int n, int nsa, int nsb, const int32_t* __restric
                                const int32_t* __restrict__ bexc, const double* __restrict__ G,
                                const double* __restrict__ psi, double e_core, double* __restrict__ sigma,
                                double* __restrict__ partial) {
-  __shared__ double red[kB / 64];
+  __shared__ double red[16];  // up to 1024 threads
   int ia = blockIdx.y;
   int ib = blockIdx.x * blockDim.x + threadIdx.x;
   const size_t dim = (size_t)nsa * nsb;
@@ -297,6 +297,10 @@ static void launch_gemm(const SigmaDims& d, const double* At, const double* D, d
   }
 }
 
+// The sigma gather walks whole beta rows per block (up to 1024 threads), so a
+// block reads each row of every G_k once.
+static int gather_threads(const Space& s) { return (int)std::min<int64_t>(1024, (s.nsb + 63) / 64 * 64); }
+
 static bool fused_sigma() {
   static int v = -1;
   if (v < 0) {
@@ -319,6 +323,9 @@ void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double*
   double* scal = partial + nb;
   dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
   const double d8 = 8.0 * (double)d.dim;
+  const int gth = gather_threads(s);
+  const dim3 ggrid((s.nsb + gth - 1) / gth, s.nsa);
+  const int64_t gnb = (int64_t)ggrid.x * ggrid.y;
   if (fused_sigma()) {
     launch_gemm_fused(s, d, H.d_A, psi, G, st);
   } else {
@@ -327,9 +334,9 @@ void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double*
     launch_gemm(d, H.d_A, D, G, st);
   }
   SQ_PROF(PC_SIGMA_GATHER, st, d8 * (d.npair + 2), 0.0,
-          k_sigma_gather<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
+          k_sigma_gather<<<ggrid, gth, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
   if (energy_host) {
-    reduce_impl(partial, nb, scal, 0, false, 1.0, st);
+    reduce_impl(partial, gnb, scal, 0, false, 1.0, st);
     SQ_HIP(hipMemcpyAsync(energy_host, scal, sizeof(double), hipMemcpyDeviceToHost, st));
     SQ_HIP(hipStreamSynchronize(st));
   }
@@ -347,6 +354,9 @@ void sigma_energy_dev(const Space& s, const Hamiltonian& H, const double* psi, d
   int64_t nb = (int64_t)s.nsa * ((s.nsb + kB - 1) / kB);
   dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
   const double d8 = 8.0 * (double)d.dim;
+  const int gth = gather_threads(s);
+  const dim3 ggrid((s.nsb + gth - 1) / gth, s.nsa);
+  const int64_t gnb = (int64_t)ggrid.x * ggrid.y;
   if (fused_sigma()) {
     launch_gemm_fused(s, d, H.d_A, psi, G, st);
   } else {
@@ -355,8 +365,8 @@ void sigma_energy_dev(const Space& s, const Hamiltonian& H, const double* psi, d
     launch_gemm(d, H.d_A, D, G, st);
   }
   SQ_PROF(PC_SIGMA_GATHER, st, d8 * (d.npair + 2), 0.0,
-          k_sigma_gather<<<grid, kB, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
-  reduce_impl(partial, nb, e_dev, 0, false, 1.0, st);
+          k_sigma_gather<<<ggrid, gth, 0, st>>>(s.n, s.nsa, s.nsb, ae, be, G, psi, H.e_core, sigma, partial));
+  reduce_impl(partial, gnb, e_dev, 0, false, 1.0, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -35,6 +35,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <mutex>
 #include <numeric>
 
@@ -44,10 +45,10 @@ namespace sq {
 
 constexpr int kMaxPairs = 6;        // hypercubes of at most 64 strings per spin
 constexpr int kTileMax = 4096;      // doubles per vector per tile (large tiles)
-constexpr int kSmallMax = 1024;     // largest tile run by the one-wave kernel
 constexpr int kTileMaxCols = 1024;  // columns per tile
 constexpr int kTB = 256;            // threads per block of the reductions
 constexpr int kLargeNT = 1024;      // threads per block of the large-tile kernels
+constexpr int kSmallNT = 512;       // threads per block of the target-size tile kernels
 constexpr int kMaxStageGates = 48;  // stage entries (sa_single counts twice)
 constexpr int kMaxPairEntries = 8;  // entries per pair within a stage
 constexpr int kTypes = 9;           // orbit types (alpha kind x beta kind)
@@ -332,10 +333,13 @@ static T* upload(std::vector<void*>& allocs, const std::vector<T>& v) {
 
 // Elements per one-wave tile (SQ_TILE_TARGET overrides); tiles above it (a
 // single group pair larger than the target) run in the 256-thread kernel.
-static int tile_target() {
+static int tile_target(int64_t dim) {
   const char* env = getenv("SQ_TILE_TARGET");
-  int t = env ? atoi(env) : 512;
-  return std::min(std::max(t, 16), kSmallMax);
+  if (env) return std::min(std::max(atoi(env), 16), kTileMax);
+  // about 256 tiles or more (one per CU), 256..4096 elements each
+  int t = 256;
+  while (t < kTileMax && (int64_t)t * 2 * 256 <= dim) t *= 2;
+  return t;
 }
 
 // 5-bit pair fields of a string given its local occupancies (2 bits per pair)
@@ -366,7 +370,7 @@ static void finalize_stage(const Space& sp, Stage& S) {
   GroupInfo B = group_strings(sp.h_bstr, S.npairs, S.pp);
   // tiles: each alpha group x runs of whole beta groups, up to the target
   // size; a beta group that alone exceeds the target gets its own tile
-  const int target = tile_target();
+  const int target = tile_target(sp.dim);
   std::vector<int4> small, large;
   const int nag = (int)A.goff.size() - 1;
   const int nbg = (int)B.goff.size() - 1;
@@ -378,7 +382,7 @@ static void finalize_stage(const Space& sp, Stage& S) {
       const int c0 = B.goff[gb];
       int c1 = B.goff[gb + 1];
       ++gb;
-      while (gb < nbg && (int64_t)R * (B.goff[gb + 1] - c0) <= target) {
+      while (gb < nbg && (int64_t)R * (B.goff[gb + 1] - c0) <= target && B.goff[gb + 1] - c0 <= kTileMaxCols) {
         c1 = B.goff[gb + 1];
         ++gb;
       }
@@ -1085,12 +1089,14 @@ static unsigned persistent_grid(int ntiles, int blocks_per_cu) {
 template <class K>
 static void allow_lds(K kernel, size_t bytes) {
   static std::mutex mu;
-  static size_t allowed = 64 * 1024;
+  static std::map<const void*, size_t> allowed;  // per kernel
+  const void* key = reinterpret_cast<const void*>(kernel);
   std::lock_guard<std::mutex> lock(mu);
-  if (bytes > allowed) {
-    SQ_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)bytes));
-    allowed = bytes;
+  size_t& a = allowed[key];
+  if (a == 0) a = 64 * 1024;
+  if (bytes > a) {
+    SQ_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    a = bytes;
   }
 }
 
@@ -1105,7 +1111,7 @@ static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st)
     k_stage_fwd<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT,
                             lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), st>>>(S.large, dagger ? 1 : 0, psi);
   if (S.nsmall > 0)
-    k_stage_fwd<64><<<persistent_grid(S.nsmall, 8), 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), st>>>(
+    k_stage_fwd<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), st>>>(
         S.small, dagger ? 1 : 0, psi);
 }
 
@@ -1116,9 +1122,11 @@ static void launch_rev(const Stage& S, double* psi, double* lam, double* partial
     k_stage_rev<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT, lds, st>>>(S.large, psi, lam, partial,
                                                                                           wb);
   }
-  if (S.nsmall > 0)
-    k_stage_rev<64><<<persistent_grid(S.nsmall, 8), 64, lds_bytes(S.small.emax, S.small.wmax, S.npairs, true), st>>>(
-        S.small, psi, lam, partial, wb);
+  if (S.nsmall > 0) {
+    const size_t lds = lds_bytes(S.small.emax, S.small.wmax, S.npairs, true);
+    allow_lds(k_stage_rev<kSmallNT>, lds);
+    k_stage_rev<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds, st>>>(S.small, psi, lam, partial, wb);
+  }
 }
 
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
