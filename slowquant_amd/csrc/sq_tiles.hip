@@ -91,12 +91,27 @@ constexpr int kMatVals = 36;
 // descriptor: kind (2 bits: 0 pair empty, 1 pair doubly occupied, 2 singly
 // occupied with hypercube bit 0, 3 singly occupied with hypercube bit 1) and
 // the hypercube bit index (3 bits).
-// One tile: alpha rows arows[r0, r0+R), beta stage positions [c0, c1), and
-// the alpha group's pair descriptor.
+// One tile: alpha rows arows[r0, r0+R), beta stage positions [c0, c1), the
+// alpha group's pair descriptor and the tile's column run.
 struct TileRec {
   int32_t r0, R, c0, c1;
   uint32_t adesc;
-  int32_t pad[3];
+  int32_t run;
+  int32_t pad[2];
+};
+
+// Orbit column lists of one run of beta stage positions [c0, c1), shared by
+// every tile over that run.  Per pair j the run's table holds W = c1 - c0
+// 16-bit entries at ctab[toff + j*W]: first the columns whose beta pair j is
+// singly occupied with hypercube bit 0 (cnt[0][j] of them: 4-member orbits,
+// or 2-member beta orbits when the alpha pair is not single), then the
+// columns with beta pair j empty (cnt[1][j]), then doubly occupied
+// (cnt[2][j]).  Entry = column within the run (10 bits) | hypercube bit of
+// the pair << 10.  Columns holding the upper member of a single beta pair are
+// reached from their lower member and listed nowhere.
+struct ColRun {
+  int32_t c0, c1, toff, pad;
+  uint16_t cnt[3][8];
 };
 
 struct StageDev {
@@ -110,9 +125,10 @@ struct StageDev {
   int64_t poff;              // this stage's block of the gradient partials
   const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
-  const uint32_t* bdesc;     // per beta stage position: descriptor
   const int32_t* bcol;       // per beta stage position: canonical beta column
   const struct TileRec* tiles;
+  const ColRun* runs;        // column runs of the stage's tiles
+  const uint16_t* ctab;      // their orbit column tables (runs start 16-byte aligned)
 };
 
 // Per tiled stage, what the matrix and gradient kernels need.
@@ -408,18 +424,53 @@ static void finalize_stage(const Space& sp, Stage& S) {
     adesc[ga] = d;
   }
   for (int k = 0; k < sp.nsb; ++k) bdesc[k] = pair_desc(B.occ[k], B.bit[k], S.npairs);
+  // column runs (distinct [c0, c1)) and their orbit column tables
+  std::map<std::pair<int, int>, int> run_of;
+  std::vector<ColRun> runs;
+  std::vector<uint16_t> ctab;
+  auto run_index = [&](int c0, int c1) {
+    auto it = run_of.find({c0, c1});
+    if (it != run_of.end()) return it->second;
+    ColRun r{};
+    r.c0 = c0;
+    r.c1 = c1;
+    r.toff = (int)ctab.size();
+    const int W = c1 - c0;
+    for (int j = 0; j < S.npairs; ++j) {
+      std::vector<uint16_t> lists[3];
+      for (int k = c0; k < c1; ++k) {
+        const uint32_t f = (bdesc[k] >> (5 * j)) & 31u;
+        const uint32_t kind = f & 3u;
+        const uint16_t e = (uint16_t)((uint32_t)(k - c0) | ((f >> 2) << 10));
+        if (kind == 2u) lists[0].push_back(e);
+        else if (kind == 0u) lists[1].push_back(e);
+        else if (kind == 1u) lists[2].push_back(e);
+      }
+      for (int t = 0; t < 3; ++t) {
+        r.cnt[t][j] = (uint16_t)lists[t].size();
+        ctab.insert(ctab.end(), lists[t].begin(), lists[t].end());
+      }
+      ctab.resize((size_t)r.toff + (size_t)(j + 1) * W, 0);
+    }
+    ctab.resize((ctab.size() + 7) / 8 * 8, 0);  // next run 16-byte aligned
+    runs.push_back(r);
+    run_of[{c0, c1}] = (int)runs.size() - 1;
+    return (int)runs.size() - 1;
+  };
   std::vector<TileRec> recs(tiles.size());
   for (size_t t = 0; t < tiles.size(); ++t) {
     const int ga = tiles[t].x;
-    recs[t] = TileRec{A.goff[ga], A.goff[ga + 1] - A.goff[ga], tiles[t].y, tiles[t].z, adesc[ga], {0, 0, 0}};
+    recs[t] = TileRec{A.goff[ga], A.goff[ga + 1] - A.goff[ga], tiles[t].y, tiles[t].z, adesc[ga],
+                      run_index(tiles[t].y, tiles[t].z), {0, 0}};
   }
   StageDev d{};
   d.npairs = S.npairs;
   d.nsb = sp.nsb;
   d.arows = upload(S.allocs, A.order);
-  d.bdesc = upload(S.allocs, bdesc);
   d.bcol = upload(S.allocs, B.order);
   d.tiles = upload(S.allocs, recs);
+  d.runs = upload(S.allocs, runs);
+  d.ctab = upload(S.allocs, ctab);
   S.d_entries = upload(S.allocs, S.entries);
   const char* xenv = getenv("SQ_XCD_REMAP");
   d.xcd = (xenv && xenv[0] == '0') ? 0 : 1;
@@ -769,28 +820,33 @@ struct TileView {
   int R, W, E, c0;
   float invW;
   uint32_t adesc;
+  const ColRun* run;
 };
 
 struct TileLds {
   double* X;
   double* L;
-  double* mats;
-  uint32_t* cdesc;
+  uint16_t* tab;  // the run's orbit column table, [npairs][W]
   uint32_t* ccol;
   int32_t* rows;
 };
 
+__host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+__host__ __device__ constexpr size_t tab_entries(int npairs, int wmax) { return ((size_t)npairs * wmax + 7) / 8 * 8; }
+
 static size_t lds_bytes(int emax, int wmax, int npairs, bool rev) {
-  return (size_t)emax * 8 * (rev ? 2 : 1) + (size_t)npairs * kMatVals * 8 + (size_t)wmax * 8 + 64 * 4;
+  return align16((size_t)emax * 8 * (rev ? 2 : 1)) + align16(tab_entries(npairs, wmax) * 2) + (size_t)wmax * 4 + 64 * 4;
 }
 
 __device__ __forceinline__ TileLds carve_lds(double* smem, const StageDev& S, bool rev) {
   TileLds m;
+  char* p = reinterpret_cast<char*>(smem);
   m.X = smem;
   m.L = rev ? m.X + S.emax : nullptr;
-  m.mats = m.X + (rev ? 2 : 1) * S.emax;
-  m.cdesc = reinterpret_cast<uint32_t*>(m.mats + S.npairs * kMatVals);
-  m.ccol = m.cdesc + S.wmax;
+  p += align16((size_t)S.emax * 8 * (rev ? 2 : 1));
+  m.tab = reinterpret_cast<uint16_t*>(p);
+  p += align16(tab_entries(S.npairs, S.wmax) * 2);
+  m.ccol = reinterpret_cast<uint32_t*>(p);
   m.rows = reinterpret_cast<int32_t*>(m.ccol + S.wmax);
   return m;
 }
@@ -803,14 +859,8 @@ __device__ __forceinline__ int xcd_tile(int b, int n, int on) {
   return x * per + min(x, rem) + q;
 }
 
-// Tile descriptors into LDS: alpha rows, the columns' pair descriptors and
-// canonical indices.  The stage's pair matrices are loaded once per block
-// (load_mats).
-template <int NT>
-__device__ __forceinline__ void load_mats(const StageDev& S, const TileLds& m) {
-  for (int i = threadIdx.x; i < S.npairs * kMatVals; i += NT) m.mats[i] = S.mats[i];
-}
-
+// Tile descriptors into LDS: alpha rows, the columns' canonical indices and
+// the run's orbit column table (16-byte copies).
 template <int NT>
 __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile) {
   const TileRec T = S.tiles[S.tile0 + tile];
@@ -821,12 +871,13 @@ __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds&
   t.c0 = T.c0;
   t.invW = 1.0f / (float)t.W;
   t.adesc = T.adesc;
+  t.run = S.runs + T.run;
   for (int r = threadIdx.x; r < t.R; r += NT) m.rows[r] = S.arows[T.r0 + r];
-#pragma unroll 4
-  for (int c = threadIdx.x; c < t.W; c += NT) {
-    m.cdesc[c] = S.bdesc[T.c0 + c];
-    m.ccol[c] = (uint32_t)S.bcol[T.c0 + c];
-  }
+  for (int c = threadIdx.x; c < t.W; c += NT) m.ccol[c] = (uint32_t)S.bcol[T.c0 + c];
+  const uint4* src = reinterpret_cast<const uint4*>(S.ctab + t.run->toff);
+  uint4* dst = reinterpret_cast<uint4*>(m.tab);
+  const int n16 = (int)(tab_entries(S.npairs, t.W) / 8);
+  for (int i = threadIdx.x; i < n16; i += NT) dst[i] = src[i];
   __syncthreads();
   return t;
 }
@@ -864,72 +915,99 @@ __device__ __forceinline__ void tile_store(const StageDev& S, const TileView& t,
   }
 }
 
-// Orbit of pair j owned by work item i: element offsets of members 0..3 in
-// the tile and the class of its type in the compact layout (-1: item i is
-// not an orbit representative; -2: a one-member orbit no gate acts on, m0 set).
-struct Orbit {
-  int m0, m1, m2, m3;
-  int cls;
-};
-
-__device__ __forceinline__ void orbit_of(const TileView& t, const uint32_t* cdesc, int j, int i, Orbit& ob) {
-  ob.cls = -1;
-  const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
-  const bool asingle = (fa & 3u) >= 2u;
-  const int abit = (int)(fa >> 2);
-  const int rr = (int)(((float)i + 0.5f) * t.invW);
-  const int c = i - rr * t.W;
-  const int r0 = asingle ? (((rr >> abit) << (abit + 1)) | (rr & ((1 << abit) - 1))) : rr;
-  const uint32_t fb = (cdesc[c] >> (5 * j)) & 31u;
-  const uint32_t kb = fb & 3u;
-  if (kb == 3u) return;  // upper member of a beta pair
-  const bool bsingle = kb == 2u;
-  const int W = t.W;
-  ob.m0 = r0 * W + c;
-  if (!asingle && !bsingle) {  // no gate of the pair acts
-    ob.cls = -2;
-    return;
-  }
-  const int dr = asingle ? (W << abit) : 0;
-  const int dc = bsingle ? (1 << (fb >> 2)) : 0;
-  ob.m1 = ob.m0 + dr;
-  ob.m2 = ob.m0 + dc;
-  ob.m3 = ob.m0 + dr + dc;
-  if (asingle && bsingle) ob.cls = 0;
-  else if (asingle) ob.cls = kb == 0u ? 1 : 2;
-  else ob.cls = (fa & 3u) == 0u ? 3 : 4;
+// Work item i over (rows) x (n list entries): row rr and list index k (inv = 1/n).
+__device__ __forceinline__ void split_item(int i, int n, float inv, int& rr, int& k) {
+  rr = (int)(((float)i + 0.5f) * inv);
+  k = i - rr * n;
 }
 
-constexpr int kBatch = 4;  // orbits per thread in flight
+// Row of the lower alpha member of hypercube row rr: a zero bit inserted at abit.
+__device__ __forceinline__ int ins0(int rr, int abit) {
+  return ((rr >> abit) << (abit + 1)) | (rr & ((1 << abit) - 1));
+}
 
-// v = M v (or M^T v) for one orbit, in place in X.
-__device__ __forceinline__ void orbit_apply(double* X, const Orbit& ob, const double* Mp, bool tr) {
-  if (ob.cls == 0) {
-    const double v0 = X[ob.m0], v1 = X[ob.m1], v2 = X[ob.m2], v3 = X[ob.m3];
-    double o[4];
+// o = M v (TR: M^T v), M row-major 4x4.
+template <bool TR>
+__device__ __forceinline__ void mat4(const double* __restrict__ M, const double (&v)[4], double (&o)[4]) {
 #pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const double* r = tr ? Mp + a : Mp + 4 * a;
-      const int st = tr ? 4 : 1;
-      o[a] = r[0] * v0 + r[st] * v1 + r[2 * st] * v2 + r[3 * st] * v3;
-    }
-    X[ob.m0] = o[0];
-    X[ob.m1] = o[1];
-    X[ob.m2] = o[2];
-    X[ob.m3] = o[3];
-  } else {
-    const int m1 = ob.cls <= 2 ? ob.m1 : ob.m2;
-    const double* B = Mp + c_cls_off[ob.cls];
-    const double v0 = X[ob.m0], v1 = X[m1];
-    const double b01 = tr ? B[2] : B[1], b10 = tr ? B[1] : B[2];
-    X[ob.m0] = B[0] * v0 + b01 * v1;
-    X[m1] = b10 * v0 + B[3] * v1;
+  for (int a = 0; a < 4; ++a) {
+    double acc = (TR ? M[a] : M[4 * a]) * v[0];
+#pragma unroll
+    for (int b = 1; b < 4; ++b) acc += (TR ? M[4 * b + a] : M[4 * a + b]) * v[b];
+    o[a] = acc;
   }
 }
 
-// Sum of a double over the 64 lanes of a wave with DPP moves (quad_perm
-// swaps, row_shr 4/8, row_bcast 15/31; no LDS traffic); the total lands in
-// lane 63.  Every lane of the wave must be active.
+// One pair pass of a forward stage on the tile in LDS.  Orbits come from the
+// run's column lists, so every work item is one orbit and a wave runs one
+// orbit type except where two lists meet.  Orbit members: 0 = (alpha lower,
+// beta lower) at m0, 1 = alpha upper (+ dr), 2 = beta upper (+ dc), 3 = both.
+template <int NT, bool TR>
+__device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, const double* __restrict__ Mp, int j) {
+  const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
+  const int n4 = t.run->cnt[0][j], n2e = t.run->cnt[1][j], n2d = t.run->cnt[2][j];
+  const uint16_t* tab = m.tab + j * t.W;
+  double* X = m.X;
+  const int W = t.W;
+  if ((fa & 3u) >= 2u) {  // alpha pair single: 4-member orbits, then 2-member alpha orbits
+    const int abit = (int)(fa >> 2);
+    const int Rh = t.R >> 1, dr = W << abit, n2 = n2e + n2d;
+    const int nA = Rh * n4, nT = nA + Rh * n2;
+    const float iA = 1.0f / (float)max(n4, 1), iB = 1.0f / (float)max(n2, 1);
+    for (int i = threadIdx.x; i < nT; i += NT) {
+      if (i < nA) {
+        int rr, k;
+        split_item(i, n4, iA, rr, k);
+        const uint32_t e = tab[k];
+        const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
+        const int m2 = m0 + (1 << (e >> 10));
+        const double v[4] = {X[m0], X[m0 + dr], X[m2], X[m2 + dr]};
+        double o[4];
+        mat4<TR>(Mp, v, o);
+        X[m0] = o[0];
+        X[m0 + dr] = o[1];
+        X[m2] = o[2];
+        X[m2 + dr] = o[3];
+      } else {
+        int rr, k;
+        split_item(i - nA, n2, iB, rr, k);
+        const uint32_t e = tab[n4 + k];
+        const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
+        const bool dbl = k >= n2e;  // beta pair doubly occupied
+        const double b0 = dbl ? Mp[20] : Mp[16], b1 = dbl ? Mp[21] : Mp[17];
+        const double b2 = dbl ? Mp[22] : Mp[18], b3 = dbl ? Mp[23] : Mp[19];
+        const double v0 = X[m0], v1 = X[m0 + dr];
+        X[m0] = b0 * v0 + (TR ? b2 : b1) * v1;
+        X[m0 + dr] = (TR ? b1 : b2) * v0 + b3 * v1;
+      }
+    }
+  } else {  // alpha pair empty or double: 2-member beta orbits; other amplitudes stay
+    const double* B = Mp + ((fa & 3u) == 0u ? 24 : 28);
+    const int nA = t.R * n4;
+    const float iA = 1.0f / (float)max(n4, 1);
+    for (int i = threadIdx.x; i < nA; i += NT) {
+      int rr, k;
+      split_item(i, n4, iA, rr, k);
+      const uint32_t e = tab[k];
+      const int m0 = rr * W + (int)(e & 1023u);
+      const int m2 = m0 + (1 << (e >> 10));
+      const double v0 = X[m0], v1 = X[m2];
+      X[m0] = B[0] * v0 + (TR ? B[2] : B[1]) * v1;
+      X[m2] = (TR ? B[1] : B[2]) * v0 + B[3] * v1;
+    }
+    if (Mp[kPairVals] == 0.0) {  // derivative pair: T_g annihilates the amplitudes no gate moves
+      const int n2 = n2e + n2d, nB = t.R * n2;
+      const float iB = 1.0f / (float)max(n2, 1);
+      for (int i = threadIdx.x; i < nB; i += NT) {
+        int rr, k;
+        split_item(i, n2, iB, rr, k);
+        X[rr * W + (int)(tab[n4 + k] & 1023u)] = 0.0;
+      }
+    }
+  }
+}
+
+// DPP lane moves of a double (bound_ctrl: lanes without a source read 0).
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_move(double v) {
   const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROWS, 0xf, true);
@@ -937,130 +1015,196 @@ __device__ __forceinline__ double dpp_move(double v) {
   return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double wave_sum_lane63(double v) {
-  v += dpp_move<0xb1, 0xf>(v);   // quad_perm [1,0,3,2]
-  v += dpp_move<0x4e, 0xf>(v);   // quad_perm [2,3,0,1]: quad sums
-  v += dpp_move<0x114, 0xf>(v);  // row_shr:4
-  v += dpp_move<0x118, 0xf>(v);  // row_shr:8: row sums in lanes 15 of each row
-  v += dpp_move<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
-  v += dpp_move<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+// Row sums (16 lanes) of 8 per-lane values with halving exchanges inside
+// each quad (xor 1, xor 2) and row shifts by 4 and 8: lane l with
+// l % 16 >= 12 ends with the row sums of values
+// q = (l & 1) * 4 + ((l >> 1) & 1) * 2 + {0, 1} in w[0], w[1].
+// Every lane of the wave must be active.
+__device__ __forceinline__ void row_fold8(const double (&Q)[8], double (&w)[2], int lane) {
+  const bool o1 = lane & 1, o2 = lane & 2;
+  double w1[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const double keep = o1 ? Q[4 + i] : Q[i], send = o1 ? Q[i] : Q[4 + i];
+    w1[i] = keep + dpp_move<0xb1, 0xf>(send);  // quad_perm [1,0,3,2]
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const double keep = o2 ? w1[2 + i] : w1[i], send = o2 ? w1[i] : w1[2 + i];
+    w[i] = keep + dpp_move<0x4e, 0xf>(send);  // quad_perm [2,3,0,1]
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    w[i] += dpp_move<0x114, 0xf>(w[i]);  // row_shr:4
+    w[i] += dpp_move<0x118, 0xf>(w[i]);  // row_shr:8
+  }
+}
+
+// Row sum of one value; lane l with l % 16 == 15 holds it.
+__device__ __forceinline__ double row_sum(double v) {
+  v += dpp_move<0xb1, 0xf>(v);
+  v += dpp_move<0x4e, 0xf>(v);
+  v += dpp_move<0x114, 0xf>(v);
+  v += dpp_move<0x118, 0xf>(v);
   return v;
 }
 
 // Gates of one stage on one vector (canonical order, in place).
 // dagger: U^+ (transposed pair matrices).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, int dagger, double* __restrict__ psi) {
+__global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ mats, int dagger,
+                                                  double* __restrict__ psi) {
   extern __shared__ double smem[];
   const TileLds m = carve_lds(smem, S, false);
-  load_mats<NT>(S, m);
   // persistent blocks: block b walks tiles b, b + grid, ... (grid is a
   // multiple of 8, so every tile of a block maps into its XCD's range)
   for (int ti = blockIdx.x; ti < S.nt; ti += gridDim.x) {
-  const TileView t = tile_setup<NT>(S, m, xcd_tile(ti, S.nt, S.xcd));
-  tile_load<NT>(S, t, m, psi, m.X);
-  __syncthreads();
-  for (int j = 0; j < S.npairs; ++j) {
-    const bool asingle = ((t.adesc >> (5 * j)) & 3u) >= 2u;
-    const int items = asingle ? t.E >> 1 : t.E;
-    const double* Mp = m.mats + j * kMatVals;
-    const bool zero_rest = Mp[kPairVals] == 0.0;  // derivative pair: T_g annihilates the other types
-    for (int i0 = threadIdx.x; i0 < items; i0 += NT * kBatch) {
-      Orbit ob[kBatch];
-#pragma unroll
-      for (int b = 0; b < kBatch; ++b) {
-        const int i = i0 + b * NT;
-        ob[b].cls = -1;
-        if (i < items) orbit_of(t, m.cdesc, j, i, ob[b]);
-      }
-#pragma unroll
-      for (int b = 0; b < kBatch; ++b) {
-        if (ob[b].cls >= 0) orbit_apply(m.X, ob[b], Mp, dagger != 0);
-        else if (ob[b].cls == -2 && zero_rest) m.X[ob[b].m0] = 0.0;
-      }
-    }
+    const TileView t = tile_setup<NT>(S, m, xcd_tile(ti, S.nt, S.xcd));
+    tile_load<NT>(S, t, m, psi, m.X);
     __syncthreads();
-  }
-  tile_store<NT>(S, t, m, psi, m.X);
-  __syncthreads();  // the next tile overwrites the LDS tables
+    for (int j = 0; j < S.npairs; ++j) {
+      if (dagger) fwd_pair<NT, true>(t, m, mats + j * kMatVals, j);
+      else fwd_pair<NT, false>(t, m, mats + j * kMatVals, j);
+      __syncthreads();
+    }
+    tile_store<NT>(S, t, m, psi, m.X);
+    __syncthreads();  // the next tile overwrites the LDS tables
   }
 }
 
 // Reverse stage on (psi, lam), in place: per pair, the antisymmetric orbit
 // products Q[a<b] += lam_a psi_b - lam_b psi_a before un-applying the pair
 // matrix (M^T) on both; per-tile sums go to partial[poff + (j*10 + v)*ntiles + tile].
+// Per pair the row sums land in red[j][.][row] (8 values when the alpha pair
+// is single, else one: the beta orbit type of the alpha occupation).
 template <int NT>
-__global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, double* __restrict__ psi, double* __restrict__ lam,
+__global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __restrict__ mats,
+                                                  double* __restrict__ psi, double* __restrict__ lam,
                                                   double* __restrict__ partial, int write_back) {
+  constexpr int kRows = NT / 16;
   extern __shared__ double smem[];
-  __shared__ double red[kQVals][NT / 64];
+  __shared__ double red[kMaxPairs][8][kRows];
   const TileLds m = carve_lds(smem, S, true);
-  load_mats<NT>(S, m);
+  const int lane = threadIdx.x & 63, row = threadIdx.x >> 4;
+  const bool wb = write_back != 0;
   for (int ti = blockIdx.x; ti < S.nt; ti += gridDim.x) {
-  const int tile = xcd_tile(ti, S.nt, S.xcd);
-  const TileView t = tile_setup<NT>(S, m, tile);
-  tile_load<NT>(S, t, m, psi, m.X);
-  tile_load<NT>(S, t, m, lam, m.L);
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int ntiles_stage = S.ntiles_stage;
-  const int my_tile = S.tile0 + tile;
-  for (int j = 0; j < S.npairs; ++j) {
-    const bool asingle = ((t.adesc >> (5 * j)) & 3u) >= 2u;
-    const int items = asingle ? t.E >> 1 : t.E;
-    const double* Mp = m.mats + j * kMatVals;
-    double Q[kQVals];
-#pragma unroll
-    for (int q = 0; q < kQVals; ++q) Q[q] = 0.0;
-    for (int i = threadIdx.x; i < items; i += NT) {
-      Orbit ob;
-      orbit_of(t, m.cdesc, j, i, ob);
-      if (ob.cls < 0) continue;
-      if (ob.cls == 0) {
-        const double v0 = m.X[ob.m0], v1 = m.X[ob.m1], v2 = m.X[ob.m2], v3 = m.X[ob.m3];
-        const double w0 = m.L[ob.m0], w1 = m.L[ob.m1], w2 = m.L[ob.m2], w3 = m.L[ob.m3];
-        Q[0] += w0 * v1 - w1 * v0;
-        Q[1] += w0 * v2 - w2 * v0;
-        Q[2] += w0 * v3 - w3 * v0;
-        Q[3] += w1 * v2 - w2 * v1;
-        Q[4] += w1 * v3 - w3 * v1;
-        Q[5] += w2 * v3 - w3 * v2;
-      } else {
-        const int m1 = ob.cls <= 2 ? ob.m1 : ob.m2;
-        const double x = m.L[ob.m0] * m.X[m1] - m.L[m1] * m.X[ob.m0];
-#pragma unroll
-        for (int cls = 1; cls < 5; ++cls)
-          if (ob.cls == cls) Q[5 + cls] += x;
-      }
-      if (write_back) {
-        orbit_apply(m.X, ob, Mp, true);
-        orbit_apply(m.L, ob, Mp, true);
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < kQVals; ++q) {
-      const double d = wave_sum_lane63(Q[q]);
-      if (NT == 64) {
-        if (lane == 63) partial[S.poff + ((size_t)j * kQVals + q) * ntiles_stage + my_tile] = d;
-      } else if (lane == 63) {
-        red[q][wave] = d;
-      }
-    }
-    if (NT > 64) {
-      __syncthreads();
-      if (threadIdx.x < kQVals) {
-        double v = 0.0;
-        for (int w = 0; w < NT / 64; ++w) v += red[threadIdx.x][w];
-        partial[S.poff + ((size_t)j * kQVals + threadIdx.x) * ntiles_stage + my_tile] = v;
-      }
-    }
+    const int tile = xcd_tile(ti, S.nt, S.xcd);
+    const TileView t = tile_setup<NT>(S, m, tile);
+    tile_load<NT>(S, t, m, psi, m.X);
+    tile_load<NT>(S, t, m, lam, m.L);
     __syncthreads();
-  }
-  if (write_back) {  // not for the first stage of the program: the un-applied states are not needed
-    tile_store<NT>(S, t, m, psi, m.X);
-    tile_store<NT>(S, t, m, lam, m.L);
-  }
-  __syncthreads();  // the next tile overwrites the LDS tables
+    const int W = t.W;
+    double* X = m.X;
+    double* L = m.L;
+    for (int j = 0; j < S.npairs; ++j) {
+      const double* __restrict__ Mp = mats + j * kMatVals;
+      const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
+      const int n4 = t.run->cnt[0][j], n2e = t.run->cnt[1][j], n2d = t.run->cnt[2][j];
+      const uint16_t* tab = m.tab + j * W;
+      double Q[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) Q[q] = 0.0;
+      if ((fa & 3u) >= 2u) {
+        const int abit = (int)(fa >> 2);
+        const int Rh = t.R >> 1, dr = W << abit, n2 = n2e + n2d;
+        const int nA = Rh * n4, nT = nA + Rh * n2;
+        const float iA = 1.0f / (float)max(n4, 1), iB = 1.0f / (float)max(n2, 1);
+        for (int i = threadIdx.x; i < nT; i += NT) {
+          if (i < nA) {
+            int rr, k;
+            split_item(i, n4, iA, rr, k);
+            const uint32_t e = tab[k];
+            const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
+            const int m2 = m0 + (1 << (e >> 10));
+            const double v[4] = {X[m0], X[m0 + dr], X[m2], X[m2 + dr]};
+            const double w[4] = {L[m0], L[m0 + dr], L[m2], L[m2 + dr]};
+            Q[0] += w[0] * v[1] - w[1] * v[0];
+            Q[1] += w[0] * v[2] - w[2] * v[0];
+            Q[2] += w[0] * v[3] - w[3] * v[0];
+            Q[3] += w[1] * v[2] - w[2] * v[1];
+            Q[4] += w[1] * v[3] - w[3] * v[1];
+            Q[5] += w[2] * v[3] - w[3] * v[2];
+            if (wb) {
+              double ov[4], ow[4];
+              mat4<true>(Mp, v, ov);
+              mat4<true>(Mp, w, ow);
+              X[m0] = ov[0];
+              X[m0 + dr] = ov[1];
+              X[m2] = ov[2];
+              X[m2 + dr] = ov[3];
+              L[m0] = ow[0];
+              L[m0 + dr] = ow[1];
+              L[m2] = ow[2];
+              L[m2 + dr] = ow[3];
+            }
+          } else {
+            int rr, k;
+            split_item(i - nA, n2, iB, rr, k);
+            const uint32_t e = tab[n4 + k];
+            const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
+            const bool dbl = k >= n2e;
+            const double v0 = X[m0], v1 = X[m0 + dr], w0 = L[m0], w1 = L[m0 + dr];
+            const double x = w0 * v1 - w1 * v0;
+            Q[6] += dbl ? 0.0 : x;
+            Q[7] += dbl ? x : 0.0;
+            if (wb) {
+              const double b0 = dbl ? Mp[20] : Mp[16], b1 = dbl ? Mp[21] : Mp[17];
+              const double b2 = dbl ? Mp[22] : Mp[18], b3 = dbl ? Mp[23] : Mp[19];
+              X[m0] = b0 * v0 + b2 * v1;
+              X[m0 + dr] = b1 * v0 + b3 * v1;
+              L[m0] = b0 * w0 + b2 * w1;
+              L[m0 + dr] = b1 * w0 + b3 * w1;
+            }
+          }
+        }
+        double w2[2];
+        row_fold8(Q, w2, lane);
+        if ((lane & 15) >= 12) {
+          const int q = (lane & 1) * 4 + ((lane >> 1) & 1) * 2;
+          red[j][q][row] = w2[0];
+          red[j][q + 1][row] = w2[1];
+        }
+      } else {
+        const double* B = Mp + ((fa & 3u) == 0u ? 24 : 28);
+        const int nA = t.R * n4;
+        const float iA = 1.0f / (float)max(n4, 1);
+        for (int i = threadIdx.x; i < nA; i += NT) {
+          int rr, k;
+          split_item(i, n4, iA, rr, k);
+          const uint32_t e = tab[k];
+          const int m0 = rr * W + (int)(e & 1023u);
+          const int m2 = m0 + (1 << (e >> 10));
+          const double v0 = X[m0], v1 = X[m2], w0 = L[m0], w1 = L[m2];
+          Q[0] += w0 * v1 - w1 * v0;
+          if (wb) {
+            X[m0] = B[0] * v0 + B[2] * v1;
+            X[m2] = B[1] * v0 + B[3] * v1;
+            L[m0] = B[0] * w0 + B[2] * w1;
+            L[m2] = B[1] * w0 + B[3] * w1;
+          }
+        }
+        const double s = row_sum(Q[0]);
+        if ((lane & 15) == 15) red[j][0][row] = s;
+      }
+      __syncthreads();
+    }
+    const int my_tile = S.tile0 + tile;
+    for (int x = threadIdx.x; x < S.npairs * kQVals; x += NT) {
+      const int j = x / kQVals, q = x - j * kQVals;
+      const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
+      int src = -1;
+      if ((fa & 3u) >= 2u) src = q < 8 ? q : -1;
+      else src = q == ((fa & 3u) == 0u ? 8 : 9) ? 0 : -1;
+      double v = 0.0;
+      if (src >= 0)
+        for (int r = 0; r < kRows; ++r) v += red[j][src][r];
+      partial[S.poff + (size_t)x * S.ntiles_stage + my_tile] = v;
+    }
+    if (wb) {  // not for the first stage of the program: the un-applied states are not needed
+      tile_store<NT>(S, t, m, psi, m.X);
+      tile_store<NT>(S, t, m, lam, m.L);
+    }
+    __syncthreads();  // the next tile overwrites the LDS tables and red
   }
 }
 
@@ -1109,23 +1253,25 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
 static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st) {
   if (S.ntiles > S.nsmall)
     k_stage_fwd<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT,
-                            lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), st>>>(S.large, dagger ? 1 : 0, psi);
+                            lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), st>>>(S.large, S.large.mats,
+                                                                                          dagger ? 1 : 0, psi);
   if (S.nsmall > 0)
     k_stage_fwd<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), st>>>(
-        S.small, dagger ? 1 : 0, psi);
+        S.small, S.small.mats, dagger ? 1 : 0, psi);
 }
 
 static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
   if (S.ntiles > S.nsmall) {
     const size_t lds = lds_bytes(S.large.emax, S.large.wmax, S.npairs, true);
     allow_lds(k_stage_rev<kLargeNT>, lds);
-    k_stage_rev<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT, lds, st>>>(S.large, psi, lam, partial,
-                                                                                          wb);
+    k_stage_rev<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT, lds, st>>>(S.large, S.large.mats, psi,
+                                                                                          lam, partial, wb);
   }
   if (S.nsmall > 0) {
     const size_t lds = lds_bytes(S.small.emax, S.small.wmax, S.npairs, true);
     allow_lds(k_stage_rev<kSmallNT>, lds);
-    k_stage_rev<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds, st>>>(S.small, psi, lam, partial, wb);
+    k_stage_rev<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds, st>>>(S.small, S.small.mats, psi, lam, partial,
+                                                                       wb);
   }
 }
 
