@@ -44,11 +44,10 @@
 namespace sq {
 
 constexpr int kMaxPairs = 6;        // hypercubes of at most 64 strings per spin
-constexpr int kTileMax = 4096;      // doubles per vector per tile (large tiles)
+constexpr int kTileMax = 4096;      // doubles per vector per tile
 constexpr int kTileMaxCols = 1024;  // columns per tile
 constexpr int kTB = 256;            // threads per block of the reductions
-constexpr int kLargeNT = 1024;      // threads per block of the large-tile kernels
-constexpr int kSmallNT = 512;       // threads per block of the target-size tile kernels
+constexpr int kNT = 512;            // threads per block of the tile kernels
 constexpr int kMaxStageGates = 48;  // stage entries (sa_single counts twice)
 constexpr int kMaxPairEntries = 8;  // entries per pair within a stage
 constexpr int kTypes = 9;           // orbit types (alpha kind x beta kind)
@@ -117,11 +116,7 @@ struct ColRun {
 struct StageDev {
   int npairs;
   int nsb;
-  int tile0;                 // first tile of this launch (small tiles, then large ones)
-  int xcd;                   // 1: XCD-contiguous tile ranges (SQ_XCD_REMAP=0 turns it off)
-  int emax, wmax;            // largest tile of this launch: elements, columns
-  int ntiles_stage;          // tiles of the whole stage (small and large)
-  int nt;                    // tiles of this launch
+  int nt;                    // tiles (largest first)
   int64_t poff;              // this stage's block of the gradient partials
   const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
@@ -130,6 +125,15 @@ struct StageDev {
   const ColRun* runs;        // column runs of the stage's tiles
   const uint16_t* ctab;      // their orbit column tables (runs start 16-byte aligned)
 };
+
+// Dynamic LDS of one tile, carved in this order (16-byte aligned parts):
+// X[E] (| L[E] in the reverse sweep) | orbit column table [npairs][W]
+// (16-bit) | canonical columns [W] | alpha rows [R].
+__host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+__host__ __device__ constexpr size_t tab_entries(int npairs, int W) { return ((size_t)npairs * W + 7) / 8 * 8; }
+__host__ __device__ constexpr size_t tile_lds(int E, int W, int R, int npairs, int nvec) {
+  return align16((size_t)E * 8 * nvec) + align16(tab_entries(npairs, W) * 2) + align16((size_t)W * 4) + (size_t)R * 4;
+}
 
 // Per tiled stage, what the matrix and gradient kernels need.
 struct StageInfo {
@@ -146,9 +150,10 @@ struct Stage {
   int pp[kMaxPairs];
   std::vector<StageEntry> entries;
   int pe[kMaxPairs + 1];
-  int ntiles = 0, nsmall = 0;  // tiles [0, nsmall) run one wave each, the rest 256 threads
+  int ntiles = 0;
   int index = -1;              // position among the tiled stages
-  StageDev small{}, large{};   // launch descriptors
+  StageDev dev{};              // launch descriptor
+  size_t lds_fwd = 0, lds_rev = 0;  // dynamic LDS of the forward / reverse launches (largest tile)
   const StageEntry* d_entries = nullptr;
   std::vector<void*> allocs;
 };
@@ -347,14 +352,14 @@ static T* upload(std::vector<void*>& allocs, const std::vector<T>& v) {
   return d;
 }
 
-// Elements per one-wave tile (SQ_TILE_TARGET overrides); tiles above it (a
-// single group pair larger than the target) run in the 256-thread kernel.
+// Elements per tile that runs of beta groups are filled up to (SQ_TILE_TARGET
+// overrides); a single group pair above it still makes one tile.
 static int tile_target(int64_t dim) {
   const char* env = getenv("SQ_TILE_TARGET");
   if (env) return std::min(std::max(atoi(env), 16), kTileMax);
-  // about 256 tiles or more (one per CU), 256..4096 elements each
+  // about 200 tiles or more (one or two per CU), 256..4096 elements each
   int t = 256;
-  while (t < kTileMax && (int64_t)t * 2 * 256 <= dim) t *= 2;
+  while (t < kTileMax && (int64_t)t * 2 * 192 <= dim) t *= 2;
   return t;
 }
 
@@ -387,7 +392,7 @@ static void finalize_stage(const Space& sp, Stage& S) {
   // tiles: each alpha group x runs of whole beta groups, up to the target
   // size; a beta group that alone exceeds the target gets its own tile
   const int target = tile_target(sp.dim);
-  std::vector<int4> small, large;
+  std::vector<int4> tiles;
   const int nag = (int)A.goff.size() - 1;
   const int nbg = (int)B.goff.size() - 1;
   for (int ga = 0; ga < nag; ++ga) {
@@ -404,13 +409,12 @@ static void finalize_stage(const Space& sp, Stage& S) {
       }
       const int64_t E = (int64_t)R * (c1 - c0);
       SQ_CHECK(E <= kTileMax && c1 - c0 <= kTileMaxCols, SQ_ERR_ARG, "tile does not fit: hypercube groups too large");
-      (E <= target ? small : large).push_back(int4{ga, c0, c1, 0});
+      tiles.push_back(int4{ga, c0, c1, (int)E});
     }
   }
-  std::vector<int4> tiles(small);
-  tiles.insert(tiles.end(), large.begin(), large.end());
+  // largest tiles first: they start in the first dispatch round
+  std::stable_sort(tiles.begin(), tiles.end(), [](const int4& a, const int4& b) { return a.w > b.w; });
   S.ntiles = (int)tiles.size();
-  S.nsmall = (int)small.size();
   std::vector<uint32_t> adesc(nag), bdesc(sp.nsb);
   for (int ga = 0; ga < nag; ++ga) {
     // kind 2 (single, bit 0) for single pairs; the row's own bit is its hypercube index
@@ -472,28 +476,14 @@ static void finalize_stage(const Space& sp, Stage& S) {
   d.runs = upload(S.allocs, runs);
   d.ctab = upload(S.allocs, ctab);
   S.d_entries = upload(S.allocs, S.entries);
-  const char* xenv = getenv("SQ_XCD_REMAP");
-  d.xcd = (xenv && xenv[0] == '0') ? 0 : 1;
-  S.small = d;
-  S.large = d;
-  S.small.tile0 = 0;
-  S.large.tile0 = S.nsmall;
-  S.small.ntiles_stage = S.ntiles;
-  S.large.ntiles_stage = S.ntiles;
-  S.small.nt = S.nsmall;
-  S.large.nt = S.ntiles - S.nsmall;
-  auto extent = [&](int t0, int t1, StageDev& D) {
-    D.emax = 0;
-    D.wmax = 0;
-    for (int t = t0; t < t1; ++t) {
-      const int R = A.goff[tiles[t].x + 1] - A.goff[tiles[t].x];
-      const int W = tiles[t].z - tiles[t].y;
-      D.emax = std::max(D.emax, R * W);
-      D.wmax = std::max(D.wmax, W);
-    }
-  };
-  extent(0, S.nsmall, S.small);
-  extent(S.nsmall, S.ntiles, S.large);
+  d.nt = S.ntiles;
+  S.dev = d;
+  S.lds_fwd = S.lds_rev = 0;
+  for (const TileRec& r : recs) {
+    const int W = r.c1 - r.c0;
+    S.lds_fwd = std::max(S.lds_fwd, tile_lds(r.R * W, W, r.R, S.npairs, 1));
+    S.lds_rev = std::max(S.lds_rev, tile_lds(r.R * W, W, r.R, S.npairs, 2));
+  }
 }
 
 TilePlan* build_tile_plan(const Space& sp, const Program& P) {
@@ -558,8 +548,7 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
       I.poff = off;
       I.entries = S.d_entries;
       info.push_back(I);
-      S.small.poff = off;
-      S.large.poff = off;
+      S.dev.poff = off;
       off += (int64_t)S.ntiles * S.npairs * kQVals;
     }
     T->ntiled = (int)info.size();
@@ -568,8 +557,7 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
     SQ_HIP(hipMalloc(&T->d_mats, sizeof(double) * std::max<int64_t>((int64_t)T->ntiled * kMaxPairs * kMatVals, 1)));
     for (auto& S : T->stages) {
       if (!S.tiled) continue;
-      S.small.mats = T->d_mats + (size_t)S.index * kMaxPairs * kMatVals;
-      S.large.mats = S.small.mats;
+      S.dev.mats = T->d_mats + (size_t)S.index * kMaxPairs * kMatVals;
     }
     std::vector<void*> tmp;
     T->d_info = upload(tmp, info);
@@ -616,15 +604,11 @@ const TilePlan* tiles_for(const Space& s, const Program& p) {
 // k_stage_grad   per stage and pair, sum the outer products over tiles and
 //                contract them with each gate's conjugated generator
 //
-// Tile kernels are templated on the block size: NT = 64 (one wave, tiles of
-// up to the plan's target size, many resident blocks per CU) or NT = 256
-// (tiles of up to kTileMax).  LDS is dynamic, sized to the launch's largest
-// tile: X[emax] (| L[emax] in the reverse sweep) | mats[npairs*32] |
-// cdesc[wmax] | ccol[wmax] | rows[64].
-//
-// Blocks are dispatched round-robin over the 8 XCDs (block b on XCD b % 8);
-// tiles are remapped so that each XCD runs a contiguous range of the tile
-// list (consecutive tiles share alpha rows).
+// Tile kernels run kNT threads per block, one block per tile, one launch per
+// stage; tiles are sorted largest first so that they start in the first
+// dispatch round.  LDS is dynamic and carved per tile (tile_lds); the
+// launch requests the largest tile's footprint.  Pair matrices are read from
+// global memory with uniform (scalar) loads.
 
 // ---- 4x4 orbit algebra (matrix kernels) ------------------------------------
 
@@ -831,39 +815,26 @@ struct TileLds {
   int32_t* rows;
 };
 
-__host__ __device__ constexpr size_t align16(size_t b) { return (b + 15) / 16 * 16; }
-__host__ __device__ constexpr size_t tab_entries(int npairs, int wmax) { return ((size_t)npairs * wmax + 7) / 8 * 8; }
-
-static size_t lds_bytes(int emax, int wmax, int npairs, bool rev) {
-  return align16((size_t)emax * 8 * (rev ? 2 : 1)) + align16(tab_entries(npairs, wmax) * 2) + (size_t)wmax * 4 + 64 * 4;
-}
-
-__device__ __forceinline__ TileLds carve_lds(double* smem, const StageDev& S, bool rev) {
+// LDS parts of one tile (layout of tile_lds).
+__device__ __forceinline__ TileLds carve_lds(double* smem, int E, int W, int npairs, bool rev) {
   TileLds m;
   char* p = reinterpret_cast<char*>(smem);
   m.X = smem;
-  m.L = rev ? m.X + S.emax : nullptr;
-  p += align16((size_t)S.emax * 8 * (rev ? 2 : 1));
+  m.L = rev ? m.X + E : nullptr;
+  p += align16((size_t)E * 8 * (rev ? 2 : 1));
   m.tab = reinterpret_cast<uint16_t*>(p);
-  p += align16(tab_entries(S.npairs, S.wmax) * 2);
+  p += align16(tab_entries(npairs, W) * 2);
   m.ccol = reinterpret_cast<uint32_t*>(p);
-  m.rows = reinterpret_cast<int32_t*>(m.ccol + S.wmax);
+  p += align16((size_t)W * 4);
+  m.rows = reinterpret_cast<int32_t*>(p);
   return m;
-}
-
-// Tile index of this block: each XCD gets a contiguous range of n tiles.
-__device__ __forceinline__ int xcd_tile(int b, int n, int on) {
-  if (!on) return b;
-  const int x = b & 7, q = b >> 3;
-  const int per = n >> 3, rem = n & 7;
-  return x * per + min(x, rem) + q;
 }
 
 // Tile descriptors into LDS: alpha rows, the columns' canonical indices and
 // the run's orbit column table (16-byte copies).
 template <int NT>
-__device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds& m, int tile) {
-  const TileRec T = S.tiles[S.tile0 + tile];
+__device__ __forceinline__ TileView tile_setup(const StageDev& S, double* smem, bool rev, int tile, TileLds& m) {
+  const TileRec T = S.tiles[tile];
   TileView t;
   t.R = T.R;
   t.W = T.c1 - T.c0;
@@ -872,6 +843,7 @@ __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds&
   t.invW = 1.0f / (float)t.W;
   t.adesc = T.adesc;
   t.run = S.runs + T.run;
+  m = carve_lds(smem, t.E, t.W, S.npairs, rev);
   for (int r = threadIdx.x; r < t.R; r += NT) m.rows[r] = S.arows[T.r0 + r];
   for (int c = threadIdx.x; c < t.W; c += NT) m.ccol[c] = (uint32_t)S.bcol[T.c0 + c];
   const uint4* src = reinterpret_cast<const uint4*>(S.ctab + t.run->toff);
@@ -882,36 +854,43 @@ __device__ __forceinline__ TileView tile_setup(const StageDev& S, const TileLds&
   return t;
 }
 
-// Global <-> LDS copies of a tile (gather through the row and column tables),
-// eight loads in flight per thread.
-template <int NT>
+// Global <-> LDS copies of a tile for NV vectors (psi | lam), gathered
+// through the row and column tables, eight loads per vector in flight.
+template <int NT, int NV>
 __device__ __forceinline__ void tile_load(const StageDev& S, const TileView& t, const TileLds& m,
-                                          const double* __restrict__ g, double* X) {
+                                          const double* __restrict__ g0, const double* __restrict__ g1) {
   constexpr int U = 8;
   for (int e0 = threadIdx.x; e0 < t.E; e0 += NT * U) {
-    double v[U];
+    double v[NV][U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * NT;
       if (e < t.E) {
         const int r = (int)(((float)e + 0.5f) * t.invW);
-        v[u] = g[(size_t)m.rows[r] * S.nsb + m.ccol[e - r * t.W]];
+        const size_t idx = (size_t)m.rows[r] * S.nsb + m.ccol[e - r * t.W];
+        v[0][u] = g0[idx];
+        if (NV > 1) v[NV - 1][u] = g1[idx];
       }
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const int e = e0 + u * NT;
-      if (e < t.E) X[e] = v[u];
+      if (e < t.E) {
+        m.X[e] = v[0][u];
+        if (NV > 1) m.L[e] = v[NV - 1][u];
+      }
     }
   }
 }
 
-template <int NT>
-__device__ __forceinline__ void tile_store(const StageDev& S, const TileView& t, const TileLds& m, double* g,
-                                           const double* X) {
+template <int NT, int NV>
+__device__ __forceinline__ void tile_store(const StageDev& S, const TileView& t, const TileLds& m, double* g0,
+                                           double* g1) {
   for (int e = threadIdx.x; e < t.E; e += NT) {
     const int r = (int)(((float)e + 0.5f) * t.invW);
-    g[(size_t)m.rows[r] * S.nsb + m.ccol[e - r * t.W]] = X[e];
+    const size_t idx = (size_t)m.rows[r] * S.nsb + m.ccol[e - r * t.W];
+    g0[idx] = m.X[e];
+    if (NV > 1) g1[idx] = m.L[e];
   }
 }
 
@@ -1055,19 +1034,18 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ mats, int dagger,
                                                   double* __restrict__ psi) {
   extern __shared__ double smem[];
-  const TileLds m = carve_lds(smem, S, false);
-  // persistent blocks: block b walks tiles b, b + grid, ... (grid is a
-  // multiple of 8, so every tile of a block maps into its XCD's range)
-  for (int ti = blockIdx.x; ti < S.nt; ti += gridDim.x) {
-    const TileView t = tile_setup<NT>(S, m, xcd_tile(ti, S.nt, S.xcd));
-    tile_load<NT>(S, t, m, psi, m.X);
+  // block b walks tiles b, b + grid, ... (tiles are sorted largest first)
+  for (int tile = blockIdx.x; tile < S.nt; tile += gridDim.x) {
+    TileLds m;
+    const TileView t = tile_setup<NT>(S, smem, false, tile, m);
+    tile_load<NT, 1>(S, t, m, psi, nullptr);
     __syncthreads();
     for (int j = 0; j < S.npairs; ++j) {
       if (dagger) fwd_pair<NT, true>(t, m, mats + j * kMatVals, j);
       else fwd_pair<NT, false>(t, m, mats + j * kMatVals, j);
       __syncthreads();
     }
-    tile_store<NT>(S, t, m, psi, m.X);
+    tile_store<NT, 1>(S, t, m, psi, nullptr);
     __syncthreads();  // the next tile overwrites the LDS tables
   }
 }
@@ -1075,23 +1053,21 @@ __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __re
 // Reverse stage on (psi, lam), in place: per pair, the antisymmetric orbit
 // products Q[a<b] += lam_a psi_b - lam_b psi_a before un-applying the pair
 // matrix (M^T) on both; per-tile sums go to partial[poff + (j*10 + v)*ntiles + tile].
-// Per pair the row sums land in red[j][.][row] (8 values when the alpha pair
-// is single, else one: the beta orbit type of the alpha occupation).
+// Per pair the wave sums land in red[j][.][wave] (8 values when the alpha
+// pair is single, else one: the beta orbit type of the alpha occupation).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __restrict__ mats,
                                                   double* __restrict__ psi, double* __restrict__ lam,
                                                   double* __restrict__ partial, int write_back) {
-  constexpr int kRows = NT / 16;
+  constexpr int kWaves = NT / 64;
   extern __shared__ double smem[];
-  __shared__ double red[kMaxPairs][8][kRows];
-  const TileLds m = carve_lds(smem, S, true);
-  const int lane = threadIdx.x & 63, row = threadIdx.x >> 4;
+  __shared__ double red[kMaxPairs][8][kWaves];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool wb = write_back != 0;
-  for (int ti = blockIdx.x; ti < S.nt; ti += gridDim.x) {
-    const int tile = xcd_tile(ti, S.nt, S.xcd);
-    const TileView t = tile_setup<NT>(S, m, tile);
-    tile_load<NT>(S, t, m, psi, m.X);
-    tile_load<NT>(S, t, m, lam, m.L);
+  for (int tile = blockIdx.x; tile < S.nt; tile += gridDim.x) {
+    TileLds m;
+    const TileView t = tile_setup<NT>(S, smem, true, tile, m);
+    tile_load<NT, 2>(S, t, m, psi, lam);
     __syncthreads();
     const int W = t.W;
     double* X = m.X;
@@ -1159,10 +1135,15 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
         }
         double w2[2];
         row_fold8(Q, w2, lane);
-        if ((lane & 15) >= 12) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {  // across the rows of the wave
+          w2[i] += __shfl_xor(w2[i], 16, 64);
+          w2[i] += __shfl_xor(w2[i], 32, 64);
+        }
+        if (lane >= 60) {
           const int q = (lane & 1) * 4 + ((lane >> 1) & 1) * 2;
-          red[j][q][row] = w2[0];
-          red[j][q + 1][row] = w2[1];
+          red[j][q][wave] = w2[0];
+          red[j][q + 1][wave] = w2[1];
         }
       } else {
         const double* B = Mp + ((fa & 3u) == 0u ? 24 : 28);
@@ -1183,12 +1164,13 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
             L[m2] = B[1] * w0 + B[3] * w1;
           }
         }
-        const double s = row_sum(Q[0]);
-        if ((lane & 15) == 15) red[j][0][row] = s;
+        double s = row_sum(Q[0]);
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (lane == 63) red[j][0][wave] = s;
       }
       __syncthreads();
     }
-    const int my_tile = S.tile0 + tile;
     for (int x = threadIdx.x; x < S.npairs * kQVals; x += NT) {
       const int j = x / kQVals, q = x - j * kQVals;
       const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
@@ -1197,13 +1179,11 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
       else src = q == ((fa & 3u) == 0u ? 8 : 9) ? 0 : -1;
       double v = 0.0;
       if (src >= 0)
-        for (int r = 0; r < kRows; ++r) v += red[j][src][r];
-      partial[S.poff + (size_t)x * S.ntiles_stage + my_tile] = v;
+        for (int w = 0; w < kWaves; ++w) v += red[j][src][w];
+      partial[S.poff + (size_t)x * S.nt + tile] = v;
     }
-    if (wb) {  // not for the first stage of the program: the un-applied states are not needed
-      tile_store<NT>(S, t, m, psi, m.X);
-      tile_store<NT>(S, t, m, lam, m.L);
-    }
+    if (wb)  // not for the first stage of the program: the un-applied states are not needed
+      tile_store<NT, 2>(S, t, m, psi, lam);
     __syncthreads();  // the next tile overwrites the LDS tables and red
   }
 }
@@ -1213,23 +1193,14 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
 
 static double stage_bytes(const Space& s, int vectors) { return 16.0 * (double)s.dim * vectors; }
 
-// Persistent grids: a multiple of 8 blocks (one contiguous tile range per
-// XCD), at most blocks_per_cu blocks per CU (SQ_TILE_BPC overrides it for
-// the one-wave kernels), never more than the tiles.
-static unsigned persistent_grid(int ntiles, int blocks_per_cu) {
-  static const int env_bpc = [] {
-    const char* e = getenv("SQ_TILE_BPC");
-    return e ? std::max(1, atoi(e)) : 0;
-  }();
-  if (env_bpc > 0 && blocks_per_cu > 1) blocks_per_cu = env_bpc;
-  const int cap = 256 * blocks_per_cu;
-  int g = std::min(ntiles, cap);
-  g = (g + 7) / 8 * 8;
-  return (unsigned)std::max(g, 8);
+// Tile grids: one block per tile up to blocks_per_cu blocks per CU (more
+// tiles are walked by the same blocks).
+static unsigned tile_grid(int ntiles, int blocks_per_cu) {
+  return (unsigned)std::max(1, std::min(ntiles, 256 * blocks_per_cu));
 }
 
-// Dynamic LDS above 64 KB (large-tile reverse sweep) must be enabled per
-// kernel; the request covers exactly what the launch needs.
+// Dynamic LDS above 64 KB must be enabled per kernel; the request covers
+// exactly what the launch needs.
 template <class K>
 static void allow_lds(K kernel, size_t bytes) {
   static std::mutex mu;
@@ -1249,30 +1220,14 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
           k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
 }
 
-// Large tiles (whole 1024-thread blocks) run first, then the one-wave tiles.
 static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st) {
-  if (S.ntiles > S.nsmall)
-    k_stage_fwd<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT,
-                            lds_bytes(S.large.emax, S.large.wmax, S.npairs, false), st>>>(S.large, S.large.mats,
-                                                                                          dagger ? 1 : 0, psi);
-  if (S.nsmall > 0)
-    k_stage_fwd<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds_bytes(S.small.emax, S.small.wmax, S.npairs, false), st>>>(
-        S.small, S.small.mats, dagger ? 1 : 0, psi);
+  allow_lds(k_stage_fwd<kNT>, S.lds_fwd);
+  k_stage_fwd<kNT><<<tile_grid(S.ntiles, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, psi);
 }
 
 static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
-  if (S.ntiles > S.nsmall) {
-    const size_t lds = lds_bytes(S.large.emax, S.large.wmax, S.npairs, true);
-    allow_lds(k_stage_rev<kLargeNT>, lds);
-    k_stage_rev<kLargeNT><<<persistent_grid(S.ntiles - S.nsmall, 1), kLargeNT, lds, st>>>(S.large, S.large.mats, psi,
-                                                                                          lam, partial, wb);
-  }
-  if (S.nsmall > 0) {
-    const size_t lds = lds_bytes(S.small.emax, S.small.wmax, S.npairs, true);
-    allow_lds(k_stage_rev<kSmallNT>, lds);
-    k_stage_rev<kSmallNT><<<persistent_grid(S.nsmall, 8), kSmallNT, lds, st>>>(S.small, S.small.mats, psi, lam, partial,
-                                                                       wb);
-  }
+  allow_lds(k_stage_rev<kNT>, S.lds_rev);
+  k_stage_rev<kNT><<<tile_grid(S.ntiles, 8), kNT, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
 }
 
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
