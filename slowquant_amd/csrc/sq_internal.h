@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -97,8 +99,12 @@ struct Space {
   int32_t* d_bexc = nullptr;   // [npair][nsb]
   int32_t* d_aexc_full = nullptr;  // [n*n][nsa]: sign*(target+1) for E_pq|s> (x = p*n+q)
   int32_t* d_bexc_full = nullptr;
+  int2* d_gta = nullptr;  // sigma GEMM operand tables [K4 + 16][nsa], [K4 + 16][nsb] (sq_sigma.hip)
+  int2* d_gtb = nullptr;
   std::vector<uint32_t> h_astr, h_bstr;
 };
+
+void build_gemm_tables(Space& s);
 
 struct TilePlan;
 TilePlan* build_tile_plan(const Space& sp, const struct Program& P);
@@ -186,6 +192,22 @@ void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const do
                    double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st);
 void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi, double* lam,
                    double* sp_, double* sl_, double* gwork, double* partial, double* grad_dev, hipStream_t st);
+
+// Dynamic LDS above 64 KB must be enabled per kernel; the request covers
+// exactly what a launch needs (raised only, kept per kernel).
+template <class K>
+inline void allow_lds(K kernel, size_t bytes) {
+  static std::mutex mu;
+  static std::map<const void*, size_t> allowed;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lock(mu);
+  size_t& a = allowed[key];
+  if (a == 0) a = 64 * 1024;
+  if (bytes > a) {
+    SQ_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    a = bytes;
+  }
+}
 
 }  // namespace sq
 

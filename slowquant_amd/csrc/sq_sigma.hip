@@ -177,17 +177,27 @@ __global__ __launch_bounds__(512) void k_gemm_mfma(const double* __restrict__ At
 // its column c; row npair is psi itself, rows above it are zero.
 __constant__ uint8_t c_kp[160], c_kq[160];
 
+// Fused G = A D with D generated in registers.  Row k of D reads one
+// amplitude per spin part: D[k][ia, ib] = ca * psi[ra + ib] + cb * psi[ia*nsb + cb_off]
+// with {ra, ca} = gta[k][ia] (alpha partner row offset, sign / occupation)
+// and {cb_off, cb} = gtb[k][ib]; absent parts carry coefficient 0 and point
+// at the amplitude itself, so the loop has no branches.  The k-steps are
+// unrolled by two with ping-pong registers: step s issues the amplitude
+// loads of step s+1 and the table loads of step s+3 before its MFMAs, and
+// nothing loaded moves between registers.  Steps run to K8 = K4 rounded up
+// to 8 (At slice rows K4..K8 are zero in LDS).
 template <int MT>
 __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ At, const double* __restrict__ psi,
-                                                    const int32_t* __restrict__ aexc, const int32_t* __restrict__ bexc,
+                                                    const int2* __restrict__ gta, const int2* __restrict__ gtb,
                                                     double* __restrict__ G, int npair, int K4, int kpad, int nsa,
                                                     int nsb) {
   extern __shared__ double sA[];
   constexpr int W = MT * 16;
+  const int K8 = (K4 + 7) & ~7;
   const int m0 = blockIdx.y * W;
-  for (int i = threadIdx.x; i < K4 * W; i += blockDim.x) {
+  for (int i = threadIdx.x; i < K8 * W; i += blockDim.x) {
     int k = i / W, m = m0 + i % W;
-    sA[i] = m < kpad ? At[(size_t)k * kpad + m] : 0.0;
+    sA[i] = (m < kpad && k < K4) ? At[(size_t)k * kpad + m] : 0.0;
   }
   __syncthreads();
   const int64_t dim = (int64_t)nsa * nsb;
@@ -195,32 +205,57 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
   const int64_t c = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 16 + (lane & 15);
   const int kq = lane >> 4;
   const bool valid = c < dim;
-  const int ia = valid ? (int)(c / nsb) : 0;
-  const int ib = valid ? (int)(c - (int64_t)ia * nsb) : 0;
-  const double* row = psi + (size_t)ia * nsb;
-  const double pc = valid ? psi[c] : 0.0;
-  auto dval = [&](int k) -> double {
-    if (!valid || k > npair) return 0.0;
-    if (k == npair) return pc;
-    const int32_t ea = aexc[(size_t)k * nsa + ia];
-    const int32_t eb = bexc[(size_t)k * nsb + ib];
-    if (c_kp[k] == c_kq[k]) return (double)(ea + eb) * pc;
-    double v = 0.0;
-    if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
-    if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
-    return v;
-  };
+  const int64_t cc = valid ? c : dim - 1;  // columns past the end compute garbage that is never stored
+  const int ia = (int)(cc / nsb);
+  const int ib = (int)(cc - (int64_t)ia * nsb);
+  const int2* ta = gta + (size_t)kq * nsa + ia;
+  const int2* tb = gtb + (size_t)kq * nsb + ib;
+  const size_t sa = 4 * (size_t)nsa, sb = 4 * (size_t)nsb;
+  const double* pcol = psi + ib;
+  const double* prow = psi + (size_t)ia * nsb;
   typedef double double4_t __attribute__((ext_vector_type(4)));
   double4_t acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-  double bnext = dval(kq);
-  for (int k4 = 0; k4 < K4; k4 += 4) {
-    const double b = bnext;
-    if (k4 + 4 < K4) bnext = dval(k4 + 4 + kq);
-    const double* arow = sA + (size_t)(k4 + kq) * W + (lane & 15);
+  // slot 0: even steps, slot 1: odd steps
+  int2 ta0 = ta[0], tb0 = tb[0];
+  int2 ta1 = ta[sa], tb1 = tb[sb];
+  double xa0 = pcol[ta0.x], xb0 = prow[tb0.x];
+  double ca0 = (double)ta0.y, cb0 = (double)tb0.y;
+  double xa1, xb1, ca1, cb1;
+  ta0 = ta[2 * sa];
+  tb0 = tb[2 * sb];
+  const double* arow = sA + (size_t)kq * W + (lane & 15);
+  const int nsteps = K8 / 4;
+  for (int s = 0; s < nsteps; s += 2) {
+    // step s
+    xa1 = pcol[ta1.x];
+    xb1 = prow[tb1.x];
+    ca1 = (double)ta1.y;
+    cb1 = (double)tb1.y;
+    ta1 = ta[(size_t)(s + 3) * sa];
+    tb1 = tb[(size_t)(s + 3) * sb];
+    __asm__ volatile("" ::: "memory");  // keep the loads ahead of this step's MFMAs
+    {
+      const double b = ca0 * xa0 + cb0 * xb0;
 #pragma unroll
-    for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+      for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+    }
+    arow += 4 * W;
+    // step s + 1
+    xa0 = pcol[ta0.x];
+    xb0 = prow[tb0.x];
+    ca0 = (double)ta0.y;
+    cb0 = (double)tb0.y;
+    ta0 = ta[(size_t)(s + 4) * sa];
+    tb0 = tb[(size_t)(s + 4) * sb];
+    __asm__ volatile("" ::: "memory");
+    {
+      const double b = ca1 * xa1 + cb1 * xb1;
+#pragma unroll
+      for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+    }
+    arow += 4 * W;
   }
   if (!valid) return;
 #pragma unroll
@@ -231,6 +266,45 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
       if (m < npair) G[(size_t)m * dim + c] = acc[t][r];
     }
   }
+}
+
+// GEMM operand tables (rows k < K4 + 16; the rows past K4 feed the steps
+// up to K8 and the prefetch past the end): alpha strings {partner row * nsb, coefficient}, beta strings
+// {partner column, coefficient}.  Diagonal pairs (p, p) read the amplitude
+// itself with the occupation of p; row npair is psi (alpha part, 1); absent
+// parts and padding rows read the amplitude itself with coefficient 0.
+__global__ void k_gemm_tables(int npair, int nstr, int nsb, int alpha, const int32_t* __restrict__ exc,
+                              int2* __restrict__ tab) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (s >= nstr) return;
+  const int own = alpha ? s * nsb : s;
+  int2 e = make_int2(own, 0);
+  if (k < npair) {
+    int p = 0;
+    while ((p + 1) * (p + 2) / 2 <= k) ++p;
+    const int q = k - p * (p + 1) / 2;
+    const int32_t x = exc[(size_t)k * nstr + s];
+    if (p == q) e = make_int2(own, x);
+    else if (x != 0) e = make_int2(alpha ? (abs(x) - 1) * nsb : abs(x) - 1, x > 0 ? 1 : -1);
+  } else if (k == npair && alpha) {
+    e = make_int2(own, 1);
+  }
+  tab[(size_t)k * nstr + s] = e;
+}
+
+void build_gemm_tables(Space& s) {
+  const SigmaDims d = sigma_dims(s);
+  const int rows = d.K4 + 16;
+  SQ_HIP(hipMalloc(&s.d_gta, sizeof(int2) * (size_t)rows * std::max(s.nsa, 1)));
+  SQ_HIP(hipMalloc(&s.d_gtb, sizeof(int2) * (size_t)rows * std::max(s.nsb, 1)));
+  if (s.nsa > 0 && s.nsb > 0) {
+    k_gemm_tables<<<dim3((s.nsa + 255) / 256, rows), 256>>>(s.npair, s.nsa, s.nsb, 1, s.d_aexc, s.d_gta);
+    SQ_HIP(hipGetLastError());
+    k_gemm_tables<<<dim3((s.nsb + 255) / 256, rows), 256>>>(s.npair, s.nsb, s.nsb, 0, s.d_bexc, s.d_gtb);
+    SQ_HIP(hipGetLastError());
+  }
+  SQ_HIP(hipDeviceSynchronize());
 }
 
 static void upload_pair_tables(int n) {
@@ -250,11 +324,12 @@ static void launch_gemm_fused(const Space& s, const SigmaDims& d, const double* 
   const int threads = 512;
   const int64_t cols_per_block = (threads / 64) * 16;
   const int mt_total = d.kpad / 16;
+  const int K8 = (d.K4 + 7) & ~7;
   int mt = mt_total;
-  while (mt > 1 && (size_t)d.K4 * mt * 16 * sizeof(double) > 65536) --mt;
+  while (mt > 1 && (size_t)K8 * mt * 16 * sizeof(double) > 65536) --mt;
   const int groups = (mt_total + mt - 1) / mt;
   dim3 grid((unsigned)((d.dim + cols_per_block - 1) / cols_per_block), groups);
-  size_t lds = sizeof(double) * (size_t)d.K4 * mt * 16;
+  size_t lds = sizeof(double) * (size_t)K8 * mt * 16;
   // algorithmic traffic: psi read once per m-group (+ gathers hitting cache), G written
   const double gbytes = 8.0 * (double)d.dim * (groups + d.npair);
   const double gflops = 2.0 * (double)d.npair * (d.npair + 1) * (double)d.dim;
@@ -262,7 +337,7 @@ static void launch_gemm_fused(const Space& s, const SigmaDims& d, const double* 
 #define SQ_GF(MT)                                                                                          \
   case MT:                                                                                                 \
     SQ_PROF(PC_GEMM, st, gbytes, gflops,                                                                   \
-            k_gemm_fused<MT><<<grid, threads, lds, st>>>(At, psi, s.d_aexc, s.d_bexc, G, d.npair, d.K4,    \
+            k_gemm_fused<MT><<<grid, threads, lds, st>>>(At, psi, s.d_gta, s.d_gtb, G, d.npair, d.K4,      \
                                                          d.kpad, s.nsa, s.nsb));                           \
     break;
     SQ_GF(1) SQ_GF(2) SQ_GF(3) SQ_GF(4) SQ_GF(5) SQ_GF(6) SQ_GF(7) SQ_GF(8) SQ_GF(9)

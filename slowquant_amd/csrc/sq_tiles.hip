@@ -1199,22 +1199,6 @@ static unsigned tile_grid(int ntiles, int blocks_per_cu) {
   return (unsigned)std::max(1, std::min(ntiles, 256 * blocks_per_cu));
 }
 
-// Dynamic LDS above 64 KB must be enabled per kernel; the request covers
-// exactly what the launch needs.
-template <class K>
-static void allow_lds(K kernel, size_t bytes) {
-  static std::mutex mu;
-  static std::map<const void*, size_t> allowed;  // per kernel
-  const void* key = reinterpret_cast<const void*>(kernel);
-  std::lock_guard<std::mutex> lock(mu);
-  size_t& a = allowed[key];
-  if (a == 0) a = 64 * 1024;
-  if (bytes > a) {
-    SQ_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    a = bytes;
-  }
-}
-
 static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, hipStream_t st) {
   SQ_PROF(PC_OTHER, st, 0.0, 0.0,
           k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
