@@ -187,7 +187,8 @@ __constant__ uint8_t c_kp[160], c_kq[160];
 // nothing loaded moves between registers.  Steps run to K8 = K4 rounded up
 // to 8 (At slice rows K4..K8 are zero in LDS).
 template <int MT>
-__global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ At, const double* __restrict__ psi,
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(MT <= 5 ? 6 : 4, 8)))
+void k_gemm_fused(const double* __restrict__ At, const double* __restrict__ psi,
                                                     const int2* __restrict__ gta, const int2* __restrict__ gtb,
                                                     double* __restrict__ G, int npair, int K4, int kpad, int nsa,
                                                     int nsb) {
@@ -208,50 +209,50 @@ __global__ __launch_bounds__(512) void k_gemm_fused(const double* __restrict__ A
   const int64_t cc = valid ? c : dim - 1;  // columns past the end compute garbage that is never stored
   const int ia = (int)(cc / nsb);
   const int ib = (int)(cc - (int64_t)ia * nsb);
-  const int2* ta = gta + (size_t)kq * nsa + ia;
-  const int2* tb = gtb + (size_t)kq * nsb + ib;
-  const size_t sa = 4 * (size_t)nsa, sb = 4 * (size_t)nsb;
-  const double* pcol = psi + ib;
-  const double* prow = psi + (size_t)ia * nsb;
+  // 32-bit element indices over uniform bases (dim < 2^29 keeps byte offsets in 32 bits)
+  const uint32_t ta_i = (uint32_t)(kq * nsa + ia), tb_i = (uint32_t)(kq * nsb + ib);
+  const uint32_t sa = 4u * (uint32_t)nsa, sb = 4u * (uint32_t)nsb;
+  const uint32_t row = (uint32_t)ia * (uint32_t)nsb, col = (uint32_t)ib;
   typedef double double4_t __attribute__((ext_vector_type(4)));
   double4_t acc[MT];
 #pragma unroll
   for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
   // slot 0: even steps, slot 1: odd steps
-  int2 ta0 = ta[0], tb0 = tb[0];
-  int2 ta1 = ta[sa], tb1 = tb[sb];
-  double xa0 = pcol[ta0.x], xb0 = prow[tb0.x];
-  double ca0 = (double)ta0.y, cb0 = (double)tb0.y;
-  double xa1, xb1, ca1, cb1;
-  ta0 = ta[2 * sa];
-  tb0 = tb[2 * sb];
+  int2 ta0 = gta[ta_i], tb0 = gtb[tb_i];
+  int2 ta1 = gta[ta_i + sa], tb1 = gtb[tb_i + sb];
+  double xa0 = psi[(uint32_t)ta0.x + col], xb0 = psi[row + (uint32_t)tb0.x];
+  int ca0 = ta0.y, cb0 = tb0.y;
+  double xa1, xb1;
+  int ca1, cb1;
+  ta0 = gta[ta_i + 2 * sa];
+  tb0 = gtb[tb_i + 2 * sb];
   const double* arow = sA + (size_t)kq * W + (lane & 15);
   const int nsteps = K8 / 4;
   for (int s = 0; s < nsteps; s += 2) {
     // step s
-    xa1 = pcol[ta1.x];
-    xb1 = prow[tb1.x];
-    ca1 = (double)ta1.y;
-    cb1 = (double)tb1.y;
-    ta1 = ta[(size_t)(s + 3) * sa];
-    tb1 = tb[(size_t)(s + 3) * sb];
+    xa1 = psi[(uint32_t)ta1.x + col];
+    xb1 = psi[row + (uint32_t)tb1.x];
+    ca1 = ta1.y;
+    cb1 = tb1.y;
+    ta1 = gta[ta_i + (uint32_t)(s + 3) * sa];
+    tb1 = gtb[tb_i + (uint32_t)(s + 3) * sb];
     __asm__ volatile("" ::: "memory");  // keep the loads ahead of this step's MFMAs
     {
-      const double b = ca0 * xa0 + cb0 * xb0;
+      const double b = (double)ca0 * xa0 + (double)cb0 * xb0;
 #pragma unroll
       for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
     }
     arow += 4 * W;
     // step s + 1
-    xa0 = pcol[ta0.x];
-    xb0 = prow[tb0.x];
-    ca0 = (double)ta0.y;
-    cb0 = (double)tb0.y;
-    ta0 = ta[(size_t)(s + 4) * sa];
-    tb0 = tb[(size_t)(s + 4) * sb];
+    xa0 = psi[(uint32_t)ta0.x + col];
+    xb0 = psi[row + (uint32_t)tb0.x];
+    ca0 = ta0.y;
+    cb0 = tb0.y;
+    ta0 = gta[ta_i + (uint32_t)(s + 4) * sa];
+    tb0 = gtb[tb_i + (uint32_t)(s + 4) * sb];
     __asm__ volatile("" ::: "memory");
     {
-      const double b = ca1 * xa1 + cb1 * xb1;
+      const double b = (double)ca1 * xa1 + (double)cb1 * xb1;
 #pragma unroll
       for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
     }
