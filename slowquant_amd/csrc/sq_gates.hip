@@ -741,12 +741,12 @@ int sq_state_prep(const sq_space* sp, const sq_program* pr, const double* theta,
     const Program& p = pr->p;
     SQ_CHECK(theta || p.gates.empty(), SQ_ERR_ARG, "theta is NULL");
     hipStream_t st = as_stream(stream);
-    copy_vec(psi_in, psi_out, s.dim, st);
     const TilePlan* T = tiles_for(s, p);
     if (T) {
-      tiles_forward(s, p, *T, theta, psi_out, work + 2 * s.dim, work, -1, dagger != 0, st);
+      tiles_forward(s, p, *T, theta, psi_in, psi_out, work + 2 * s.dim, work, -1, dagger != 0, st);
       return;
     }
+    copy_vec(psi_in, psi_out, s.dim, st);
     int n = (int)p.gates.size();
     for (int k = 0; k < n; ++k) {
       int g = dagger ? n - 1 - k : k;

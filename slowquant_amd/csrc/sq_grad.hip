@@ -48,12 +48,12 @@ static void finish(const GradBufs& b, int n, double* energy, double* grad, hipSt
 // inserts T_g after gate g.
 static void forward(const Space& s, const Program& p, const double* theta, const double* psi0, double* psi,
                     const GradBufs& b, int deriv_gate, hipStream_t st) {
-  copy_vec(psi0, psi, s.dim, st);
   const TilePlan* T = tiles_for(s, p);
   if (T) {
-    tiles_forward(s, p, *T, theta, psi, b.s1, b.t1, deriv_gate, false, st);
+    tiles_forward(s, p, *T, theta, psi0, psi, b.s1, b.t1, deriv_gate, false, st);
     return;
   }
+  copy_vec(psi0, psi, s.dim, st);
   for (int g = 0; g < (int)p.gates.size(); ++g) {
     apply_gate_impl(s, p, g, theta, false, psi, b.t1, st);
     if (g == deriv_gate) {
@@ -71,7 +71,7 @@ void energy_grad_adjoint(const Space& s, const Program& p, const Hamiltonian& H,
   sigma_energy_dev(s, H, b.psi, b.lam, b.scal, b.sigw, st);
   const TilePlan* T = tiles_for(s, p);
   if (T) {
-    tiles_reverse(s, p, *T, theta, b.psi, b.lam, b.s1, b.s2, b.t1, b.partial, b.grad, st);
+    tiles_reverse(s, p, *T, theta, b.psi, b.lam, b.t1, b.partial, b.grad, true, st);  // matrices from forward()
   } else {
     for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, b.psi, b.lam, b.grad, b.t1, b.partial, st);
   }
@@ -86,7 +86,7 @@ void adjoint_reverse(const Space& s, const Program& p, const double* theta, doub
   GradBufs b = carve(s, n, work);
   const TilePlan* T = tiles_for(s, p);
   if (T) {
-    tiles_reverse(s, p, *T, theta, psi, lam, b.s1, b.s2, b.t1, b.partial, b.grad, st);
+    tiles_reverse(s, p, *T, theta, psi, lam, b.t1, b.partial, b.grad, false, st);
   } else {
     for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, psi, lam, b.grad, b.t1, b.partial, st);
   }
@@ -118,7 +118,7 @@ void energy_grad_reference_sweep(const Space& s, const Program& p, const Hamilto
   // bra = U^+ H psi
   const TilePlan* T = tiles_for(s, p);
   if (T) {
-    tiles_forward(s, p, *T, theta, b.lam, b.s1, b.t1, -1, true, st);
+    tiles_forward(s, p, *T, theta, b.lam, b.lam, b.s1, b.t1, -1, true, st);
   } else {
     for (int g = n - 1; g >= 0; --g) apply_gate_impl(s, p, g, theta, true, b.lam, b.t1, st);
   }

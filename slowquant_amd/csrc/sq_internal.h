@@ -188,10 +188,10 @@ void reverse_step_impl(const Space& s, const Program& p, int g, const double* th
 
 // Implemented in sq_tiles.hip
 const TilePlan* tiles_for(const Space& s, const Program& p);  // nullptr when no tiled stage applies
-void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi,
-                   double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st);
+void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, const double* psi_in,
+                   double* psi, double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st);
 void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi, double* lam,
-                   double* sp_, double* sl_, double* gwork, double* partial, double* grad_dev, hipStream_t st);
+                   double* gwork, double* partial, double* grad_dev, bool mats_ready, hipStream_t st);
 
 // Dynamic LDS above 64 KB must be enabled per kernel; the request covers
 // exactly what a launch needs (raised only, kept per kernel).

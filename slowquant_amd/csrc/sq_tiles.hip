@@ -667,38 +667,55 @@ __device__ __forceinline__ void t_rows(uint32_t code, double scale, const double
 __device__ __constant__ int c_cls_type[5] = {8, 6, 7, 2, 5};
 __device__ __constant__ int c_cls_off[5] = {0, 16, 20, 24, 28};
 
-__device__ __forceinline__ void entry_cs(const StageEntry& en, const double* __restrict__ theta, double& c,
-                                         double& s) {
-  const double th = theta[en.gate];
-  s = 0.0;
-  c = 1.0;
-  if (fabs(th) >= 1e-28) sincos(en.scale * th, &s, &c);
+// Stage entries [e0, e0 + n) staged in LDS with their rotation cosine/sine
+// (one thread per entry), so the products below run on LDS values instead of
+// chains of dependent global loads.
+struct EntryLds {
+  int gate[kMaxStageGates];
+  double scale[kMaxStageGates], c[kMaxStageGates], s[kMaxStageGates];
+  uint32_t code[kMaxStageGates][kTypes];
+};
+
+__device__ __forceinline__ void stage_entries(const StageEntry* __restrict__ entries, int e0, int n,
+                                              const double* __restrict__ theta, EntryLds& E) {
+  for (int e = threadIdx.x; e < n; e += blockDim.x) {
+    const StageEntry& en = entries[e0 + e];
+    const double th = theta[en.gate];
+    double s = 0.0, c = 1.0;
+    if (fabs(th) >= 1e-28) sincos(en.scale * th, &s, &c);
+    E.gate[e] = en.gate;
+    E.scale[e] = en.scale;
+    E.c[e] = c;
+    E.s[e] = s;
+#pragma unroll
+    for (int t = 0; t < kTypes; ++t) E.code[e][t] = en.code[t];
+  }
 }
 
 // One thread per (stage, pair, class): the pair's gate product in forward
 // order, with T_g inserted after gate g when deriv_gate is on this pair.
 __global__ __launch_bounds__(64) void k_stage_mats(const StageInfo* __restrict__ info, const double* __restrict__ theta, int deriv_gate,
                              double* __restrict__ mats) {
-  const StageInfo I = info[blockIdx.x];
+  __shared__ EntryLds E;
+  const StageInfo* __restrict__ I = info + blockIdx.x;
+  stage_entries(I->entries, 0, I->nentries, theta, E);
+  __syncthreads();
   const int j = threadIdx.x / 5, cls = threadIdx.x % 5;
-  if (j >= I.npairs) return;
+  if (j >= I->npairs) return;
   const int t = c_cls_type[cls];
   double M[16];
   #pragma unroll
   for (int q = 0; q < 16; ++q) M[q] = (q % 5 == 0) ? 1.0 : 0.0;
-  const int k0 = I.pe[j], k1 = I.pe[j + 1];
+  const int k0 = I->pe[j], k1 = I->pe[j + 1];
   bool has_deriv = false;
   for (int k = k0; k < k1; ++k) {
-    const StageEntry& en = I.entries[k];
-    double c, s;
-    entry_cs(en, theta, c, s);
-    rot_rows(en.code[t], c, s, M);
-    if (en.gate == deriv_gate && (k + 1 == k1 || I.entries[k + 1].gate != deriv_gate)) {
+    rot_rows(E.code[k][t], E.c[k], E.s[k], M);
+    if (E.gate[k] == deriv_gate && (k + 1 == k1 || E.gate[k + 1] != deriv_gate)) {
       has_deriv = true;
       double o[16];
       #pragma unroll
       for (int q = 0; q < 16; ++q) o[q] = 0.0;
-      for (int q = k; q >= k0 && I.entries[q].gate == deriv_gate; --q) t_rows(I.entries[q].code[t], I.entries[q].scale, M, o);
+      for (int q = k; q >= k0 && E.gate[q] == deriv_gate; --q) t_rows(E.code[q][t], E.scale[q], M, o);
       #pragma unroll
       for (int q = 0; q < 16; ++q) M[q] = o[q];
     }
@@ -730,10 +747,14 @@ __global__ __launch_bounds__(kTB) void k_stage_grad(const StageInfo* __restrict_
   __shared__ double Q[kQVals];
   __shared__ double dots[kMaxPairEntries * 5];
   __shared__ double red[kQVals * (kTB / 64)];
+  __shared__ EntryLds E;
   const StageInfo I = info[blockIdx.x / kMaxPairs];
   const int j = blockIdx.x % kMaxPairs;
   if (j >= I.npairs) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int k0 = I.pe[j], k1 = I.pe[j + 1];
+  const int nent = k1 - k0;
+  stage_entries(I.entries, k0, nent, theta, E);  // entries of this pair, indexed from 0
   {
     const double* src = partial + I.poff + (size_t)j * kQVals * I.ntiles;
     double acc[kQVals];
@@ -757,10 +778,8 @@ __global__ __launch_bounds__(kTB) void k_stage_grad(const StageInfo* __restrict_
     }
     __syncthreads();
   }
-  const int k0 = I.pe[j], k1 = I.pe[j + 1];
-  const int nent = k1 - k0;
   if (threadIdx.x < nent * 5) {
-    const int k = k0 + threadIdx.x / 5, cls = threadIdx.x % 5;
+    const int k = threadIdx.x / 5, cls = threadIdx.x % 5;
     const int t = c_cls_type[cls];
     double M[16];
     #pragma unroll
@@ -768,12 +787,10 @@ __global__ __launch_bounds__(kTB) void k_stage_grad(const StageInfo* __restrict_
     double I4[16];
     #pragma unroll
     for (int q = 0; q < 16; ++q) I4[q] = (q % 5 == 0) ? 1.0 : 0.0;
-    t_rows(I.entries[k].code[t], 1.0, I4, M);  // M = T_k
-    for (int e = k + 1; e < k1; ++e) {
-      double c, s;
-      entry_cs(I.entries[e], theta, c, s);
-      rot_rows(I.entries[e].code[t], c, s, M);
-      rot_cols(I.entries[e].code[t], c, s, M);
+    t_rows(E.code[k][t], 1.0, I4, M);  // M = T_k
+    for (int e = k + 1; e < nent; ++e) {
+      rot_rows(E.code[e][t], E.c[e], E.s[e], M);
+      rot_cols(E.code[e][t], E.c[e], E.s[e], M);
     }
     double d = 0.0;
     if (cls == 0) {
@@ -782,19 +799,16 @@ __global__ __launch_bounds__(kTB) void k_stage_grad(const StageInfo* __restrict_
     } else {
       d = (cls <= 2 ? M[1] : M[2]) * Q[5 + cls];  // M[0][m1], m1 = 1 (alpha pair) or 2 (beta pair)
     }
-    dots[threadIdx.x] = d * I.entries[k].scale;
+    dots[threadIdx.x] = d * E.scale[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int q = 0; q < nent;) {
-      const int g = I.entries[k0 + q].gate;
-      double tot = 0.0;
-      while (q < nent && I.entries[k0 + q].gate == g) {
-        for (int cls = 0; cls < 5; ++cls) tot += dots[q * 5 + cls];
-        ++q;
-      }
-      grad[g] = 2.0 * tot;
-    }
+  // one thread per gate: the first entry of each run of equal gates sums the run
+  if (threadIdx.x < nent && (threadIdx.x == 0 || E.gate[threadIdx.x - 1] != E.gate[threadIdx.x])) {
+    const int g = E.gate[threadIdx.x];
+    double tot = 0.0;
+    for (int q = threadIdx.x; q < nent && E.gate[q] == g; ++q)
+      for (int cls = 0; cls < 5; ++cls) tot += dots[q * 5 + cls];
+    grad[g] = 2.0 * tot;
   }
 }
 
@@ -1029,23 +1043,24 @@ __device__ __forceinline__ double row_sum(double v) {
 }
 
 // Gates of one stage on one vector (canonical order, in place).
-// dagger: U^+ (transposed pair matrices).
+// dagger: U^+ (transposed pair matrices).  src may be dst (in place) or the
+// sweep's input vector (first stage).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ mats, int dagger,
-                                                  double* __restrict__ psi) {
+                                                  const double* src, double* dst) {
   extern __shared__ double smem[];
   // block b walks tiles b, b + grid, ... (tiles are sorted largest first)
   for (int tile = blockIdx.x; tile < S.nt; tile += gridDim.x) {
     TileLds m;
     const TileView t = tile_setup<NT>(S, smem, false, tile, m);
-    tile_load<NT, 1>(S, t, m, psi, nullptr);
+    tile_load<NT, 1>(S, t, m, src, nullptr);
     __syncthreads();
     for (int j = 0; j < S.npairs; ++j) {
       if (dagger) fwd_pair<NT, true>(t, m, mats + j * kMatVals, j);
       else fwd_pair<NT, false>(t, m, mats + j * kMatVals, j);
       __syncthreads();
     }
-    tile_store<NT, 1>(S, t, m, psi, nullptr);
+    tile_store<NT, 1>(S, t, m, dst, nullptr);
     __syncthreads();  // the next tile overwrites the LDS tables
   }
 }
@@ -1204,9 +1219,9 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
           k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
 }
 
-static void launch_fwd(const Stage& S, bool dagger, double* psi, hipStream_t st) {
+static void launch_fwd(const Stage& S, bool dagger, const double* src, double* dst, hipStream_t st) {
   allow_lds(k_stage_fwd<kNT>, S.lds_fwd);
-  k_stage_fwd<kNT><<<tile_grid(S.ntiles, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, psi);
+  k_stage_fwd<kNT><<<tile_grid(S.ntiles, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
 }
 
 static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
@@ -1215,18 +1230,23 @@ static void launch_rev(const Stage& S, double* psi, double* lam, double* partial
 }
 
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
-// program, tiles where possible, in place on psi (canonical order).
+// program, tiles where possible, psi = U psi_in (canonical order; psi_in may
+// be psi).  A first tiled stage reads psi_in itself (its tiles cover every
+// amplitude), otherwise psi_in is copied first.
 // scratch >= dim doubles (derivative insertion on per-gate stages);
 // gwork >= 2*dim doubles for gates that run through the per-gate kernels.
 // deriv_gate >= 0 inserts T_g right after gate g (plain gradient; forward only).
-void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi,
-                   double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st) {
+void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, const double* psi_in,
+                   double* psi, double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st) {
   launch_mats(T, theta, dagger ? -1 : deriv_gate, st);
   const int ns = (int)T.stages.size();
+  const double* src = psi_in;
   for (int ii = 0; ii < ns; ++ii) {
     const int i = dagger ? ns - 1 - ii : ii;
     const Stage& S = T.stages[i];
     if (!S.tiled) {
+      if (src != psi) copy_vec(src, psi, s.dim, st);
+      src = psi;
       apply_gate_impl(s, p, S.g0, theta, dagger, psi, gwork, st);
       if (S.g0 == deriv_gate) {
         grad_action_impl(s, p, S.g0, psi, scratch, st);
@@ -1234,17 +1254,19 @@ void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const do
       }
       continue;
     }
-    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, dagger, psi, st));
+    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, dagger, src, psi, st));
+    src = psi;
   }
+  if (src != psi) copy_vec(src, psi, s.dim, st);
 }
 
 // Reverse sweep on (psi, lam) with gradient dots into grad_dev[g] (device).
 // gwork >= 2*dim; partial: reduction scratch of the per-gate kernels.
+// mats_ready: the pair matrices of theta (no derivative insertion) are still
+// in place from a forward sweep on this stream.
 void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi, double* lam,
-                   double* sp_, double* sl_, double* gwork, double* partial, double* grad_dev, hipStream_t st) {
-  (void)sp_;
-  (void)sl_;
-  launch_mats(T, theta, -1, st);
+                   double* gwork, double* partial, double* grad_dev, bool mats_ready, hipStream_t st) {
+  if (!mats_ready) launch_mats(T, theta, -1, st);
   for (int i = (int)T.stages.size() - 1; i >= 0; --i) {
     const Stage& S = T.stages[i];
     if (!S.tiled) {
