@@ -14,6 +14,9 @@ The JSON line also carries:
   roofline        dominant kernel: algorithmic bytes or flops per launch over
                   its average duration, timed with HIP events on the engine's
                   stream in a separate profiled pass over the same workload
+  rotation        the forward sweep by per-gate rotation kernels (HBM
+                  fraction they reach) and by the fused tile stages (the
+                  per-gate sweep's bytes over the fused sweep's time)
   cpu_baseline    the reference algorithm (C port of its Numba scan kernel,
                   Python driver as in the reference) on a bounded sample,
                   extrapolated to one evaluation
@@ -82,6 +85,57 @@ def measured_traffic(kernel_class):
     except OSError:
         pass
     return None
+
+
+def rotation_profile(sp, lay, prog, theta, psi0):
+    """The forward sweep U(theta) psi0 two ways: per-gate rotation kernels (a
+    program built with SQ_TILES=0: one HBM pass per rotation string over the
+    amplitudes it touches) and the fused tile stages.  The per-gate sweep's
+    bytes over the fused sweep's time is the bandwidth an unfused sweep would
+    need to match it."""
+    from slowquant_amd import engine as E
+    from slowquant_amd import profiling
+
+    prev = os.environ.get("SQ_TILES")
+    os.environ["SQ_TILES"] = "0"
+    try:
+        prog_pg = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+        prog_pg.state_prep(theta, psi0)  # the tile decision is taken on first use
+    finally:
+        if prev is None:
+            del os.environ["SQ_TILES"]
+        else:
+            os.environ["SQ_TILES"] = prev
+    prog.state_prep(theta, psi0)
+    reps = 3
+    pg = profiling.profile(lambda: prog_pg.state_prep(theta, psi0), reps)
+    fu = profiling.profile(lambda: prog.state_prep(theta, psi0), reps)
+    rot = pg.get("rotate", {})
+    pg_bytes = rot["bytes_per_launch"] * rot["launches_per_call"] if rot else 0.0  # rotations only
+    pg_ms = sum(k["ms_per_call"] for k in pg.values())
+    fu_ms = sum(k["ms_per_call"] for k in fu.values())
+    tf = fu.get("tile_forward", {})
+    return {
+        "per_gate_kernels": {
+            "kernel": "k_rotate",
+            "launches_per_sweep": rot.get("launches_per_call"),
+            "avg_us": rot.get("avg_us"),
+            "achieved_GBs": rot.get("achieved_GBs"),
+            "frac_of_hbm": None if not rot else rot["achieved_GBs"] / HBM_PEAK_GBS,
+            "sweep_bytes": pg_bytes,
+            "sweep_ms": pg_ms,
+        },
+        "fused_stages": {
+            "kernel": "k_stage_fwd",
+            "launches_per_sweep": tf.get("launches_per_call"),
+            "avg_us": tf.get("avg_us"),
+            "achieved_GBs": tf.get("achieved_GBs"),
+            "sweep_ms": fu_ms,
+            "per_gate_bytes_over_fused_time_GBs": pg_bytes / (fu_ms * 1e-3) / 1e9 if fu_ms > 0 else None,
+            "per_gate_bytes_over_fused_time_frac_of_hbm": (pg_bytes / (fu_ms * 1e-3) / 1e9) / HBM_PEAK_GBS
+            if fu_ms > 0 else None,
+        },
+    }
 
 
 def kernel_profile(sp, prog, H, theta, psi0, reps):
@@ -282,6 +336,7 @@ def main() -> int:
         "energy": e_adj,
         "roofline": roof,
         "kernels": prof["kernels"],
+        "rotation": rotation_profile(sp, lay, prog, th, psi0),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(h, g, th, lay, args.cpu_budget)

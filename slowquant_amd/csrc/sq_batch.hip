@@ -321,43 +321,7 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       std::map<std::pair<const void*, std::vector<int>>, std::pair<int, int>> list_cache;
       // (string, image | parity << 31) of every string a spin part applies to
       auto spin_list = [&](const std::vector<uint32_t>& strs, const SpinOps& o, int& off, int& len) {
-        if (o.n == 0) {
-          off = -1;
-          len = (int)strs.size();
-          return;
-        }
-        std::vector<int> key{o.n};
-        for (int j = 0; j < o.n; ++j) key.push_back(o.orb[j] * 2 + (o.cre[j] ? 1 : 0));
-        auto ck = std::make_pair((const void*)&strs, key);
-        auto it = list_cache.find(ck);
-        if (it != list_cache.end()) {
-          off = it->second.first;
-          len = it->second.second;
-          return;
-        }
-        std::map<uint32_t, int> rank;
-        for (int i = 0; i < (int)strs.size(); ++i) rank[strs[i]] = i;
-        off = (int)lists.size();
-        for (int i = 0; i < (int)strs.size(); ++i) {
-          uint32_t m = strs[i];
-          int par = 0;
-          bool ok = true;
-          for (int j = o.n - 1; j >= 0 && ok; --j) {
-            const uint32_t bit = 1u << o.orb[j];
-            par += __builtin_popcount(m & (bit - 1u));
-            if (o.cre[j]) {
-              if (m & bit) ok = false;
-              m |= bit;
-            } else {
-              if (!(m & bit)) ok = false;
-              m &= ~bit;
-            }
-          }
-          if (!ok) continue;
-          lists.push_back(int2{i, rank.at(m) | (int)((uint32_t)(par & 1) << 31)});
-        }
-        len = (int)lists.size() - off;
-        list_cache[ck] = {off, len};
+        spin_part_list(strs, o, lists, list_cache, off, len);
       };
       std::vector<BatchSpace> bsp;
       std::vector<BatchRot> rots;

@@ -108,6 +108,48 @@ static int64_t count_applies(const SpinOps& o, const std::vector<uint32_t>& strs
   return c;
 }
 
+void spin_part_list(const std::vector<uint32_t>& strs, const SpinOps& o, std::vector<int2>& lists,
+                    std::map<std::pair<const void*, std::vector<int>>, std::pair<int, int>>& cache, int& off,
+                    int& len) {
+  if (o.n == 0) {
+    off = -1;
+    len = (int)strs.size();
+    return;
+  }
+  std::vector<int> key{o.n};
+  for (int j = 0; j < o.n; ++j) key.push_back(o.orb[j] * 2 + (o.cre[j] ? 1 : 0));
+  const auto ck = std::make_pair((const void*)&strs, key);
+  const auto it = cache.find(ck);
+  if (it != cache.end()) {
+    off = it->second.first;
+    len = it->second.second;
+    return;
+  }
+  std::map<uint32_t, int> rank;
+  for (int i = 0; i < (int)strs.size(); ++i) rank[strs[i]] = i;
+  off = (int)lists.size();
+  for (int i = 0; i < (int)strs.size(); ++i) {
+    uint32_t m = strs[i];
+    int par = 0;
+    bool ok = true;
+    for (int j = o.n - 1; j >= 0 && ok; --j) {
+      const uint32_t bit = 1u << o.orb[j];
+      par += __builtin_popcount(m & (bit - 1u));
+      if (o.cre[j]) {
+        if (m & bit) ok = false;
+        m |= bit;
+      } else {
+        if (!(m & bit)) ok = false;
+        m &= ~bit;
+      }
+    }
+    if (!ok) continue;
+    lists.push_back(int2{i, rank.at(m) | (int)((uint32_t)(par & 1) << 31)});
+  }
+  len = (int)lists.size() - off;
+  cache[ck] = {off, len};
+}
+
 static void add_gate(Program& P, const sq_gate_desc& g, int n_orb) {
   auto check_idx = [&](int count, int limit) {
     SQ_CHECK(g.n_idx == count, SQ_ERR_ARG, "wrong number of indices for excitation type");
@@ -383,54 +425,66 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
   return r;
 }
 
-// In-place pair rotation for G = coef * Ga * Gb (one string, distinct indices).
-// 2-D grid: x over beta strings, y over alpha strings.
-__global__ void k_rotate(SpaceView v, Term G, const double* __restrict__ theta, int gidx, double scale, double sgn,
-                         double* __restrict__ psi) {
-  double th = sgn * scale * theta[gidx];
-  if (fabs(theta[gidx]) < 1e-28) return;
-  int ia = blockIdx.y;
-  uint32_t a = v.astr[ia];
-  int par = 0;
-  if (!apply_ops(G.a, a, par)) return;
-  int ib = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ib >= v.nsb) return;
-  uint32_t b = v.bstr[ib];
-  if (!apply_ops(G.b, b, par)) return;
-  double sigma = ((par & 1) ? -1.0 : 1.0) * (G.coef < 0 ? -1.0 : 1.0);
-  size_t D = (size_t)ia * v.nsb + ib;
-  size_t J = (size_t)v.arank[a] * v.nsb + v.brank[b];
-  double s, c;
-  sincos(th, &s, &c);
-  double pd = psi[D], pj = psi[J];
-  psi[D] = c * pd - sigma * s * pj;
-  psi[J] = sigma * s * pd + c * pj;
+// One rotation half over its spin-part lists: thread (row r, column c) owns
+// the pair of determinants (A_r, B_c) -> (image of A_r, image of B_c), with
+// A, B list entries (string, image | parity << 31) or, for an identity spin
+// part, every string.  Only touched pairs get threads; along an identity beta
+// part consecutive lanes touch consecutive amplitudes.  The rotation's cosine
+// and sine are computed once per block.
+__device__ __forceinline__ int2 rot_entry(const int2* __restrict__ lists, int off, int i) {
+  return off < 0 ? int2{i, i} : lists[off + i];
 }
 
-// Reverse-sweep step for a rotation gate on (psi, lam): partial dot
-// <lam|T psi> (before un-applying), then rotate both by -theta.
-__global__ void k_rotate_adj(SpaceView v, Term G, const double* __restrict__ theta, int gidx, double scale,
-                             double* __restrict__ psi, double* __restrict__ lam, double* __restrict__ partial) {
+__global__ void k_rotate_list(int nsb, const int2* __restrict__ lists, int aoff, int na, int boff, int nb,
+                              double csign, const double* __restrict__ theta, int gidx, double scale, double sgn,
+                              double* __restrict__ psi) {
+  __shared__ double cs[2];
+  const double t = theta[gidx];
+  if (fabs(t) < 1e-28) return;  // uniform over the grid
+  if (threadIdx.x == 0) sincos(sgn * scale * t, &cs[1], &cs[0]);
+  __syncthreads();
+  const double c = cs[0], s = cs[1];
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col >= nb) return;
+  const int2 B = rot_entry(lists, boff, col);
+  const int jb = B.y & 0x7fffffff, pb = (int)((uint32_t)B.y >> 31);
+  for (int row = blockIdx.y; row < na; row += gridDim.y) {
+    const int2 A = rot_entry(lists, aoff, row);
+    const int ja = A.y & 0x7fffffff, pa = (int)((uint32_t)A.y >> 31);
+    const double sigma = ((pa ^ pb) ? -1.0 : 1.0) * csign;
+    const size_t D = (size_t)A.x * nsb + B.x, J = (size_t)ja * nsb + jb;
+    const double pd = psi[D], pj = psi[J];
+    psi[D] = c * pd - sigma * s * pj;
+    psi[J] = sigma * s * pd + c * pj;
+  }
+}
+
+// Reverse-sweep step of one rotation half over its lists: partial dots
+// scale * <lam|T psi> (before un-applying) per block, then both vectors rotate
+// by -theta.
+__global__ void k_rotate_list_adj(int nsb, const int2* __restrict__ lists, int aoff, int na, int boff, int nb,
+                                  double csign, const double* __restrict__ theta, int gidx, double scale,
+                                  double* __restrict__ psi, double* __restrict__ lam, double* __restrict__ partial) {
   __shared__ double red[kBlk / 64];
-  double th = -scale * theta[gidx];
-  bool skip = fabs(theta[gidx]) < 1e-28;
-  int ia = blockIdx.y;
-  uint32_t a = v.astr[ia];
-  int par = 0;
+  __shared__ double cs[2];
+  const double t = theta[gidx];
+  const bool skip = fabs(t) < 1e-28;
+  if (threadIdx.x == 0) sincos(-scale * t, &cs[1], &cs[0]);
+  __syncthreads();
+  const double c = cs[0], s = cs[1];
   double contrib = 0.0;
-  bool ok = apply_ops(G.a, a, par);
-  int ib = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ok && ib < v.nsb) {
-    uint32_t b = v.bstr[ib];
-    if (apply_ops(G.b, b, par)) {
-      double sigma = ((par & 1) ? -1.0 : 1.0) * (G.coef < 0 ? -1.0 : 1.0);
-      size_t D = (size_t)ia * v.nsb + ib;
-      size_t J = (size_t)v.arank[a] * v.nsb + v.brank[b];
-      double pd = psi[D], pj = psi[J], ld = lam[D], lj = lam[J];
-      contrib = scale * sigma * (lj * pd - ld * pj);
+  const int col = blockIdx.x * blockDim.x + threadIdx.x;
+  if (col < nb) {
+    const int2 B = rot_entry(lists, boff, col);
+    const int jb = B.y & 0x7fffffff, pb = (int)((uint32_t)B.y >> 31);
+    for (int row = blockIdx.y; row < na; row += gridDim.y) {
+      const int2 A = rot_entry(lists, aoff, row);
+      const int ja = A.y & 0x7fffffff, pa = (int)((uint32_t)A.y >> 31);
+      const double sigma = ((pa ^ pb) ? -1.0 : 1.0) * csign;
+      const size_t D = (size_t)A.x * nsb + B.x, J = (size_t)ja * nsb + jb;
+      const double pd = psi[D], pj = psi[J], ld = lam[D], lj = lam[J];
+      contrib += scale * sigma * (lj * pd - ld * pj);
       if (!skip) {
-        double s, c;
-        sincos(th, &s, &c);
         psi[D] = c * pd - sigma * s * pj;
         psi[J] = sigma * s * pd + c * pj;
         lam[D] = c * ld - sigma * s * lj;
@@ -438,7 +492,7 @@ __global__ void k_rotate_adj(SpaceView v, Term G, const double* __restrict__ the
       }
     }
   }
-  double tot = block_sum(contrib, red);
+  const double tot = block_sum(contrib, red);
   if (threadIdx.x == 0) partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
 }
 
@@ -534,6 +588,35 @@ void copy_vec(const double* in, double* out, int64_t n, hipStream_t st) {
 
 int64_t nblocks2d(const Space& s) { return (int64_t)((s.nsb + kBlk - 1) / kBlk) * s.nsa; }
 
+// Rotation half h of gate g over its spin-part lists: columns over the beta
+// list, rows over the alpha list (grid.y capped, rows strided).
+static dim3 rot_grid(const int4& L) {
+  return dim3((unsigned)std::max(1, (L.w + kBlk - 1) / kBlk), (unsigned)std::max(1, std::min(L.y, 65535)));
+}
+
+static void rotate_half(const Space& s, const Program& p, int g, int h, const double* theta_dev, double scale,
+                        double sgn, double* psi, hipStream_t st) {
+  const int4 L = p.rot_lists[2 * g + h];
+  if (L.y == 0 || L.w == 0) return;
+  const Term& T = p.terms[p.gates[g].term_begin + 2 * h];
+  SQ_PROF(PC_ROTATE, st, 32.0 * (double)L.y * (double)L.w, 0.0,
+          k_rotate_list<<<rot_grid(L), kBlk, 0, st>>>(s.nsb, p.d_lists, L.x, L.y, L.z, L.w, T.coef < 0 ? -1.0 : 1.0,
+                                                      theta_dev, g, scale, sgn, psi));
+}
+
+// Reverse step of one rotation half; returns the number of partial dots.
+static int64_t rotate_half_adj(const Space& s, const Program& p, int g, int h, const double* theta_dev, double scale,
+                               double* psi, double* lam, double* partial, hipStream_t st) {
+  const int4 L = p.rot_lists[2 * g + h];
+  if (L.y == 0 || L.w == 0) return 0;
+  const Term& T = p.terms[p.gates[g].term_begin + 2 * h];
+  const dim3 grid = rot_grid(L);
+  SQ_PROF(PC_ROTATE_ADJ, st, 64.0 * (double)L.y * (double)L.w, 0.0,
+          k_rotate_list_adj<<<grid, kBlk, 0, st>>>(s.nsb, p.d_lists, L.x, L.y, L.z, L.w, T.coef < 0 ? -1.0 : 1.0,
+                                                   theta_dev, g, scale, psi, lam, partial));
+  return (int64_t)grid.x * grid.y;
+}
+
 // theta_dev points at a device theta vector; negate applies -theta (dagger).
 // work: >= 2*dim doubles (polynomial temporaries).
 void apply_gate_impl(const Space& s, const Program& p, int g, const double* theta_dev, bool negate, double* psi,
@@ -543,14 +626,11 @@ void apply_gate_impl(const Space& s, const Program& p, int g, const double* thet
   dim3 grid = grid2d(s);
   double sgn = negate ? -1.0 : 1.0;
   if (gd.kind == GC_ROT) {
-    SQ_PROF(PC_ROTATE, st, 32.0 * gd.pairs[0], 0.0,
-            k_rotate<<<grid, kBlk, 0, st>>>(v, p.terms[gd.term_begin], theta_dev, g, gd.scale, sgn, psi));
+    rotate_half(s, p, g, 0, theta_dev, gd.scale, sgn, psi, st);
   } else if (gd.kind == GC_SA_SINGLE) {
     // the reference applies the alpha rotation, then the beta one
-    SQ_PROF(PC_ROTATE, st, 32.0 * gd.pairs[0], 0.0,
-            k_rotate<<<grid, kBlk, 0, st>>>(v, p.terms[gd.term_begin], theta_dev, g, 1.0, sgn, psi));
-    SQ_PROF(PC_ROTATE, st, 32.0 * gd.pairs[1], 0.0,
-            k_rotate<<<grid, kBlk, 0, st>>>(v, p.terms[gd.term_begin + 2], theta_dev, g, 1.0, sgn, psi));
+    rotate_half(s, p, g, 0, theta_dev, 1.0, sgn, psi, st);
+    rotate_half(s, p, g, 1, theta_dev, 1.0, sgn, psi, st);
   } else {
     if (gd.term_end == gd.term_begin) return;  // identity generator
     int slot = poly_slot(gd);
@@ -603,21 +683,16 @@ void reduce_impl(const double* partial, int64_t n, double* out_dev, int slot, bo
 void reverse_step_impl(const Space& s, const Program& p, int g, const double* theta_dev, double* psi, double* lam,
                        double* grad_dev, double* work, double* partial, hipStream_t st) {
   const GateDev& gd = p.gates[g];
-  SpaceView v = view(s);
-  dim3 grid = grid2d(s);
   int64_t nb = nblocks2d(s);
   if (gd.kind == GC_ROT) {
-    SQ_PROF(PC_ROTATE_ADJ, st, 64.0 * gd.pairs[0], 0.0,
-            k_rotate_adj<<<grid, kBlk, 0, st>>>(v, p.terms[gd.term_begin], theta_dev, g, gd.scale, psi, lam, partial));
-    reduce_impl(partial, nb, grad_dev, g, false, 2.0, st);
+    const int64_t np = rotate_half_adj(s, p, g, 0, theta_dev, gd.scale, psi, lam, partial, st);
+    reduce_impl(partial, np, grad_dev, g, false, 2.0, st);
   } else if (gd.kind == GC_SA_SINGLE) {
     // un-apply beta then alpha; each half contributes its own <lam|T psi>
-    SQ_PROF(PC_ROTATE_ADJ, st, 64.0 * gd.pairs[1], 0.0,
-            k_rotate_adj<<<grid, kBlk, 0, st>>>(v, p.terms[gd.term_begin + 2], theta_dev, g, 1.0, psi, lam, partial));
-    reduce_impl(partial, nb, grad_dev, g, false, 2.0, st);
-    SQ_PROF(PC_ROTATE_ADJ, st, 64.0 * gd.pairs[0], 0.0,
-            k_rotate_adj<<<grid, kBlk, 0, st>>>(v, p.terms[gd.term_begin], theta_dev, g, 1.0, psi, lam, partial));
-    reduce_impl(partial, nb, grad_dev, g, true, 2.0, st);
+    int64_t np = rotate_half_adj(s, p, g, 1, theta_dev, 1.0, psi, lam, partial, st);
+    reduce_impl(partial, np, grad_dev, g, false, 2.0, st);
+    np = rotate_half_adj(s, p, g, 0, theta_dev, 1.0, psi, lam, partial, st);
+    reduce_impl(partial, np, grad_dev, g, true, 2.0, st);
   } else {
     if (gd.term_end == gd.term_begin) {
       reduce_impl(partial, 0, grad_dev, g, false, 0.0, st);
@@ -654,6 +729,24 @@ int sq_program_create(const sq_space* sp, const sq_gate_desc* gates, int n_gates
     P.space = &sp->s;
     try {
       for (int i = 0; i < n_gates; ++i) add_gate(P, gates[i], sp->s.n);
+      {
+        std::vector<int2> lists;
+        std::map<std::pair<const void*, std::vector<int>>, std::pair<int, int>> cache;
+        P.rot_lists.assign(2 * P.gates.size(), int4{-1, 0, -1, 0});
+        for (size_t g = 0; g < P.gates.size(); ++g) {
+          const GateDev& gd = P.gates[g];
+          if (gd.kind != GC_ROT && gd.kind != GC_SA_SINGLE) continue;
+          for (int h = 0; h < (gd.kind == GC_SA_SINGLE ? 2 : 1); ++h) {
+            const Term& T = P.terms[gd.term_begin + 2 * h];
+            int4& L = P.rot_lists[2 * g + h];
+            spin_part_list(sp->s.h_astr, T.a, lists, cache, L.x, L.y);
+            spin_part_list(sp->s.h_bstr, T.b, lists, cache, L.z, L.w);
+          }
+        }
+        SQ_HIP(hipMalloc(&P.d_lists, sizeof(int2) * std::max<size_t>(lists.size(), 1)));
+        if (!lists.empty())
+          SQ_HIP(hipMemcpy(P.d_lists, lists.data(), sizeof(int2) * lists.size(), hipMemcpyHostToDevice));
+      }
       SQ_HIP(hipMalloc(&P.d_gates, sizeof(GateDev) * std::max<size_t>(P.gates.size(), 1)));
       SQ_HIP(hipMalloc(&P.d_terms, sizeof(Term) * std::max<size_t>(P.terms.size(), 1)));
       if (!P.gates.empty())
@@ -673,6 +766,7 @@ int sq_program_destroy(sq_program* pr) {
     if (!pr) return;
     if (pr->p.d_gates) (void)hipFree(pr->p.d_gates);
     if (pr->p.d_terms) (void)hipFree(pr->p.d_terms);
+    if (pr->p.d_lists) (void)hipFree(pr->p.d_lists);
     if (pr->p.tiles) destroy_tile_plan(pr->p.tiles);
     delete pr;
   });

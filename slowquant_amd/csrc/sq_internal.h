@@ -120,7 +120,19 @@ struct Program {
   GateDev* d_gates = nullptr;
   Term* d_terms = nullptr;
   std::vector<int> pair_p, pair_q;  // orbital pair (p, p+1) of "brick" gates, -1 otherwise
+  // Rotation gates: per gate and half (2g + h) the spin-part lists
+  // {alpha offset, alpha length, beta offset, beta length} into d_lists
+  // (offset -1: the spin part is the identity, every string).
+  std::vector<int4> rot_lists;
+  int2* d_lists = nullptr;
 };
+
+// Strings a spin part applies to, as (string index, image | parity << 31)
+// entries appended to lists (shared per (strings, operator) through cache);
+// off = -1 and len = all strings for an identity part.
+void spin_part_list(const std::vector<uint32_t>& strs, const SpinOps& o, std::vector<int2>& lists,
+                    std::map<std::pair<const void*, std::vector<int>>, std::pair<int, int>>& cache, int& off,
+                    int& len);
 
 struct Hamiltonian {
   const Space* space;

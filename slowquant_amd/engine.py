@@ -56,9 +56,12 @@ class Space:
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
-        if h is not None and N._lib is not None:
-            N.lib().sq_space_destroy(h)
-            self._h = None
+        self._h = None
+        try:
+            if h is not None and N._lib is not None:
+                N.lib().sq_space_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
+            pass
 
     def work(self, what: int) -> torch.Tensor:
         if what not in self._work:
@@ -114,9 +117,12 @@ class Program:
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
-        if h is not None and N._lib is not None:
-            N.lib().sq_program_destroy(h)
-            self._h = None
+        self._h = None
+        try:
+            if h is not None and N._lib is not None:
+                N.lib().sq_program_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
+            pass
 
     def theta_tensor(self, theta) -> torch.Tensor:
         if isinstance(theta, torch.Tensor):
@@ -192,9 +198,12 @@ class Hamiltonian:
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
-        if h is not None and N._lib is not None:
-            N.lib().sq_hamiltonian_destroy(h)
-            self._h = None
+        self._h = None
+        try:
+            if h is not None and N._lib is not None:
+                N.lib().sq_hamiltonian_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
+            pass
 
     def sigma(self, psi: torch.Tensor, want_energy: bool = True):
         out = torch.empty_like(psi)
@@ -242,9 +251,12 @@ class Operator:
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
-        if h is not None and N._lib is not None:
-            N.lib().sq_operator_destroy(h)
-            self._h = None
+        self._h = None
+        try:
+            if h is not None and N._lib is not None:
+                N.lib().sq_operator_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
+            pass
 
     def apply(self, psi: torch.Tensor) -> torch.Tensor:
         out = torch.empty_like(psi)
@@ -311,9 +323,12 @@ class Batch:
 
     def __del__(self) -> None:
         h = getattr(self, "_h", None)
-        if h is not None and N._lib is not None:
-            N.lib().sq_batch_destroy(h)
-            self._h = None
+        self._h = None
+        try:
+            if h is not None and N._lib is not None:
+                N.lib().sq_batch_destroy(h)
+        except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
+            pass
 
     def set_theta(self, theta) -> None:
         th = np.ascontiguousarray(np.concatenate([np.asarray(t, dtype=np.float64).ravel() for t in theta])
