@@ -184,6 +184,11 @@ int sq_batch_destroy(sq_batch* b);
 int64_t sq_batch_num_params(const sq_batch* b);
 /* theta: host array, instances' parameters concatenated in instance order. */
 int sq_batch_set_theta(sq_batch* b, const double* theta, void* stream);
+/* Kernel shape picked by the autotuner: rotation pairs of determinants per
+ * thread in flight (1, 2 or 4) and sigma GEMM 16-column tiles per wave (1 or
+ * 2); no counterpart in the reference (its batching has no kernel shapes).
+ * Drops a captured graph. */
+int sq_batch_configure(sq_batch* b, int rot_pairs, int gemm_tiles);
 /* One launch on stream. */
 int sq_batch_run(sq_batch* b, void* stream);
 /* Same launch captured once into a hipGraph, then replayed. */

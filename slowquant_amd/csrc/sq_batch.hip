@@ -89,17 +89,6 @@ __device__ __forceinline__ void for_each_det(int nsa, int nsb, F&& f) {
     for (int ib = c0; ib < nsb; ib += cols) f(ia, ib, ia * nsb + ib);
 }
 
-// Visit every (alpha entry, beta entry) of a rotation's lists.
-template <class F>
-__device__ __forceinline__ void for_each_pair(int na, int nb, F&& f) {
-  const int cols = nb < (int)blockDim.x ? (nb > 0 ? nb : 1) : (int)blockDim.x;
-  const int rps = (int)blockDim.x / cols;
-  const int r0 = (int)threadIdx.x / cols, c0 = (int)threadIdx.x - r0 * cols;
-  if (r0 >= rps) return;
-  for (int ai = r0; ai < na; ai += rps)
-    for (int bi = c0; bi < nb; bi += cols) f(ai, bi);
-}
-
 __device__ __forceinline__ int2 list_entry(const int2* lists, int off, int i) {
   return off < 0 ? int2{i, i} : lists[off + i];
 }
@@ -115,6 +104,63 @@ __device__ double block_reduce(double v, double* red) {
   return t;  // every thread gets the sum
 }
 
+// One rotation half of the forward (REV = false) or reverse sweep over its
+// spin-part lists, U pairs of determinants per thread in flight: the loads of
+// all U pairs are issued before any of them is updated (the pairs of one
+// rotation are disjoint).  Returns this thread's part of
+// scale * <lam|T psi> (reverse sweep).
+template <int U, bool REV>
+__device__ __forceinline__ double rotate_lists(const int2* __restrict__ lists, const BatchRot& R, int nsb, double c,
+                                               double s, bool apply, double* psi, double* lam) {
+  const int na = R.na, nb = R.nb;
+  const int cols = nb < (int)blockDim.x ? (nb > 0 ? nb : 1) : (int)blockDim.x;
+  const int rps = (int)blockDim.x / cols;
+  const int r0 = (int)threadIdx.x / cols, c0 = (int)threadIdx.x - r0 * cols;
+  double dot = 0.0;
+  if (r0 >= rps) return 0.0;
+  for (int bi = c0; bi < nb; bi += cols) {
+    const int2 Bq = list_entry(lists, R.boff, bi);
+    for (int a0 = r0; a0 < na; a0 += rps * U) {
+      int x[U], J[U];
+      double sg[U], pd[U], pj[U], ld[U], lj[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const int ai = a0 + u * rps;
+        x[u] = -1;
+        if (ai < na) {
+          const int2 A = list_entry(lists, R.aoff, ai);
+          x[u] = A.x * nsb + Bq.x;
+          J[u] = (A.y & 0x7fffffff) * nsb + (Bq.y & 0x7fffffff);
+          sg[u] = (((uint32_t)(A.y ^ Bq.y) >> 31) ? -1.0 : 1.0) * R.sgn;
+          pd[u] = psi[x[u]];
+          pj[u] = psi[J[u]];
+          if (REV) {
+            ld[u] = lam[x[u]];
+            lj[u] = lam[J[u]];
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (x[u] < 0) continue;
+        if (REV) dot += R.scale * sg[u] * (lj[u] * pd[u] - ld[u] * pj[u]);
+        if (apply) {
+          psi[x[u]] = c * pd[u] - sg[u] * s * pj[u];
+          psi[J[u]] = sg[u] * s * pd[u] + c * pj[u];
+          if (REV) {
+            lam[x[u]] = c * ld[u] - sg[u] * s * lj[u];
+            lam[J[u]] = sg[u] * s * ld[u] + c * lj[u];
+          }
+        }
+      }
+    }
+  }
+  return dot;
+}
+
+// U: rotation pairs per thread in flight; CT: 16-column tiles per wave in the
+// sigma GEMM (sharing each LDS read of A).  Both are picked by the autotuner.
+template <int U, int CT>
 __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
   __shared__ double red[kBT / 64];
   extern __shared__ double sA[];
@@ -146,15 +192,7 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       if (fabs(t0) >= 1e-28) {
         double s, c;
         sincos(R.scale * t0, &s, &c);
-        for_each_pair(R.na, R.nb, [&](int ai, int bi) {
-          const int2 A = list_entry(B.lists, R.aoff, ai), Bq = list_entry(B.lists, R.boff, bi);
-          const int x = A.x * nsb + Bq.x;
-          const int J = (A.y & 0x7fffffff) * nsb + (Bq.y & 0x7fffffff);
-          const double sg = (((uint32_t)(A.y ^ Bq.y) >> 31) ? -1.0 : 1.0) * R.sgn;
-          const double pd = psi[x], pj = psi[J];
-          psi[x] = c * pd - sg * s * pj;
-          psi[J] = sg * s * pd + c * pj;
-        });
+        (void)rotate_lists<U, false>(B.lists, R, nsb, c, s, true, psi, nullptr);
       }
       __syncthreads();
     }
@@ -170,42 +208,64 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63, kq = lane >> 4, li = lane & 15;
       const int nwaves = blockDim.x >> 6;
       const int MT = S.kpad / 16;
-      for (int c0 = wave * 16; c0 < dim; c0 += nwaves * 16) {
-        const int c = c0 + li;
-        const bool ok = c < dim;
-        const int ia = ok ? c / nsb : 0, ib = ok ? c - ia * nsb : 0;
-        const double pc = ok ? psi[c] : 0.0;
-        auto dval = [&](int k) -> double {
-          if (!ok || k > S.npair) return 0.0;
-          if (k == S.npair) return pc;
-          const int32_t ea = S.aexc[k * S.nsa + ia];
-          const int32_t eb = S.bexc[k * nsb + ib];
-          if (c_bdiag[k]) return (double)(ea + eb) * pc;
+      for (int c0 = wave * 16 * CT; c0 < dim; c0 += nwaves * 16 * CT) {
+        int ccol[CT], ia[CT], ib[CT];
+        bool ok[CT];
+        double pc[CT];
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+          ccol[u] = c0 + u * 16 + li;
+          ok[u] = ccol[u] < dim;
+          ia[u] = ok[u] ? ccol[u] / nsb : 0;
+          ib[u] = ok[u] ? ccol[u] - ia[u] * nsb : 0;
+          pc[u] = ok[u] ? psi[ccol[u]] : 0.0;
+        }
+        auto dval = [&](int u, int k) -> double {
+          if (!ok[u] || k > S.npair) return 0.0;
+          if (k == S.npair) return pc[u];
+          const int32_t ea = S.aexc[k * S.nsa + ia[u]];
+          const int32_t eb = S.bexc[k * nsb + ib[u]];
+          if (c_bdiag[k]) return (double)(ea + eb) * pc[u];
           double v = 0.0;
-          if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(abs(ea) - 1) * nsb + ib];
-          if (eb) v += (eb > 0 ? 1.0 : -1.0) * psi[ia * nsb + abs(eb) - 1];
+          if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(abs(ea) - 1) * nsb + ib[u]];
+          if (eb) v += (eb > 0 ? 1.0 : -1.0) * psi[ia[u] * nsb + abs(eb) - 1];
           return v;
         };
-        double4_t acc[kBatchMT];
+        double4_t acc[CT][kBatchMT];
 #pragma unroll
-        for (int t = 0; t < kBatchMT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-        double bnext = dval(kq);
+        for (int u = 0; u < CT; ++u)
+#pragma unroll
+          for (int t = 0; t < kBatchMT; ++t) acc[u][t] = double4_t{0.0, 0.0, 0.0, 0.0};
+        double bnext[CT];
+#pragma unroll
+        for (int u = 0; u < CT; ++u) bnext[u] = dval(u, kq);
         for (int k4 = 0; k4 < S.K4; k4 += 4) {
-          const double bv = bnext;
-          if (k4 + 4 < S.K4) bnext = dval(k4 + 4 + kq);
+          double bv[CT];
+#pragma unroll
+          for (int u = 0; u < CT; ++u) {
+            bv[u] = bnext[u];
+            if (k4 + 4 < S.K4) bnext[u] = dval(u, k4 + 4 + kq);
+          }
           const double* arow = sA + (k4 + kq) * S.kpad + li;
 #pragma unroll
-          for (int t = 0; t < kBatchMT; ++t)
-            if (t < MT) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], bv, acc[t], 0, 0, 0);
+          for (int t = 0; t < kBatchMT; ++t) {
+            if (t < MT) {
+              const double a = arow[t * 16];
+#pragma unroll
+              for (int u = 0; u < CT; ++u) acc[u][t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, bv[u], acc[u][t], 0, 0, 0);
+            }
+          }
         }
-        if (ok) {
+#pragma unroll
+        for (int u = 0; u < CT; ++u) {
+          if (!ok[u]) continue;
 #pragma unroll
           for (int t = 0; t < kBatchMT; ++t) {
             if (t >= MT) continue;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
               const int m = t * 16 + kq + 4 * r;
-              if (m < S.npair) G[(size_t)m * dim + c] = acc[t][r];
+              if (m < S.npair) G[(size_t)m * dim + ccol[u]] = acc[u][t][r];
             }
           }
         }
@@ -244,21 +304,7 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
       const bool skip = fabs(t0) < 1e-28;
       double s, c;
       sincos(-R.scale * t0, &s, &c);
-      double dot = 0.0;
-      for_each_pair(R.na, R.nb, [&](int ai, int bi) {
-        const int2 A = list_entry(B.lists, R.aoff, ai), Bq = list_entry(B.lists, R.boff, bi);
-        const int x = A.x * nsb + Bq.x;
-        const int J = (A.y & 0x7fffffff) * nsb + (Bq.y & 0x7fffffff);
-        const double sg = (((uint32_t)(A.y ^ Bq.y) >> 31) ? -1.0 : 1.0) * R.sgn;
-        const double pd = psi[x], pj = psi[J], ld = lam[x], lj = lam[J];
-        dot += R.scale * sg * (lj * pd - ld * pj);
-        if (!skip) {
-          psi[x] = c * pd - sg * s * pj;
-          psi[J] = sg * s * pd + c * pj;
-          lam[x] = c * ld - sg * s * lj;
-          lam[J] = sg * s * ld + c * lj;
-        }
-      });
+      const double dot = rotate_lists<U, true>(B.lists, R, nsb, c, s, !skip, psi, lam);
       const double tot = block_reduce(dot, red);
       if (threadIdx.x == 0) B.grad[I.out0 + R.grad_idx] += 2.0 * tot;
       __syncthreads();
@@ -278,6 +324,14 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int32_t> grad_off;
+  int rot_pairs = 2;    // rotation pairs per thread in flight (1, 2, 4)
+  int gemm_tiles = 1;   // sigma GEMM column tiles per wave (1, 2)
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
   ~Batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -462,8 +516,32 @@ int sq_batch_set_theta(sq_batch* bo, const double* theta, void* stream) {
   });
 }
 
+using BatchKernel = void (*)(BatchDev);
+
+static BatchKernel batch_kernel(const Batch& B) {
+  static const BatchKernel table[3][2] = {{k_batch<1, 1>, k_batch<1, 2>},
+                                          {k_batch<2, 1>, k_batch<2, 2>},
+                                          {k_batch<4, 1>, k_batch<4, 2>}};
+  const int u = B.rot_pairs == 1 ? 0 : (B.rot_pairs == 2 ? 1 : 2);
+  return table[u][B.gemm_tiles - 1];
+}
+
 static void batch_launch(Batch& B, hipStream_t st) {
-  SQ_PROF(PC_BATCH, st, B.bytes, B.flops, k_batch<<<B.nwg, kBT, B.lds, st>>>(B.dev));
+  SQ_PROF(PC_BATCH, st, B.bytes, B.flops, batch_kernel(B)<<<B.nwg, kBT, B.lds, st>>>(B.dev));
+}
+
+/* Kernel shape: rotation pairs per thread in flight (1, 2 or 4) and sigma
+ * GEMM column tiles per wave (1 or 2); drops a captured graph. */
+int sq_batch_configure(sq_batch* bo, int rot_pairs, int gemm_tiles) {
+  return guarded([&] {
+    SQ_CHECK(bo, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(rot_pairs == 1 || rot_pairs == 2 || rot_pairs == 4, SQ_ERR_ARG, "rot_pairs must be 1, 2 or 4");
+    SQ_CHECK(gemm_tiles == 1 || gemm_tiles == 2, SQ_ERR_ARG, "gemm_tiles must be 1 or 2");
+    Batch& B = bo->b;
+    if (B.rot_pairs != rot_pairs || B.gemm_tiles != gemm_tiles) B.drop_graph();
+    B.rot_pairs = rot_pairs;
+    B.gemm_tiles = gemm_tiles;
+  });
 }
 
 int sq_batch_run(sq_batch* bo, void* stream) {
@@ -479,7 +557,7 @@ int sq_batch_run_graph(sq_batch* bo, void* stream) {
       hipStream_t cap;
       SQ_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
       SQ_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-      k_batch<<<B.nwg, kBT, B.lds, cap>>>(B.dev);
+      batch_kernel(B)<<<B.nwg, kBT, B.lds, cap>>>(B.dev);
       hipError_t e = hipGetLastError();
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);

@@ -318,6 +318,7 @@ class Batch:
         N.check(N.lib().sq_batch_create(n, sp, pr, hh, int(n_workgroups), ctypes.byref(b)))
         self._h = b
         self.n_workgroups = int(n_workgroups)
+        self.shape = (2, 1)  # the library's default kernel shape
         self.n_params = int(N.lib().sq_batch_num_params(b))
         self.offsets = np.cumsum([0] + [p.n_gates for _, p, _ in self.instances])
 
@@ -338,6 +339,12 @@ class Batch:
         self._theta_host = th  # kept alive until the asynchronous copy has run
         N.check(N.lib().sq_batch_set_theta(self._h, th.ctypes.data, _stream()))
 
+    def configure(self, rot_pairs: int = 2, gemm_tiles: int = 1) -> None:
+        """Kernel shape: rotation pairs per thread in flight (1, 2, 4) and sigma
+        GEMM column tiles per wave (1, 2)."""
+        N.check(N.lib().sq_batch_configure(self._h, int(rot_pairs), int(gemm_tiles)))
+        self.shape = (int(rot_pairs), int(gemm_tiles))
+
     def launch(self, graph: bool = False) -> None:
         fn = N.lib().sq_batch_run_graph if graph else N.lib().sq_batch_run
         N.check(fn(self._h, _stream()))
@@ -354,11 +361,15 @@ class Batch:
         return self.results()
 
 
-def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), reps: int = 3) -> Batch:
+SHAPES = tuple((u, ct) for u in (1, 2, 4) for ct in (1, 2))
+
+
+def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), shapes=SHAPES, reps: int = 3) -> Batch:
     """Build one Batch per workgroup count in ``candidates`` (capped at the
-    number of instances), time ``reps`` graph replays of each and return the
-    fastest."""
-    best, best_t = None, float("inf")
+    number of instances); for each, time ``reps`` graph replays of every
+    kernel shape in ``shapes`` ((rotation pairs in flight, GEMM column tiles
+    per wave)) and return the fastest batch, configured with its best shape."""
+    best, best_t, best_shape = None, float("inf"), None
     seen = set()
     for c in candidates:
         c = min(int(c), len(instances))
@@ -367,17 +378,23 @@ def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), reps:
         seen.add(c)
         b = Batch(instances, c)
         b.set_theta(theta)
-        b.launch(graph=True)
-        torch.cuda.synchronize()
-        t0 = torch.cuda.Event(enable_timing=True)
-        t1 = torch.cuda.Event(enable_timing=True)
-        t0.record()
-        for _ in range(reps):
+        b_best_t, b_shape = float("inf"), None
+        for shape in shapes:
+            b.configure(*shape)
             b.launch(graph=True)
-        t1.record()
-        t1.synchronize()
-        t = t0.elapsed_time(t1)
-        if t < best_t:
-            best, best_t = b, t
+            torch.cuda.synchronize()
+            t0 = torch.cuda.Event(enable_timing=True)
+            t1 = torch.cuda.Event(enable_timing=True)
+            t0.record()
+            for _ in range(reps):
+                b.launch(graph=True)
+            t1.record()
+            t1.synchronize()
+            t = t0.elapsed_time(t1)
+            if t < b_best_t:
+                b_best_t, b_shape = t, shape
+        if b_best_t < best_t:
+            best, best_t, best_shape = b, b_best_t, b_shape
+    best.configure(*best_shape)
     best.tuned_ms = best_t / reps
     return best

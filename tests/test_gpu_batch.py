@@ -69,6 +69,28 @@ def test_batch_matches_instance_loop_and_oracle():
         assert np.abs(g[k] - g_ref).max() < 1e-10
 
 
+def test_batch_kernel_shapes():
+    """Every autotuner shape (rotation pairs in flight, GEMM column tiles per
+    wave) gives the per-instance results, eagerly and through the graph."""
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, loop_energy_grad
+
+    specs = _specs()
+    inst, thetas = build_instances(specs)
+    e_loop, g_loop = loop_energy_grad(inst, thetas)
+    b = E.Batch(inst, 3)
+    for shape in E.SHAPES:
+        b.configure(*shape)
+        e, g = b.run(thetas)
+        _close(e, g, e_loop, g_loop)
+        e, g = b.run(thetas, graph=True)
+        _close(e, g, e_loop, g_loop)
+    with pytest.raises(ValueError):
+        b.configure(3, 1)
+    with pytest.raises(ValueError):
+        b.configure(1, 4)
+
+
 def test_batch_graph_replays():
     """Replays of the captured graph with new theta and eager work between
     them; every replay equals the eager per-instance result."""
