@@ -8,6 +8,8 @@
 //                    d_i psi = U_N..U_{i+1} U_i T_i U_{i-1}..U_1 psi0 (O(N^2)).
 //   reference sweep  the reference's product-rule loop (ups_wavefunction.py:1090-1141).
 
+#include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "sq_internal.h"
@@ -38,10 +40,31 @@ static GradBufs carve(const Space& s, int n_gates, double* work) {
   return b;
 }
 
-static void finish(const GradBufs& b, int n, double* energy, double* grad, hipStream_t st) {
-  if (n > 0) SQ_HIP(hipMemcpyAsync(grad, b.grad, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-  SQ_HIP(hipMemcpyAsync(energy, b.scal, sizeof(double), hipMemcpyDeviceToHost, st));
+// Device -> host copy of n doubles through a pinned staging buffer: one DMA
+// and no pageable bounce buffer; synchronizes the stream.
+static void copy_to_host_pinned(double* host, const double* dev, size_t n, hipStream_t st) {
+  static std::mutex mu;
+  static double* buf = nullptr;
+  static size_t cap = 0;
+  std::lock_guard<std::mutex> lock(mu);
+  if (n > cap) {
+    if (buf) (void)hipHostFree(buf);
+    buf = nullptr;
+    cap = 0;
+    SQ_HIP(hipHostMalloc(reinterpret_cast<void**>(&buf), sizeof(double) * n, hipHostMallocDefault));
+    cap = n;
+  }
+  SQ_HIP(hipMemcpyAsync(buf, dev, sizeof(double) * n, hipMemcpyDeviceToHost, st));
   SQ_HIP(hipStreamSynchronize(st));
+  std::memcpy(host, buf, sizeof(double) * n);
+}
+
+// Energy (scal[0]) and gradient (scal + 64) to the host in one copy.
+static void finish(const GradBufs& b, int n, double* energy, double* grad, hipStream_t st) {
+  std::vector<double> h(64 + (size_t)n);
+  copy_to_host_pinned(h.data(), b.scal, h.size(), st);
+  *energy = h[0];
+  if (n > 0) std::memcpy(grad, h.data() + 64, sizeof(double) * n);
 }
 
 // psi = U psi0 (t1, t2: per-gate work; s1: tile scratch).  deriv_gate >= 0
@@ -90,8 +113,8 @@ void adjoint_reverse(const Space& s, const Program& p, const double* theta, doub
   } else {
     for (int g = n - 1; g >= 0; --g) reverse_step_impl(s, p, g, theta, psi, lam, b.grad, b.t1, b.partial, st);
   }
-  if (n > 0) SQ_HIP(hipMemcpyAsync(grad, b.grad, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-  SQ_HIP(hipStreamSynchronize(st));
+  if (n > 0) copy_to_host_pinned(grad, b.grad, (size_t)n, st);
+  else SQ_HIP(hipStreamSynchronize(st));
 }
 
 void energy_grad_plain(const Space& s, const Program& p, const Hamiltonian& H, const double* theta,

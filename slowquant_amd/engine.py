@@ -127,11 +127,26 @@ class Program:
     def theta_tensor(self, theta) -> torch.Tensor:
         if isinstance(theta, torch.Tensor):
             t = theta.detach().to(device="cuda", dtype=torch.float64)
-        else:
-            t = torch.as_tensor(np.asarray(theta, dtype=np.float64)).to("cuda")
-        if t.numel() != self.n_gates:
-            raise ValueError(f"Expected {self.n_gates} theta values got {t.numel()}")
-        return t.contiguous()
+            if t.numel() != self.n_gates:
+                raise ValueError(f"Expected {self.n_gates} theta values got {t.numel()}")
+            return t.contiguous()
+        arr = np.asarray(theta, dtype=np.float64).ravel()
+        if arr.size != self.n_gates:
+            raise ValueError(f"Expected {self.n_gates} theta values got {arr.size}")
+        # host values go through one pinned staging buffer (asynchronous DMA on
+        # the engine's stream); the previous upload must have left it first
+        if getattr(self, "_th_host", None) is None:
+            n = max(self.n_gates, 1)
+            self._th_host = torch.empty(n, dtype=torch.float64, pin_memory=True)
+            self._th_dev = torch.empty(n, dtype=torch.float64, device="cuda")
+            self._th_done = torch.cuda.Event()
+            self._th_done.record()
+        self._th_done.synchronize()
+        self._th_host.numpy()[: arr.size] = arr
+        dev = self._th_dev[: arr.size]
+        dev.copy_(self._th_host[: arr.size], non_blocking=True)
+        self._th_done.record()
+        return dev
 
     def state_prep(self, theta, psi_in: torch.Tensor, dagger: bool = False, out: torch.Tensor | None = None):
         th = self.theta_tensor(theta)
