@@ -263,18 +263,21 @@ def test_rdm_parity(n, na, nb):
 
 
 @pytest.mark.parametrize(
-    "n,na,opts",
+    "n,na,nb,opts",
     [
-        (6, 3, {"n_layers": 2, "do_tups": True}),
-        (5, 2, {"n_layers": 2, "do_tups": True}),
-        (8, 4, {"n_layers": 2, "do_qnp": True}),
-        (10, 5, {"n_layers": 1, "do_tups": True, "skip_last_singles": True}),
-        (14, 7, {"n_layers": 1, "do_tups": True}),
+        (6, 3, 3, {"n_layers": 2, "do_tups": True}),
+        (5, 2, 2, {"n_layers": 2, "do_tups": True}),
+        (8, 4, 4, {"n_layers": 2, "do_qnp": True}),
+        (8, 3, 5, {"n_layers": 2, "do_tups": True}),
+        (9, 5, 3, {"n_layers": 1, "do_qnp": True}),
+        (10, 5, 5, {"n_layers": 1, "do_tups": True, "skip_last_singles": True}),
+        (14, 7, 7, {"n_layers": 1, "do_tups": True}),
     ],
 )
-def test_tiled_stages_match_per_gate_kernels(n, na, opts, monkeypatch):
+def test_tiled_stages_match_per_gate_kernels(n, na, nb, opts, monkeypatch):
     """Fused brick-column passes (default) against the per-gate rotation kernels
-    (SQ_TILES=0) for state preparation, dagger and all three gradients."""
+    (SQ_TILES=0) for state preparation, dagger and all three gradients,
+    including spaces with unequal alpha and beta electron counts."""
     from slowquant_amd.synthetic import synthetic_problem
     from slowquant_amd.util import UpsStructure
 
@@ -282,7 +285,7 @@ def test_tiled_stages_match_per_gate_kernels(n, na, opts, monkeypatch):
     lay = UpsStructure()
     lay.create_tiled(n, dict(opts))
     h, g, theta = synthetic_problem(n, n, lay.n_params, 0.2, 5)
-    sp = E.Space(n, na, na)
+    sp = E.Space(n, na, nb)
     tiled = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
     monkeypatch.setenv("SQ_TILES", "0")
     plain_prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
@@ -300,3 +303,22 @@ def test_tiled_stages_match_per_gate_kernels(n, na, opts, monkeypatch):
             e1, g1 = tiled.energy_grad(H, theta, psi0, method)
             assert abs(e1 - e0) < 1e-10, method
             assert np.abs(g1 - g0).max() < 1e-10, method
+
+
+def test_back_to_back_theta_uploads():
+    """Host theta goes through one pinned staging buffer per program: calls
+    issued without a host synchronization in between must each see their own
+    theta."""
+    torch = _torch()
+    E = _engine()
+    lay = _tups(6, 2)
+    sp = E.Space(6, 3, 3)
+    prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    psi0 = E.hf_state(sp)
+    rng = np.random.default_rng(21)
+    thetas = [rng.normal(0, 0.3, lay.n_params) for _ in range(4)]
+    outs = [prog.state_prep(th, psi0) for th in thetas]  # no synchronization between calls
+    torch.cuda.synchronize()
+    for th, out in zip(thetas, outs):
+        ref = prog.state_prep(torch.tensor(th, dtype=torch.float64, device="cuda"), psi0)
+        assert torch.equal(out, ref)
