@@ -196,11 +196,6 @@ void k_gemm_fused(const double* __restrict__ At, const double* __restrict__ psi,
   constexpr int W = MT * 16;
   const int K8 = (K4 + 7) & ~7;
   const int m0 = blockIdx.y * W;
-  for (int i = threadIdx.x; i < K8 * W; i += blockDim.x) {
-    int k = i / W, m = m0 + i % W;
-    sA[i] = (m < kpad && k < K4) ? At[(size_t)k * kpad + m] : 0.0;
-  }
-  __syncthreads();
   const int64_t dim = (int64_t)nsa * nsb;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t c = ((int64_t)blockIdx.x * (blockDim.x >> 6) + wave) * 16 + (lane & 15);
@@ -226,6 +221,12 @@ void k_gemm_fused(const double* __restrict__ At, const double* __restrict__ psi,
   int ca1, cb1;
   ta0 = gta[ta_i + 2 * sa];
   tb0 = gtb[tb_i + 2 * sb];
+  // the At slice goes to LDS while the operand prologue loads are in flight
+  for (int i = threadIdx.x; i < K8 * W; i += blockDim.x) {
+    int k = i / W, m = m0 + i % W;
+    sA[i] = (m < kpad && k < K4) ? At[(size_t)k * kpad + m] : 0.0;
+  }
+  __syncthreads();
   const double* arow = sA + (size_t)kq * W + (lane & 15);
   const int nsteps = K8 / 4;
   for (int s = 0; s < nsteps; s += 2) {
