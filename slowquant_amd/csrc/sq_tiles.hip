@@ -107,10 +107,11 @@ struct TileRec {
 // columns with beta pair j empty (cnt[1][j]), then doubly occupied
 // (cnt[2][j]).  Entry = column within the run (10 bits) | hypercube bit of
 // the pair << 10.  Columns holding the upper member of a single beta pair are
-// reached from their lower member and listed nowhere.
+// reached from their lower member and listed nowhere.  The counts are 32-bit
+// so that the tile kernels read them with uniform (scalar) loads.
 struct ColRun {
   int32_t c0, c1, toff, pad;
-  uint16_t cnt[3][8];
+  int32_t cnt[3][8];
 };
 
 struct StageDev {
@@ -451,7 +452,7 @@ static void finalize_stage(const Space& sp, Stage& S) {
         else if (kind == 1u) lists[2].push_back(e);
       }
       for (int t = 0; t < 3; ++t) {
-        r.cnt[t][j] = (uint16_t)lists[t].size();
+        r.cnt[t][j] = (int32_t)lists[t].size();
         ctab.insert(ctab.end(), lists[t].begin(), lists[t].end());
       }
       ctab.resize((size_t)r.toff + (size_t)(j + 1) * W, 0);
@@ -921,7 +922,7 @@ __device__ __forceinline__ int ins0(int rr, int abit) {
 
 // o = M v (TR: M^T v), M row-major 4x4.
 template <bool TR>
-__device__ __forceinline__ void mat4(const double* __restrict__ M, const double (&v)[4], double (&o)[4]) {
+__device__ __forceinline__ void mat4(const double (&M)[16], const double (&v)[4], double (&o)[4]) {
 #pragma unroll
   for (int a = 0; a < 4; ++a) {
     double acc = (TR ? M[a] : M[4 * a]) * v[0];
@@ -935,6 +936,9 @@ __device__ __forceinline__ void mat4(const double* __restrict__ M, const double 
 // run's column lists, so every work item is one orbit and a wave runs one
 // orbit type except where two lists meet.  Orbit members: 0 = (alpha lower,
 // beta lower) at m0, 1 = alpha upper (+ dr), 2 = beta upper (+ dc), 3 = both.
+// The pair's matrix values are read into registers before the orbit loops: a
+// per-lane choice between two matrices inside the loop would otherwise turn
+// uniform loads into per-item vector loads.
 template <int NT, bool TR>
 __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, const double* __restrict__ Mp, int j) {
   const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
@@ -947,6 +951,14 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
     const int Rh = t.R >> 1, dr = W << abit, n2 = n2e + n2d;
     const int nA = Rh * n4, nT = nA + Rh * n2;
     const float iA = 1.0f / (float)max(n4, 1), iB = 1.0f / (float)max(n2, 1);
+    double M[16], be[4], bd[4];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) M[q] = Mp[q];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      be[q] = Mp[16 + q];
+      bd[q] = Mp[20 + q];
+    }
     for (int i = threadIdx.x; i < nT; i += NT) {
       if (i < nA) {
         int rr, k;
@@ -956,7 +968,7 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
         const int m2 = m0 + (1 << (e >> 10));
         const double v[4] = {X[m0], X[m0 + dr], X[m2], X[m2 + dr]};
         double o[4];
-        mat4<TR>(Mp, v, o);
+        mat4<TR>(M, v, o);
         X[m0] = o[0];
         X[m0 + dr] = o[1];
         X[m2] = o[2];
@@ -967,15 +979,17 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
         const uint32_t e = tab[n4 + k];
         const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
         const bool dbl = k >= n2e;  // beta pair doubly occupied
-        const double b0 = dbl ? Mp[20] : Mp[16], b1 = dbl ? Mp[21] : Mp[17];
-        const double b2 = dbl ? Mp[22] : Mp[18], b3 = dbl ? Mp[23] : Mp[19];
+        const double b0 = dbl ? bd[0] : be[0], b1 = dbl ? bd[1] : be[1];
+        const double b2 = dbl ? bd[2] : be[2], b3 = dbl ? bd[3] : be[3];
         const double v0 = X[m0], v1 = X[m0 + dr];
         X[m0] = b0 * v0 + (TR ? b2 : b1) * v1;
         X[m0 + dr] = (TR ? b1 : b2) * v0 + b3 * v1;
       }
     }
   } else {  // alpha pair empty or double: 2-member beta orbits; other amplitudes stay
-    const double* B = Mp + ((fa & 3u) == 0u ? 24 : 28);
+    const double* Bp = Mp + ((fa & 3u) == 0u ? 24 : 28);
+    const double B[4] = {Bp[0], Bp[1], Bp[2], Bp[3]};
+    const bool deriv = Mp[kPairVals] == 0.0;
     const int nA = t.R * n4;
     const float iA = 1.0f / (float)max(n4, 1);
     for (int i = threadIdx.x; i < nA; i += NT) {
@@ -988,7 +1002,7 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
       X[m0] = B[0] * v0 + (TR ? B[2] : B[1]) * v1;
       X[m2] = (TR ? B[1] : B[2]) * v0 + B[3] * v1;
     }
-    if (Mp[kPairVals] == 0.0) {  // derivative pair: T_g annihilates the amplitudes no gate moves
+    if (deriv) {  // derivative pair: T_g annihilates the amplitudes no gate moves
       const int n2 = n2e + n2d, nB = t.R * n2;
       const float iB = 1.0f / (float)max(n2, 1);
       for (int i = threadIdx.x; i < nB; i += NT) {
@@ -1100,6 +1114,14 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
         const int Rh = t.R >> 1, dr = W << abit, n2 = n2e + n2d;
         const int nA = Rh * n4, nT = nA + Rh * n2;
         const float iA = 1.0f / (float)max(n4, 1), iB = 1.0f / (float)max(n2, 1);
+        double M[16], be[4], bd[4];  // pair matrix in registers (see fwd_pair)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) M[q] = Mp[q];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          be[q] = Mp[16 + q];
+          bd[q] = Mp[20 + q];
+        }
         for (int i = threadIdx.x; i < nT; i += NT) {
           if (i < nA) {
             int rr, k;
@@ -1117,8 +1139,8 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
             Q[5] += w[2] * v[3] - w[3] * v[2];
             if (wb) {
               double ov[4], ow[4];
-              mat4<true>(Mp, v, ov);
-              mat4<true>(Mp, w, ow);
+              mat4<true>(M, v, ov);
+              mat4<true>(M, w, ow);
               X[m0] = ov[0];
               X[m0 + dr] = ov[1];
               X[m2] = ov[2];
@@ -1139,8 +1161,8 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
             Q[6] += dbl ? 0.0 : x;
             Q[7] += dbl ? x : 0.0;
             if (wb) {
-              const double b0 = dbl ? Mp[20] : Mp[16], b1 = dbl ? Mp[21] : Mp[17];
-              const double b2 = dbl ? Mp[22] : Mp[18], b3 = dbl ? Mp[23] : Mp[19];
+              const double b0 = dbl ? bd[0] : be[0], b1 = dbl ? bd[1] : be[1];
+              const double b2 = dbl ? bd[2] : be[2], b3 = dbl ? bd[3] : be[3];
               X[m0] = b0 * v0 + b2 * v1;
               X[m0 + dr] = b1 * v0 + b3 * v1;
               L[m0] = b0 * w0 + b2 * w1;
@@ -1161,7 +1183,8 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
           red[j][q + 1][wave] = w2[1];
         }
       } else {
-        const double* B = Mp + ((fa & 3u) == 0u ? 24 : 28);
+        const double* Bp = Mp + ((fa & 3u) == 0u ? 24 : 28);
+        const double B[4] = {Bp[0], Bp[1], Bp[2], Bp[3]};
         const int nA = t.R * n4;
         const float iA = 1.0f / (float)max(n4, 1);
         for (int i = threadIdx.x; i < nA; i += NT) {
