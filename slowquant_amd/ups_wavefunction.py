@@ -243,5 +243,55 @@ class WaveFunctionUPS(WaveFunctionBase):
         return self._prog().energy_grad(self._hamiltonian(), np.asarray(th, dtype=np.float64), self._csf_dev(),
                                         method or self.gradient_method)
 
-    def _calc_energy_rotosolve_optimization(self, parameters, theta_diffs, theta_idx):
-        raise NotImplementedError("RotoSolve is not provided by the MI355X engine")
+    def _extra_options(self, optimizer_name: str) -> dict | None:
+        if optimizer_name.lower() != "rotosolve":
+            return None
+        return {
+            "R": self.ups_layout.grad_param_R,
+            "param_names": self.ups_layout.param_names,
+            "f_rotosolve_optimized": self._calc_energy_rotosolve_optimization,
+        }
+
+    def _calc_energy_rotosolve_optimization(self, parameters, theta_diffs, theta_idx) -> list[float]:
+        """Energies with theta[theta_idx] replaced by each of theta_diffs
+        (ups_wavefunction.py:1144-1194, which propagates the shifted states as
+        one batch).  All shifted instances run in one persistent batched
+        launch when the layout and reference allow it (rotation gates, HF
+        reference, at most 12 active orbitals), else one engine energy per
+        shift."""
+        th = np.asarray(parameters, dtype=np.float64)
+        thetas = np.repeat(th[None, :], len(theta_diffs), axis=0)
+        thetas[:, theta_idx] = np.asarray(theta_diffs, dtype=np.float64)
+        energies = self._shifted_energies(thetas)
+        self.num_energy_evals += len(energies)
+        return [float(e) for e in energies]
+
+    def _shifted_batch(self, n: int):
+        """Cached batch of n instances of this wave function's program and
+        Hamiltonian, or None where the batched kernel does not apply."""
+        from . import engine
+
+        if self._pp:
+            return None
+        H, prog = self._hamiltonian(), self._prog()
+        cache = getattr(self, "_roto_batches", None)
+        if cache is None or cache.get("key") != (id(H), id(prog)):
+            cache = {"key": (id(H), id(prog)), "H": H}
+            self._roto_batches = cache
+        if n not in cache:
+            try:
+                cache[n] = engine.Batch([(self._space(), prog, H)] * n)
+            except (ValueError, RuntimeError):
+                cache[n] = None
+        return cache[n]
+
+    def _shifted_energies(self, thetas: np.ndarray) -> np.ndarray:
+        batch = self._shifted_batch(len(thetas))
+        if batch is not None:
+            e, _ = batch.run(list(thetas))
+            return e
+        H, prog, csf = self._hamiltonian(), self._prog(), self._csf_dev()
+        out = np.zeros(len(thetas))
+        for j, t in enumerate(thetas):
+            _, out[j] = H.sigma(prog.state_prep(t, csf))
+        return out

@@ -180,6 +180,11 @@ class WaveFunctionBase:
             print(f"### Number kappa: {len(self.kappa)}")
         print(f"### Number theta: {len(self._thetas)}")
 
+    def _extra_options(self, optimizer_name: str) -> dict | None:
+        """Optimizer options beyond fun/jac; RotoSolve needs them (the UPS class
+        provides them, ups_wavefunction.py:853-861, 997-1005)."""
+        return None
+
     def _reset_kappa_after_two_step(self) -> None:
         for i in range(len(self.kappa)):
             self._kappa[i] = 0.0
@@ -206,7 +211,7 @@ class WaveFunctionBase:
             )
             self._old_opt_parameters = np.zeros_like(self.thetas) + 10**20
             self._E_opt_old = 0.0
-            res = opt.minimize(self.thetas)
+            res = opt.minimize(self.thetas, extra_options=self._extra_options(optimizer_name))
             self.thetas = res.x.tolist()
             if orbital_optimization and len(self.kappa) != 0:
                 if not is_silent_subiterations:
@@ -254,7 +259,7 @@ class WaveFunctionBase:
                          energy_eval_callback=lambda: self.num_energy_evals)
         self._old_opt_parameters = np.zeros_like(parameters) + 10**20
         self._E_opt_old = 0.0
-        res = opt.minimize(parameters)
+        res = opt.minimize(parameters, extra_options=self._extra_options(optimizer_name))
         if orbital_optimization:
             self.thetas = res.x[len(self.kappa):].tolist()
             for i in range(len(self.kappa)):

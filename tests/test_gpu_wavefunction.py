@@ -145,3 +145,72 @@ def test_safuccsd_8o_energy_against_oracle():
     assert abs(e_a - e_ref) < 1e-10
     assert np.abs(g_a - g_p).max() < 1e-10
     assert np.abs(g_a - g_r).max() < 1e-10
+
+
+def test_rotosolve_shifted_energies_against_oracle():
+    """Shifted energies for RotoSolve (ups_wavefunction.py:1144-1194): the
+    batched launch and the per-shift engine loop both match the oracle, and
+    the energy as a function of one parameter is the order-R trigonometric
+    polynomial rebuilt from them, R = 4 for sa_single and 2 for double
+    (util.py:694-745), at an angle that is not a sample point (R bounds the
+    order: a gate can leave the energy flat, e.g. a rotation of two doubly
+    occupied orbitals of the HF determinant)."""
+    from oracle.apply import expectation_value
+    from oracle.ups import construct_ups_state
+    from slowquant_amd import optimizers as O
+
+    _sq, wf = _h4_wf(layers=2)
+    theta = np.random.default_rng(4).normal(0.0, 0.3, wf.ups_layout.n_params)
+    H, ci = _oracle_parts(wf)
+    names = wf.ups_layout.param_names
+    Rs = [wf.ups_layout.grad_param_R[n] for n in names]
+    picks = [Rs.index(4), Rs.index(2), len(theta) - 1]
+
+    def oracle_e(t):
+        psi = construct_ups_state(wf.csf_coeffs, ci, t, wf.ups_layout)
+        return expectation_value(psi, [H], psi, ci, do_folding=False)
+
+    assert wf._shifted_batch(2 * Rs[picks[0]] + 1) is not None  # HF reference, rotation gates
+    for idx in picks:
+        R = Rs[idx]
+        diffs = O.shift_points(R)
+        ref = []
+        for d in diffs:
+            t = theta.copy()
+            t[idx] = d
+            ref.append(oracle_e(t))
+        n0 = wf.num_energy_evals
+        e = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, idx)
+        assert wf.num_energy_evals == n0 + len(diffs)
+        assert np.abs(np.array(e) - ref).max() < 1e-10, idx
+        batched = wf._shifted_batch
+        wf._shifted_batch = lambda n: None  # the per-shift engine path
+        try:
+            e_loop = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, idx)
+        finally:
+            wf._shifted_batch = batched
+        assert np.abs(np.array(e_loop) - ref).max() < 1e-10, idx
+        assert abs(O.reconstructed_f([theta[idx]], e, R)[0] - oracle_e(theta)) < 1e-9, idx
+
+
+def test_rotosolve_optimisation():
+    """RotoSolve through run_wf_optimization_2step: H2 (2e,2o) tUPS reaches the
+    lowest eigenvalue of the oracle's Hamiltonian matrix on the CI space; H4
+    tUPS ends at a stationary point (engine gradient below 1e-6)."""
+    from oracle.apply import operator_matrix
+
+    _sq, wf = _h4_wf(layers=1)
+    wf.run_wf_optimization_2step("RotoSolve", False, tol=1e-10, is_silent_subiterations=True)
+    _, g = wf.energy_and_gradient()
+    assert np.abs(g).max() < 1e-6
+    assert wf.energy_elec < wf._calc_energy_optimization(np.zeros(wf.ups_layout.n_params).tolist(), True, False)
+
+    from slowquant_amd.hchain import linear_h_chain
+    from slowquant_amd.ups_wavefunction import WaveFunctionUPS
+
+    sq = linear_h_chain(2, 0.74)
+    wf2 = WaveFunctionUPS((2, 2), sq.hartree_fock.mo_coeff, sq, "tUPS", {"n_layers": 1})
+    wf2.run_wf_optimization_2step("RotoSolve", False, tol=1e-12, is_silent_subiterations=True)
+    H, ci = _oracle_parts(wf2)
+    e_fci = np.linalg.eigvalsh(operator_matrix(H, ci)).min()
+    assert abs(wf2.energy_elec - e_fci) < 1e-9
