@@ -117,7 +117,7 @@ def rotation_profile(sp, lay, prog, theta, psi0):
     tf = fu.get("tile_forward", {})
     return {
         "per_gate_kernels": {
-            "kernel": "k_rotate",
+            "kernel": "k_rotate_list",
             "launches_per_sweep": rot.get("launches_per_call"),
             "avg_us": rot.get("avg_us"),
             "achieved_GBs": rot.get("achieved_GBs"),
@@ -262,6 +262,91 @@ def bench_batch(args) -> dict:
     }
 
 
+def bench_tups14(args) -> dict:
+    """tUPS (14e,14o), 6 layers, CI dimension 11,778,624: one step = energy and
+    adjoint gradient, then the state and its 1-/2-RDMs.  Reports per-kernel
+    HBM and FP64-MFMA fractions and checks at full size: the energy from the
+    RDMs equals the engine energy, tr(Gamma1) = N, and the adjoint gradient
+    matches central differences (h = 1e-5) on three parameters."""
+    from slowquant_amd import engine as E
+    from slowquant_amd import profiling
+    from slowquant_amd.density_matrix import get_electronic_energy
+    from slowquant_amd.synthetic import synthetic_problem
+    from slowquant_amd.util import UpsStructure
+
+    n, layers, rank, sig = 14, 6, 28, 0.03
+    lay = UpsStructure()
+    lay.create_tiled(n, {"n_layers": layers, "do_tups": True})
+    h, g, theta = synthetic_problem(n, rank, lay.n_params, sig, SEED)
+    sp = E.Space(n, n // 2, n // 2)
+    prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    H = E.Hamiltonian(sp, h, g)
+    psi0 = E.hf_state(sp, 0)
+    th = np.asarray(theta)
+
+    def grad_part():
+        return prog.energy_grad(H, th, psi0, "adjoint")
+
+    def rdm_part():
+        return E.rdm12(sp, prog.state_prep(th, psi0))
+
+    e, gr = grad_part()
+    r1, r2 = rdm_part()
+    e_rdm = get_electronic_energy(h, g, 0, n, r1, r2)
+    fd = []
+    for i in (0, len(th) // 2, len(th) - 1):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += 1e-5
+        tm[i] -= 1e-5
+        fd.append(abs((prog.energy_grad(H, tp, psi0, "adjoint")[0] - prog.energy_grad(H, tm, psi0, "adjoint")[0])
+                      / 2e-5 - gr[i]))
+    t_grad = time_steps(grad_part, args.steps, args.warmup)
+    t_rdm = time_steps(rdm_part, max(1, args.steps // 2), 1)
+    kg = profiling.profile(grad_part, 2)
+    kr = profiling.profile(rdm_part, 2)
+
+    def roof(k, name):
+        if name not in k:
+            return None
+        r = k[name]
+        if name in profiling.MFMA_CLASSES:
+            return {"bound": "mfma", "achieved": r["achieved_TFLOPs"], "peak": FP64_MFMA_PEAK_TFLOPS,
+                    "unit": "TFLOP/s", "frac": r["achieved_TFLOPs"] / FP64_MFMA_PEAK_TFLOPS, "avg_us": r["avg_us"],
+                    "launches_per_step": r["launches_per_call"]}
+        return {"bound": "hbm", "achieved": r["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": r["achieved_GBs"] / HBM_PEAK_GBS, "avg_us": r["avg_us"],
+                "launches_per_step": r["launches_per_call"]}
+
+    t_step = t_grad + t_rdm
+    return {
+        "metric": "energy+gradient+RDM evals/s, tUPS (14e,14o) 6 layers FP64",
+        "value": 1.0 / t_step,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"tUPS (14e,14o) {layers} layers, {lay.n_params} params, CI dim {sp.dim}, synthetic "
+                               f"h/g rank {rank}, theta~N(0,{sig}^2) seed {SEED}; energy + adjoint gradient + 1-/2-RDMs",
+                   "n_orb": n, "n_layers": layers, "ci_dim": sp.dim, "n_params": lay.n_params},
+        "ms_energy_gradient": 1e3 * t_grad,
+        "ms_state_and_rdms": 1e3 * t_rdm,
+        "energy": e,
+        "checks": {"energy_minus_energy_from_rdms": float(e - e_rdm), "trace_rdm1": float(np.trace(r1)),
+                   "max_grad_minus_central_difference": float(max(fd))},
+        "roofline": {name: roof(k, name) for k, name in ((kg, "sigma_gemm_mfma"), (kg, "sigma_gather"),
+                                                           (kg, "tile_forward"), (kg, "tile_reverse"),
+                                                           (kr, "d_build"), (kr, "rdm_gram_mfma"))},
+        "kernels_energy_gradient": kg,
+        "kernels_state_and_rdms": kr,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -270,8 +355,9 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", choices=("tups12", "batch512"), default="tups12",
-                    help="tups12: the headline (12e,12o) tUPS metric; batch512: 512 batched instances")
+    ap.add_argument("--workload", choices=("tups12", "batch512", "tups14"), default="tups12",
+                    help="tups12: the headline (12e,12o) tUPS metric; batch512: 512 batched instances; "
+                         "tups14: (14e,14o) 6-layer energy, gradient and RDMs")
     ap.add_argument("--instances", type=int, default=512)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -285,6 +371,9 @@ def main() -> int:
 
     if args.workload == "batch512":
         print(json.dumps(bench_batch(args)), flush=True)
+        return 0
+    if args.workload == "tups14":
+        print(json.dumps(bench_tups14(args)), flush=True)
         return 0
     lay, h, g, theta, sp, prog, H, psi0 = build_workload()
     th = np.asarray(theta)

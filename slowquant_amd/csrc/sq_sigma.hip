@@ -510,14 +510,24 @@ __global__ void k_dbuild_full(int n, int nsa, int nsb, int nrow_pad, const int32
   for (int r = n * n + 1; r < nrow_pad; ++r) D[(size_t)r * dim + I] = 0.0;
 }
 
-// Partial M over one chunk of columns: WAVES waves per block; wave w owns
-// upper-triangle output tiles (ti <= tj) w, w+WAVES, ... of the RT x RT grid.
+// Partial M = D D^T over one chunk of columns (split over the column axis):
+// WAVES waves per block; wave w owns the upper-triangle 16x16 output tiles
+// (ti <= tj) w, w + WAVES, ... of the RT x RT grid.  Columns are staged KS at
+// a time in LDS, transposed (sD[k][row], row stride LD + 1), from loads of KS
+// consecutive columns per row (one 128-byte line); the next slice is loaded
+// into registers while the MFMAs run on the current one, so every MFMA
+// operand is one LDS read and D is read from HBM once.
 template <int RT, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(const double* __restrict__ D, int64_t dim, int64_t chunk,
                                                           double* __restrict__ partial) {
   typedef double double4_t __attribute__((ext_vector_type(4)));
+  constexpr int LD = RT * 16, LDP = LD + 1;
+  constexpr int KS = 16;
+  constexpr int NTH = WAVES * 64;
+  constexpr int NLOAD = (LD * KS + NTH - 1) / NTH;
   constexpr int NT = RT * (RT + 1) / 2;
   constexpr int PER = (NT + WAVES - 1) / WAVES;
+  __shared__ double sD[KS * LDP];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int kq = lane >> 4, li = lane & 15;
   int tile_i[PER], tile_j[PER];
@@ -536,25 +546,39 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(const double* __restri
   for (int t = 0; t < PER; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
   const int64_t c0 = (int64_t)blockIdx.x * chunk;
   const int64_t c1 = c0 + chunk < dim ? c0 + chunk : dim;
-  for (int64_t c = c0; c < c1; c += 4) {
-    const int64_t col = c + kq;
-    const bool ok = col < c1;
-    double frag[RT];
+  double pre[NLOAD];
+  auto load = [&](int64_t k0) {
 #pragma unroll
-    for (int r = 0; r < RT; ++r) frag[r] = ok ? D[(size_t)(r * 16 + li) * dim + col] : 0.0;
+    for (int u = 0; u < NLOAD; ++u) {
+      const int i = threadIdx.x + u * NTH;
+      const int r = i / KS, k = i % KS;
+      const int64_t col = k0 + k;
+      pre[u] = (i < LD * KS && col < c1) ? D[(size_t)r * dim + col] : 0.0;
+    }
+  };
+  load(c0);
+  for (int64_t k0 = c0; k0 < c1; k0 += KS) {
 #pragma unroll
-    for (int t = 0; t < PER; ++t) {
-      if (wave + WAVES * t < NT) {
-        // A[i][k] = D[ti*16+i][c+k], B[k][j] = D[tj*16+j][c+k]
-        double a = 0.0, b = 0.0;
+    for (int u = 0; u < NLOAD; ++u) {
+      const int i = threadIdx.x + u * NTH;
+      if (i < LD * KS) sD[(i % KS) * LDP + i / KS] = pre[u];
+    }
+    __syncthreads();
+    if (k0 + KS < c1) load(k0 + KS);
 #pragma unroll
-        for (int r = 0; r < RT; ++r) {
-          if (r == tile_i[t]) a = frag[r];
-          if (r == tile_j[t]) b = frag[r];
+    for (int kk = 0; kk < KS; kk += 4) {
+      const double* row = sD + (kk + kq) * LDP + li;
+#pragma unroll
+      for (int t = 0; t < PER; ++t) {
+        if (wave + WAVES * t < NT) {
+          // A[i][k] = D[ti*16+i][k0+kk+k], B[k][j] = D[tj*16+j][k0+kk+k]
+          const double a = row[tile_i[t] * 16];
+          const double b = row[tile_j[t] * 16];
+          acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
         }
-        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
       }
     }
+    __syncthreads();  // the next slice overwrites sD
   }
   const int ld = RT * 16;
   double* out = partial + (size_t)blockIdx.x * ld * ld;

@@ -246,7 +246,7 @@ def test_fucc_sa_doubles_parity():
         assert np.abs(grad - g_ref).max() < 1e-10, method
 
 
-@pytest.mark.parametrize("n,na,nb", [(4, 2, 2), (6, 3, 3), (5, 2, 3)])
+@pytest.mark.parametrize("n,na,nb", [(4, 2, 2), (6, 3, 3), (5, 2, 3), (8, 4, 4), (9, 4, 4), (10, 5, 4)])
 def test_rdm_parity(n, na, nb):
     from oracle.ci import get_indexing
     from oracle.rdm import rdm1, rdm2
