@@ -472,53 +472,21 @@ void dot_dev(const double* x, const double* y, int64_t n, double* out_dev, int s
   reduce_impl(partial, nb, out_dev, slot, false, factor, st);
 }
 
-// ---------------------------------------------------------------------------
-// RDMs.  Rows of Dfull: x = p*n + q holds (E_pq psi), row n*n holds psi, the
-// remaining padding rows are zero.  M = Dfull Dfull^T is accumulated with
-// v_mfma_f64_16x16x4f64 over determinant chunks (one partial per chunk,
-// summed in a fixed order), then
-//   rdm1_pq   = M[p*n+q][n*n]                  (= <E_pq psi|psi>)
-//   rdm2_pqrs = M[q*n+p][r*n+s] - delta_qr rdm1_ps   (<psi|E_pq E_rs|psi> - ...)
-__global__ void k_dbuild_full(int n, int nsa, int nsb, int nrow_pad, const int32_t* __restrict__ afull,
-                              const int32_t* __restrict__ bfull, const double* __restrict__ psi,
-                              double* __restrict__ D) {
-  int ia = blockIdx.y;
-  int ib = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ib >= nsb) return;
-  const size_t dim = (size_t)nsa * nsb;
-  const size_t I = (size_t)ia * nsb + ib;
-  const double* row = psi + (size_t)ia * nsb;
-  const double pI = psi[I];
-  for (int p = 0; p < n; ++p) {
-    for (int q = 0; q < n; ++q) {
-      // (E_pq psi)[I] = <I|E_pq|J> psi[J] with J the E_qp image of I.
-      int x = q * n + p;
-      int32_t ea = afull[(size_t)x * nsa + ia];
-      int32_t eb = bfull[(size_t)x * nsb + ib];
-      double v;
-      if (p == q) {
-        v = (double)(ea + eb) * pI;
-      } else {
-        v = 0.0;
-        if (ea) v += (ea > 0 ? 1.0 : -1.0) * psi[(size_t)(abs(ea) - 1) * nsb + ib];
-        if (eb) v += (eb > 0 ? 1.0 : -1.0) * row[abs(eb) - 1];
-      }
-      D[(size_t)(p * n + q) * dim + I] = v;
-    }
-  }
-  D[(size_t)(n * n) * dim + I] = pI;
-  for (int r = n * n + 1; r < nrow_pad; ++r) D[(size_t)r * dim + I] = 0.0;
-}
-
-// Partial M = D D^T over one chunk of columns (split over the column axis):
-// WAVES waves per block; wave w owns the upper-triangle 16x16 output tiles
-// (ti <= tj) w, w + WAVES, ... of the RT x RT grid.  Columns are staged KS at
-// a time in LDS, transposed (sD[k][row], row stride LD + 1), from loads of KS
-// consecutive columns per row (one 128-byte line); the next slice is loaded
-// into registers while the MFMAs run on the current one, so every MFMA
-// operand is one LDS read and D is read from HBM once.
+// Partial M = D D^T over one chunk of determinants (split over the
+// determinant axis), D generated while it is staged: D[p*n+q][I] =
+// (E_pq psi)[I] = sign * psi[J] summed over the alpha and beta images J of I
+// (full excitation tables, entry x = q*n+p as k_exc writes them), D[n*n][I]
+// = psi[I], rows above it zero, so D never touches memory.  WAVES waves per
+// block; wave w owns the upper-triangle 16x16 output tiles (ti <= tj) w,
+// w + WAVES, ... of the RT x RT grid.  Determinants are staged KS at a time
+// in LDS, transposed (sD[k][row], row stride LD + 1).  Each thread owns
+// NLOAD fixed (row, determinant offset) elements of a slice; their table
+// entries are loaded two slices ahead and their amplitudes one slice ahead,
+// so both overlap the MFMAs of the current slice.  dim < 2^31.
 template <int RT, int WAVES>
-__global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(const double* __restrict__ D, int64_t dim, int64_t chunk,
+__global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int nsb, const int32_t* __restrict__ afull,
+                                                          const int32_t* __restrict__ bfull,
+                                                          const double* __restrict__ psi, int64_t chunk,
                                                           double* __restrict__ partial) {
   typedef double double4_t __attribute__((ext_vector_type(4)));
   constexpr int LD = RT * 16, LDP = LD + 1;
@@ -544,27 +512,95 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(const double* __restri
   double4_t acc[PER];
 #pragma unroll
   for (int t = 0; t < PER; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-  const int64_t c0 = (int64_t)blockIdx.x * chunk;
-  const int64_t c1 = c0 + chunk < dim ? c0 + chunk : dim;
-  double pre[NLOAD];
-  auto load = [&](int64_t k0) {
+  const int nn = n * n;
+  const uint32_t dim = (uint32_t)nsa * (uint32_t)nsb;
+  const uint32_t c0 = (uint32_t)((int64_t)blockIdx.x * chunk);
+  const uint32_t c1 = (uint32_t)min((int64_t)c0 + chunk, (int64_t)dim);
+  const double inv_nsb = 1.0 / (double)nsb;
+  // owned elements: kind 0 zero row, 1 diagonal E_pp, 2 E_pq (p != q), 3 psi row
+  int kind[NLOAD], ex[NLOAD];
+#pragma unroll
+  for (int u = 0; u < NLOAD; ++u) {
+    const int i = threadIdx.x + u * NTH;
+    const int r = i / KS;
+    kind[u] = 0;
+    ex[u] = 0;
+    if (i < LD * KS && r < nn) {
+      const int p = r / n, q = r - p * n;
+      kind[u] = p == q ? 1 : 2;
+      ex[u] = q * n + p;
+    } else if (i < LD * KS && r == nn) {
+      kind[u] = 3;
+    }
+  }
+  int ta[NLOAD], tb[NLOAD], tia[NLOAD], tib[NLOAD];
+  auto tables = [&](uint32_t k0) {
 #pragma unroll
     for (int u = 0; u < NLOAD; ++u) {
-      const int i = threadIdx.x + u * NTH;
-      const int r = i / KS, k = i % KS;
-      const int64_t col = k0 + k;
-      pre[u] = (i < LD * KS && col < c1) ? D[(size_t)r * dim + col] : 0.0;
+      const uint32_t I = k0 + (uint32_t)((threadIdx.x + u * NTH) % KS);
+      tia[u] = -1;
+      if (kind[u] != 0 && I < c1) {
+        int ia = (int)((double)I * inv_nsb);
+        int ib = (int)I - ia * nsb;
+        if (ib < 0) {
+          --ia;
+          ib += nsb;
+        } else if (ib >= nsb) {
+          ++ia;
+          ib -= nsb;
+        }
+        tia[u] = ia;
+        tib[u] = ib;
+        if (kind[u] <= 2) {
+          ta[u] = afull[(size_t)ex[u] * nsa + ia];
+          tb[u] = bfull[(size_t)ex[u] * nsb + ib];
+        }
+      }
     }
   };
-  load(c0);
-  for (int64_t k0 = c0; k0 < c1; k0 += KS) {
+  double xa[NLOAD], xb[NLOAD];
+  int ca[NLOAD], cb[NLOAD];  // coefficients: signs or an occupation sum
+  auto amps = [&](uint32_t k0) {
+#pragma unroll
+    for (int u = 0; u < NLOAD; ++u) {
+      const uint32_t I = k0 + (uint32_t)((threadIdx.x + u * NTH) % KS);
+      ca[u] = 0;
+      cb[u] = 0;
+      xa[u] = 0.0;
+      xb[u] = 0.0;
+      if (tia[u] < 0) continue;
+      const int ia = tia[u], ib = tib[u];
+      uint32_t oa = I, ob = I;
+      if (kind[u] == 3) {
+        ca[u] = 1;
+      } else if (kind[u] == 1) {
+        ca[u] = ta[u] + tb[u];
+      } else {
+        if (ta[u]) {
+          oa = (uint32_t)(abs(ta[u]) - 1) * (uint32_t)nsb + (uint32_t)ib;
+          ca[u] = ta[u] > 0 ? 1 : -1;
+        }
+        if (tb[u]) {
+          ob = (uint32_t)ia * (uint32_t)nsb + (uint32_t)(abs(tb[u]) - 1);
+          cb[u] = tb[u] > 0 ? 1 : -1;
+        }
+      }
+      xa[u] = psi[oa];
+      xb[u] = psi[ob];
+    }
+  };
+  tables(c0);
+  amps(c0);
+  if (c0 + KS < c1) tables(c0 + KS);
+  for (uint32_t k0 = c0; k0 < c1; k0 += KS) {
 #pragma unroll
     for (int u = 0; u < NLOAD; ++u) {
       const int i = threadIdx.x + u * NTH;
-      if (i < LD * KS) sD[(i % KS) * LDP + i / KS] = pre[u];
+      if (i < LD * KS) sD[(i % KS) * LDP + i / KS] = (double)ca[u] * xa[u] + (double)cb[u] * xb[u];
     }
     __syncthreads();
-    if (k0 + KS < c1) load(k0 + KS);
+    if (k0 + KS < c1) amps(k0 + KS);
+    if (k0 + 2 * KS < c1) tables(k0 + 2 * KS);
 #pragma unroll
     for (int kk = 0; kk < KS; kk += 4) {
       const double* row = sD + (kk + kq) * LDP + li;
@@ -601,6 +637,12 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int nchunks, 
   out[i] = v;
 }
 
+// RDMs.  Rows of D: x = p*n + q holds (E_pq psi), row n*n holds psi, the
+// remaining padding rows are zero.  M = D D^T is accumulated by k_gram_mfma
+// (D generated on the fly) over determinant chunks (one partial per chunk,
+// summed in a fixed order), then
+//   rdm1_pq   = M[p*n+q][n*n]                  (= <E_pq psi|psi>)
+//   rdm2_pqrs = M[q*n+p][r*n+s] - delta_qr rdm1_ps   (<psi|E_pq E_rs|psi> - ...)
 int64_t rdm_work_bytes(const Space& s, int* rt_out, int* nchunks_out, int64_t* chunk_out) {
   const int rows = s.n * s.n + 1;
   const int rt = (rows + 15) / 16;
@@ -615,31 +657,32 @@ int64_t rdm_work_bytes(const Space& s, int* rt_out, int* nchunks_out, int64_t* c
   if (nchunks_out) *nchunks_out = (int)nchunks;
   if (chunk_out) *chunk_out = chunk;
   const int64_t ld = (int64_t)rt * 16;
-  return 8 * (ld * s.dim + nchunks * ld * ld + ld * ld + 64);
+  return 8 * (nchunks * ld * ld + ld * ld + 64);
 }
 
 void rdm_impl(const Space& s, const double* psi, double* rdm1, double* rdm2, double* work, hipStream_t st) {
   int rt, nchunks;
   int64_t chunk;
   rdm_work_bytes(s, &rt, &nchunks, &chunk);
+  SQ_CHECK(s.dim < ((int64_t)1 << 31), SQ_ERR_ARG, "RDMs need fewer than 2^31 determinants");
   const int n = s.n;
   const int64_t ld = (int64_t)rt * 16;
-  double* D = work;
-  double* partial = D + (size_t)ld * s.dim;
+  double* partial = work;
   double* M = partial + (size_t)nchunks * ld * ld;
-  dim3 grid((s.nsb + kB - 1) / kB, s.nsa);
-  SQ_PROF(PC_DBUILD, st, 8.0 * (double)s.dim * (ld + 1), 0.0,
-          k_dbuild_full<<<grid, kB, 0, st>>>(n, s.nsa, s.nsb, (int)ld, s.d_aexc_full, s.d_bexc_full, psi, D));
+  // algorithmic traffic: psi and the two excitation-table rows per E_pq; flops: upper-triangle tiles
+  const double gram_bytes = 8.0 * (double)s.dim + 4.0 * (double)(n * n) * (s.nsa + s.nsb);
   const double gram_flops = 2.0 * 256.0 * (rt * (rt + 1) / 2) * (double)s.dim;
   switch (rt) {
-#define SQ_R(RT)                                                                                 \
-  case RT:                                                                                       \
-    if (RT >= 6)                                                                                 \
-      SQ_PROF(PC_RDM_GRAM, st, 8.0 * (double)s.dim * ld, gram_flops,                             \
-              k_gram_mfma<RT, 16><<<nchunks, 1024, 0, st>>>(D, s.dim, chunk, partial));         \
-    else                                                                                         \
-      SQ_PROF(PC_RDM_GRAM, st, 8.0 * (double)s.dim * ld, gram_flops,                             \
-              k_gram_mfma<RT, 4><<<nchunks, 256, 0, st>>>(D, s.dim, chunk, partial));           \
+#define SQ_R(RT)                                                                                          \
+  case RT:                                                                                                \
+    if (RT >= 6)                                                                                          \
+      SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                    \
+              k_gram_mfma<RT, 16><<<nchunks, 1024, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi, \
+                                                            chunk, partial));                             \
+    else                                                                                                  \
+      SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                    \
+              k_gram_mfma<RT, 4><<<nchunks, 256, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi,   \
+                                                          chunk, partial));                               \
     break;
     SQ_R(1) SQ_R(2) SQ_R(3) SQ_R(4) SQ_R(5) SQ_R(6) SQ_R(7) SQ_R(8) SQ_R(9) SQ_R(10) SQ_R(11) SQ_R(12) SQ_R(13)
 #undef SQ_R
