@@ -341,7 +341,7 @@ def bench_tups14(args) -> dict:
                    "max_grad_minus_central_difference": float(max(fd))},
         "roofline": {name: roof(k, name) for k, name in ((kg, "sigma_gemm_mfma"), (kg, "sigma_gather"),
                                                            (kg, "tile_forward"), (kg, "tile_reverse"),
-                                                           (kr, "d_build"), (kr, "rdm_gram_mfma"))},
+                                                           (kr, "rdm_gram_mfma"))},
         "kernels_energy_gradient": kg,
         "kernels_state_and_rdms": kr,
     }
