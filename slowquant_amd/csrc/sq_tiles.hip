@@ -118,6 +118,8 @@ struct StageDev {
   int npairs;
   int nsb;
   int nt;                    // tiles (largest first)
+  int nb;                    // dispatch slots: block slot b runs tile order[b] (-1: none)
+  const int32_t* order;      // XCD-aware dispatch order of the tiles
   int64_t poff;              // this stage's block of the gradient partials
   const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
@@ -478,6 +480,45 @@ static void finalize_stage(const Space& sp, Stage& S) {
   d.ctab = upload(S.allocs, ctab);
   S.d_entries = upload(S.allocs, S.entries);
   d.nt = S.ntiles;
+  // Dispatch order.  Workgroups go to the XCDs round robin (slot b on XCD
+  // b % 8), each XCD with its own L2.  Tiles of one alpha group read the same
+  // rows of the state, so whole groups are given to one XCD (groups largest
+  // first, each to the XCD with the fewest elements so far) and run there
+  // back to back; SQ_TILE_XCD=0 keeps the plain largest-first order.
+  {
+    constexpr int kXcds = 8;
+    std::vector<int32_t> order;
+    const char* xenv = getenv("SQ_TILE_XCD");
+    if (xenv && xenv[0] == '0') {
+      order.resize(tiles.size());
+      std::iota(order.begin(), order.end(), 0);
+    } else {
+      std::vector<std::vector<int32_t>> of_group(nag);
+      std::vector<int64_t> gsize(nag, 0);
+      for (size_t t = 0; t < tiles.size(); ++t) {
+        of_group[tiles[t].x].push_back((int32_t)t);
+        gsize[tiles[t].x] += tiles[t].w;
+      }
+      std::vector<int> gorder(nag);
+      std::iota(gorder.begin(), gorder.end(), 0);
+      std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) { return gsize[a] > gsize[b]; });
+      std::vector<std::vector<int32_t>> bins(kXcds);
+      std::vector<int64_t> load(kXcds, 0);
+      for (int ga : gorder) {
+        if (of_group[ga].empty()) continue;
+        const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[x] += gsize[ga];
+        bins[x].insert(bins[x].end(), of_group[ga].begin(), of_group[ga].end());
+      }
+      size_t len = 0;
+      for (auto& b : bins) len = std::max(len, b.size());
+      order.assign(len * kXcds, -1);
+      for (int x = 0; x < kXcds; ++x)
+        for (size_t k = 0; k < bins[x].size(); ++k) order[k * kXcds + x] = bins[x][k];
+    }
+    d.nb = (int)order.size();
+    d.order = upload(S.allocs, order);
+  }
   S.dev = d;
   S.lds_fwd = S.lds_rev = 0;
   for (const TileRec& r : recs) {
@@ -1064,7 +1105,9 @@ __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __re
                                                   const double* src, double* dst) {
   extern __shared__ double smem[];
   // block b walks tiles b, b + grid, ... (tiles are sorted largest first)
-  for (int tile = blockIdx.x; tile < S.nt; tile += gridDim.x) {
+  for (int slot = blockIdx.x; slot < S.nb; slot += gridDim.x) {
+    const int tile = S.order[slot];
+    if (tile < 0) continue;
     TileLds m;
     const TileView t = tile_setup<NT>(S, smem, false, tile, m);
     tile_load<NT, 1>(S, t, m, src, nullptr);
@@ -1093,7 +1136,9 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
   __shared__ double red[kMaxPairs][8][kWaves];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool wb = write_back != 0;
-  for (int tile = blockIdx.x; tile < S.nt; tile += gridDim.x) {
+  for (int slot = blockIdx.x; slot < S.nb; slot += gridDim.x) {
+    const int tile = S.order[slot];
+    if (tile < 0) continue;
     TileLds m;
     const TileView t = tile_setup<NT>(S, smem, true, tile, m);
     tile_load<NT, 2>(S, t, m, psi, lam);
@@ -1231,10 +1276,11 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
 
 static double stage_bytes(const Space& s, int vectors) { return 16.0 * (double)s.dim * vectors; }
 
-// Tile grids: one block per tile up to blocks_per_cu blocks per CU (more
-// tiles are walked by the same blocks).
-static unsigned tile_grid(int ntiles, int blocks_per_cu) {
-  return (unsigned)std::max(1, std::min(ntiles, 256 * blocks_per_cu));
+// Tile grids: one block per dispatch slot up to blocks_per_cu blocks per CU
+// (more slots are walked by the same blocks; 256 * blocks_per_cu is a
+// multiple of 8, so a block stays on one XCD's share of the slots).
+static unsigned tile_grid(int nslots, int blocks_per_cu) {
+  return (unsigned)std::max(1, std::min(nslots, 256 * blocks_per_cu));
 }
 
 static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, hipStream_t st) {
@@ -1244,12 +1290,12 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
 
 static void launch_fwd(const Stage& S, bool dagger, const double* src, double* dst, hipStream_t st) {
   allow_lds(k_stage_fwd<kNT>, S.lds_fwd);
-  k_stage_fwd<kNT><<<tile_grid(S.ntiles, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
+  k_stage_fwd<kNT><<<tile_grid(S.dev.nb, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
 }
 
 static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
   allow_lds(k_stage_rev<kNT>, S.lds_rev);
-  k_stage_rev<kNT><<<tile_grid(S.ntiles, 8), kNT, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
+  k_stage_rev<kNT><<<tile_grid(S.dev.nb, 8), kNT, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
 }
 
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
