@@ -487,7 +487,7 @@ template <int RT, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int nsb, const int32_t* __restrict__ afull,
                                                           const int32_t* __restrict__ bfull,
                                                           const double* __restrict__ psi, int64_t chunk,
-                                                          double* __restrict__ partial) {
+                                                          int nchunks, int xcd, double* __restrict__ partial) {
   typedef double double4_t __attribute__((ext_vector_type(4)));
   constexpr int LD = RT * 16, LDP = LD + 1;
   constexpr int KS = 16;
@@ -514,7 +514,13 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int ns
   for (int t = 0; t < PER; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
   const int nn = n * n;
   const uint32_t dim = (uint32_t)nsa * (uint32_t)nsb;
-  const uint32_t c0 = (uint32_t)((int64_t)blockIdx.x * chunk);
+  // chunk of this block: with xcd = 1, XCD b % 8 (workgroups go to the XCDs
+  // round robin) takes a contiguous range of chunks, so the blocks running
+  // together on one XCD cover neighbouring alpha rows
+  int cidx = blockIdx.x;
+  if (xcd) cidx = (int)((blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3));
+  if (cidx >= nchunks) return;
+  const uint32_t c0 = (uint32_t)((int64_t)cidx * chunk);
   const uint32_t c1 = (uint32_t)min((int64_t)c0 + chunk, (int64_t)dim);
   const double inv_nsb = 1.0 / (double)nsb;
   // owned elements: kind 0 zero row, 1 diagonal E_pp, 2 E_pq (p != q), 3 psi row
@@ -617,7 +623,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int ns
     __syncthreads();  // the next slice overwrites sD
   }
   const int ld = RT * 16;
-  double* out = partial + (size_t)blockIdx.x * ld * ld;
+  double* out = partial + (size_t)cidx * ld * ld;
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
     if (wave + WAVES * t < NT) {
@@ -672,17 +678,20 @@ void rdm_impl(const Space& s, const double* psi, double* rdm1, double* rdm2, dou
   // algorithmic traffic: psi and the two excitation-table rows per E_pq; flops: upper-triangle tiles
   const double gram_bytes = 8.0 * (double)s.dim + 4.0 * (double)(n * n) * (s.nsa + s.nsb);
   const double gram_flops = 2.0 * 256.0 * (rt * (rt + 1) / 2) * (double)s.dim;
+  const char* xenv = getenv("SQ_GRAM_XCD");
+  const int xcd = (xenv && xenv[0] == '0') ? 0 : 1;
+  const unsigned gblocks = xcd ? (unsigned)((nchunks + 7) / 8 * 8) : (unsigned)nchunks;
   switch (rt) {
 #define SQ_R(RT)                                                                                          \
   case RT:                                                                                                \
     if (RT >= 6)                                                                                          \
       SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                    \
-              k_gram_mfma<RT, 16><<<nchunks, 1024, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi, \
-                                                            chunk, partial));                             \
+              k_gram_mfma<RT, 16><<<gblocks, 1024, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi, \
+                                                            chunk, nchunks, xcd, partial));               \
     else                                                                                                  \
       SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                    \
-              k_gram_mfma<RT, 4><<<nchunks, 256, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi,   \
-                                                          chunk, partial));                               \
+              k_gram_mfma<RT, 4><<<gblocks, 256, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi,   \
+                                                          chunk, nchunks, xcd, partial));                 \
     break;
     SQ_R(1) SQ_R(2) SQ_R(3) SQ_R(4) SQ_R(5) SQ_R(6) SQ_R(7) SQ_R(8) SQ_R(9) SQ_R(10) SQ_R(11) SQ_R(12) SQ_R(13)
 #undef SQ_R
