@@ -347,6 +347,69 @@ def bench_tups14(args) -> dict:
     }
 
 
+def bench_ucc(args) -> dict:
+    """Unfactorised UCCSD (singlet spin-adapted singles and doubles) on a
+    synthetic (ne,no) Hamiltonian, n = --ucc-orbitals (rank 2n, variance
+    0.3^2/n), theta ~ N(0, 0.05^2) seed 0: one step = exp(T)psi0 by the
+    matrix-free scaled Taylor series, the energy and the analytic gradient.
+    Checks the gradient against central differences (h = 1e-5) on three
+    parameters."""
+    from slowquant_amd import engine as E
+    from slowquant_amd import profiling
+    from slowquant_amd.synthetic import synthetic_problem
+    from slowquant_amd.util import UccStructure
+
+    n = args.ucc_orbitals
+    lay = UccStructure()
+    occ, unocc = list(range(n // 2)), list(range(n // 2, n))
+    lay.add_sa_singles(occ, unocc)
+    lay.add_sa_doubles(occ, unocc)
+    h, g, theta = synthetic_problem(n, 2 * n, lay.n_params, 0.05, SEED)
+    sp = E.Space(n, n // 2, n // 2)
+    types = ["sa_single_ucc" if t == "sa_single" else t for t in lay.excitation_operator_type]
+    prog = E.Program(sp, types, lay.excitation_indices)
+    H = E.Hamiltonian(sp, h, g)
+    psi0 = E.hf_state(sp, 0)
+    th = np.asarray(theta)
+
+    def step():
+        return E.ucc_energy_grad(prog, H, th, psi0)
+
+    e, gr = step()
+    fd = []
+    for i in (0, len(th) // 2, len(th) - 1):
+        tp, tm = th.copy(), th.copy()
+        tp[i] += 1e-5
+        tm[i] -= 1e-5
+        fd.append(abs((E.ucc_energy_grad(prog, H, tp, psi0, False)[0] - E.ucc_energy_grad(prog, H, tm, psi0, False)[0])
+                      / 2e-5 - gr[i]))
+    t = time_steps(step, args.steps, args.warmup)
+    k = profiling.profile(step, 2)
+    dom = profiling.dominant(k)
+    return {
+        "metric": f"UCCSD energy+analytic gradient evals/s, synthetic ({n}e,{n}o) FP64",
+        "value": 1.0 / t,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"UCCSD ({n}e,{n}o), {lay.n_params} params (SA singles + SA doubles), CI dim {sp.dim}, "
+                               f"synthetic h/g rank {2 * n}, theta~N(0,0.05^2) seed {SEED}",
+                   "n_orb": n, "ci_dim": sp.dim, "n_params": lay.n_params},
+        "energy": e,
+        "checks": {"max_grad_minus_central_difference": float(max(fd))},
+        "roofline": {"kernel": dom["name"], "bound": dom["bound"], "achieved": dom["achieved"], "peak": dom["peak"],
+                     "unit": dom["unit"], "frac": dom["achieved"] / dom["peak"]},
+        "kernels": k,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -355,10 +418,11 @@ def main() -> int:
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
-    ap.add_argument("--workload", choices=("tups12", "batch512", "tups14"), default="tups12",
+    ap.add_argument("--workload", choices=("tups12", "batch512", "tups14", "ucc"), default="tups12",
                     help="tups12: the headline (12e,12o) tUPS metric; batch512: 512 batched instances; "
-                         "tups14: (14e,14o) 6-layer energy, gradient and RDMs")
+                         "tups14: (14e,14o) 6-layer energy, gradient and RDMs; ucc: (8e,8o) UCCSD")
     ap.add_argument("--instances", type=int, default=512)
+    ap.add_argument("--ucc-orbitals", type=int, default=8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -374,6 +438,9 @@ def main() -> int:
         return 0
     if args.workload == "tups14":
         print(json.dumps(bench_tups14(args)), flush=True)
+        return 0
+    if args.workload == "ucc":
+        print(json.dumps(bench_ucc(args)), flush=True)
         return 0
     lay, h, g, theta, sp, prog, H, psi0 = build_workload()
     th = np.asarray(theta)

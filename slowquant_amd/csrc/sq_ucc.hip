@@ -42,41 +42,89 @@ __device__ __forceinline__ bool ops_adjoint(const SpinOps& o, uint32_t& m, int& 
   return true;
 }
 
-// nxt[I] = f * sum_t w[t] <I|O_t|J> cur[J] (+ f2 * sum over terms [u0,u1) with
-// unit weights applied to cur2); acc[I] += nxt[I] when acc != nullptr.
-__global__ void k_tapply_w(SV v, const Term* __restrict__ terms, const double* __restrict__ w, int t0, int t1,
-                           const double* __restrict__ cur, int u0, int u1, const double* __restrict__ cur2,
-                           double f, double* __restrict__ nxt, double* __restrict__ acc) {
-  int ia = blockIdx.y;
-  int ib = blockIdx.x * blockDim.x + threadIdx.x;
+// part[g][I] = sum over the terms t = g, g + G, ... of w[t] <I|O_t|J> cur[J]
+// (w = nullptr: unit weights).  Splitting the terms over G groups keeps
+// small spaces' grids full; k_tapply_sum adds the groups in a fixed order.
+__global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const double* __restrict__ w, int nt, int G,
+                              const double* __restrict__ cur, double* __restrict__ part) {
+  const int ia = blockIdx.y;
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
   if (ib >= v.nsb) return;
-  uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
+  const int g = blockIdx.z;
+  const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
   double val = 0.0;
-  for (int t = t0; t < t1; ++t) {
+  for (int t = g; t < nt; t += G) {
     const Term& T = terms[t];
     uint32_t a = a0, b = b0;
     int par = 0;
     if (!ops_adjoint(T.a, a, par) || !ops_adjoint(T.b, b, par)) continue;
-    int ra = v.arank[a], rb = v.brank[b];
+    const int ra = v.arank[a], rb = v.brank[b];
     if (ra < 0 || rb < 0) continue;
-    double c = w[t] * T.coef;
+    const double c = (w ? w[t] : 1.0) * T.coef;
     val += ((par & 1) ? -c : c) * cur[(size_t)ra * v.nsb + rb];
   }
-  if (cur2) {
-    for (int t = u0; t < u1; ++t) {
-      const Term& T = terms[t];
-      uint32_t a = a0, b = b0;
-      int par = 0;
-      if (!ops_adjoint(T.a, a, par) || !ops_adjoint(T.b, b, par)) continue;
-      int ra = v.arank[a], rb = v.brank[b];
-      if (ra < 0 || rb < 0) continue;
-      val += ((par & 1) ? -T.coef : T.coef) * cur2[(size_t)ra * v.nsb + rb];
+  const size_t dim = (size_t)v.nsa * v.nsb;
+  part[(size_t)g * dim + (size_t)ia * v.nsb + ib] = val;
+}
+
+// nxt[I] = f * sum_g part[g][I]; acc[I] += nxt[I] when acc != nullptr.
+__global__ void k_tapply_sum(const double* __restrict__ part, int G, int64_t dim, double f, double* __restrict__ nxt,
+                             double* __restrict__ acc) {
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= dim) return;
+  double v = 0.0;
+  for (int g = 0; g < G; ++g) v += part[(size_t)g * dim + I];
+  v *= f;
+  nxt[I] = v;
+  if (acc) acc[I] += v;
+}
+
+// Per gate i, per block b of amplitudes: dpart[b*ng + i] = sum over the
+// block's I of v[I] (T_i u)[I] with T_i the gate's generator (unit weight).
+// Block (x, y, z) takes the gates i = z, z + gridDim.z, ...; every (b, i) is
+// written by exactly one block and summed over b in a fixed order later.
+__global__ void k_gate_dots(SV sv, const Term* __restrict__ terms, const GateDev* __restrict__ gates, int ng,
+                            const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ dpart) {
+  __shared__ double red[kUB / 64];
+  const int ia = blockIdx.y;
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ok = ib < sv.nsb;
+  const int b = blockIdx.y * gridDim.x + blockIdx.x;
+  const uint32_t a0 = sv.astr[ia], b0 = ok ? sv.bstr[ib] : 0u;
+  const double vI = ok ? v[(size_t)ia * sv.nsb + ib] : 0.0;
+  for (int i = blockIdx.z; i < ng; i += gridDim.z) {
+    double val = 0.0;
+    if (ok && vI != 0.0) {
+      for (int t = gates[i].term_begin; t < gates[i].term_end; ++t) {
+        const Term& T = terms[t];
+        uint32_t a = a0, bb = b0;
+        int par = 0;
+        if (!ops_adjoint(T.a, a, par) || !ops_adjoint(T.b, bb, par)) continue;
+        const int ra = sv.arank[a], rb = sv.brank[bb];
+        if (ra < 0 || rb < 0) continue;
+        val += ((par & 1) ? -T.coef : T.coef) * u[(size_t)ra * sv.nsb + rb];
+      }
     }
+    double c = vI * val;
+    for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tot = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
+      dpart[(size_t)b * ng + i] = tot;
+    }
+    __syncthreads();  // red is reused by the next gate
   }
-  val *= f;
-  size_t I = (size_t)ia * v.nsb + ib;
-  nxt[I] = val;
-  if (acc) acc[I] += val;
+}
+
+// gsum[i] += f * sum_b dpart[b*ng + i] (fixed order).
+__global__ void k_gate_accum(const double* __restrict__ dpart, int nb, int ng, double f, double* __restrict__ gsum) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ng) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += dpart[(size_t)b * ng + i];
+  gsum[i] += f * s;
 }
 
 // w[t] = theta[gate of term t]
@@ -92,17 +140,95 @@ __global__ void k_zero(double* __restrict__ x, int64_t n) {
   if (i < n) x[i] = 0.0;
 }
 
+// A fixed pseudo-random start vector in [-1, 1) (integer hash of the index).
+__global__ void k_fill_pattern(double* __restrict__ x, int64_t n) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t h = (uint64_t)i * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+  h ^= h >> 31;
+  h *= 0xBF58476D1CE4E5B9ull;
+  h ^= h >> 29;
+  x[i] = (double)(h >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+__global__ void k_scale(double* __restrict__ x, int64_t n, const double* __restrict__ s, int slot) {
+  int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= s[slot];
+}
+
+__global__ void k_inv_sqrt_slot(double* __restrict__ s, int from, int to) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) s[to] = s[from] > 0.0 ? 1.0 / sqrt(s[from]) : 0.0;
+}
+
 static SV sv(const Space& s) { return SV{s.nsa, s.nsb, s.d_astr, s.d_bstr, s.d_arank, s.d_brank}; }
 
+// Term groups of the split T application: enough (I, group) threads to fill
+// the GPU for small spaces, at most 64 groups.
+static int term_groups(const Space& s) {
+  const int64_t G = 65536 / std::max<int64_t>(s.dim, 1);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(64, G));
+}
+
+// Work of the UCC routines (doubles), carved in this order.
+struct UccWork {
+  double *A, *B, *C, *D, *w, *part, *dpart, *partial, *scal;
+};
+
+static int64_t dot_blocks(const Space& s) { return (int64_t)((s.nsb + kUB - 1) / kUB) * s.nsa; }
+
+static int gate_groups(const Space& s, int ng) {
+  const int64_t nb = dot_blocks(s);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(ng, std::max<int64_t>(1, 512 / nb)));
+}
+
+int64_t ucc_work_doubles(const Space& s, const Program& p) {
+  const int64_t ng = (int64_t)std::max<size_t>(p.gates.size(), 1);
+  return 4 * s.dim + (int64_t)p.terms.size() + 64 + (int64_t)term_groups(s) * s.dim + dot_blocks(s) * ng + 1024 +
+         64 + 64;
+}
+
+static UccWork carve_ucc(const Space& s, const Program& p, double* work) {
+  UccWork u;
+  const int64_t ng = (int64_t)std::max<size_t>(p.gates.size(), 1);
+  u.A = work;
+  u.B = u.A + s.dim;
+  u.C = u.B + s.dim;
+  u.D = u.C + s.dim;
+  u.w = u.D + s.dim;
+  u.part = u.w + p.terms.size() + 64;
+  u.dpart = u.part + (size_t)term_groups(s) * s.dim;
+  u.partial = u.dpart + (size_t)dot_blocks(s) * ng;
+  u.scal = u.partial + 1024;
+  return u;
+}
+
+// nxt = f T cur (w: term weights, nullptr for unit weights); acc += nxt.
+static void tapply(const Space& s, const Program& p, const double* w, const double* cur, double f, double* nxt,
+                   double* acc, double* part, hipStream_t st) {
+  const int G = term_groups(s);
+  const dim3 grid((s.nsb + kUB - 1) / kUB, s.nsa, G);
+  SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
+          k_tapply_part<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, w, (int)p.terms.size(), G, cur, part));
+  k_tapply_sum<<<(unsigned)((s.dim + kUB - 1) / kUB), kUB, 0, st>>>(part, G, s.dim, f, nxt, acc);
+  SQ_HIP(hipGetLastError());
+}
+
+static double read_slot(const double* scal, int slot, hipStream_t st) {
+  double h = 0.0;
+  SQ_HIP(hipMemcpyAsync(&h, scal + slot, sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  return h;
+}
+
+// Taylor plan for exp(t T) with ||T||_2 <= nu: s steps of exp(t T / s) with
+// |t| nu / s <= 1/2, K terms per step until (|t| nu / s)^K / K! < 2^-60.
 struct TaylorPlan {
   int steps, K;
   double a;  // t / steps
 };
 
-static TaylorPlan taylor_plan(const Program& p, const std::vector<double>& theta, double t) {
-  double b = 0.0;
-  for (size_t g = 0; g < p.gates.size(); ++g)
-    for (int k = p.gates[g].term_begin; k < p.gates[g].term_end; ++k) b += std::fabs(t * theta[g] * p.terms[k].coef);
+static TaylorPlan taylor_plan(double nu, double t) {
+  const double b = std::fabs(t) * nu;
   TaylorPlan P;
   P.steps = std::max(1, (int)std::ceil(b / 0.5));
   const double r = b / P.steps;
@@ -116,90 +242,112 @@ static TaylorPlan taylor_plan(const Program& p, const std::vector<double>& theta
   return P;
 }
 
-// out = exp(t T(theta)) in.  work >= 3*dim + nterms doubles.
-void ucc_expm_impl(const Space& s, const Program& p, const double* theta_dev, double t, const double* in,
-                   double* out, double* work, hipStream_t st) {
-  const int ng = (int)p.gates.size();
-  std::vector<double> th(ng);
-  if (ng) SQ_HIP(hipMemcpyAsync(th.data(), theta_dev, sizeof(double) * ng, hipMemcpyDeviceToHost, st));
-  SQ_HIP(hipStreamSynchronize(st));
-  TaylorPlan P = taylor_plan(p, th, t);
-  double* A = work;
-  double* B = A + s.dim;
-  double* w = B + s.dim;
-  const int nt = (int)p.terms.size();
-  if (ng) k_term_weights<<<(ng + 255) / 256, 256, 0, st>>>(p.d_gates, ng, theta_dev, w);
+// ||T||_2 for the weights w: T is real antisymmetric (normal), so its
+// 2-norm is its spectral radius.  Power iteration (16 steps from a fixed
+// pseudo-random vector, in A and B) estimates it from below; the returned
+// value is 1.5 x the estimate + 0.05, capped by the triangle-inequality bound
+// sum_t |w_t coef_t| (each raw string has at most one unit entry per column).
+static double t_norm(const Space& s, const Program& p, const std::vector<double>& th, const UccWork& u,
+                     hipStream_t st) {
+  double bound = 0.0;
+  for (size_t g = 0; g < p.gates.size(); ++g)
+    for (int k = p.gates[g].term_begin; k < p.gates[g].term_end; ++k) bound += std::fabs(th[g] * p.terms[k].coef);
+  if (bound == 0.0 || s.dim == 0) return 0.0;
+  const unsigned nb = (unsigned)((s.dim + kUB - 1) / kUB);
+  double* x = u.A;
+  double* y = u.B;
+  k_fill_pattern<<<nb, kUB, 0, st>>>(x, s.dim);
   SQ_HIP(hipGetLastError());
-  copy_vec(in, out, s.dim, st);
-  dim3 grid((s.nsb + kUB - 1) / kUB, s.nsa);
+  for (int it = 0; it < 16; ++it) {
+    dot_dev(x, x, s.dim, u.scal, 0, 1.0, u.partial, st);
+    k_inv_sqrt_slot<<<1, 64, 0, st>>>(u.scal, 0, 1);
+    k_scale<<<nb, kUB, 0, st>>>(x, s.dim, u.scal, 1);  // |x| = 1
+    tapply(s, p, u.w, x, 1.0, y, nullptr, u.part, st);  // y = T x
+    std::swap(x, y);
+  }
+  dot_dev(x, x, s.dim, u.scal, 0, 1.0, u.partial, st);
+  const double est = std::sqrt(std::max(0.0, read_slot(u.scal, 0, st)));
+  return std::min(bound, 1.5 * est + 0.05);
+}
+
+// out = exp(t T) in over the plan for norm nu (A, B: Taylor terms).
+static void expm_apply(const Space& s, const Program& p, const UccWork& u, double nu, double t, const double* in,
+                       double* out, hipStream_t st) {
+  const TaylorPlan P = taylor_plan(nu, t);
+  if (in != out) copy_vec(in, out, s.dim, st);
+  if (nu == 0.0) return;
   for (int step = 0; step < P.steps; ++step) {
-    copy_vec(out, A, s.dim, st);
-    double* cur = A;
-    double* nxt = B;
+    copy_vec(out, u.A, s.dim, st);
+    double* cur = u.A;
+    double* nxt = u.B;
     for (int k = 1; k <= P.K; ++k) {
-      SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
-              k_tapply_w<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, w, 0, nt, cur, 0, 0, nullptr, P.a / k, nxt, out));
+      tapply(s, p, u.w, cur, P.a / k, nxt, out, u.part, st);
       std::swap(cur, nxt);
     }
   }
 }
 
-// Analytic dE/dtheta for E = <psi|H|psi>, psi = exp(T(theta)) psi0, given
-// lam = H psi: grad_dev[i] = 2 <lam| d_i psi>.  work >= 6*dim + nterms + partials.
-void ucc_grad_impl(const Space& s, const Program& p, const double* theta_dev, const double* psi0, const double* lam,
-                   double* grad_dev, double* work, double* partial, hipStream_t st) {
+static std::vector<double> theta_host(const Program& p, const double* theta_dev, hipStream_t st) {
   const int ng = (int)p.gates.size();
   std::vector<double> th(ng);
   if (ng) SQ_HIP(hipMemcpyAsync(th.data(), theta_dev, sizeof(double) * ng, hipMemcpyDeviceToHost, st));
   SQ_HIP(hipStreamSynchronize(st));
-  TaylorPlan P = taylor_plan(p, th, 1.0);
-  // the T_i block adds at most ||T_i|| to the bound; refine per gate
-  double* top = work;
-  double* bot = top + s.dim;
-  double* tA = bot + s.dim;
-  double* tB = tA + s.dim;
-  double* bA = tB + s.dim;
-  double* bB = bA + s.dim;
-  double* w = bB + s.dim;
-  const int nt = (int)p.terms.size();
+  return th;
+}
+
+static void term_weights(const Program& p, const double* theta_dev, double* w, hipStream_t st) {
+  const int ng = (int)p.gates.size();
   if (ng) k_term_weights<<<(ng + 255) / 256, 256, 0, st>>>(p.d_gates, ng, theta_dev, w);
   SQ_HIP(hipGetLastError());
-  dim3 grid((s.nsb + kUB - 1) / kUB, s.nsa);
-  for (int i = 0; i < ng; ++i) {
-    const GateDev& gi = p.gates[i];
-    double bi = 0.0;
-    for (int k = gi.term_begin; k < gi.term_end; ++k) bi += std::fabs(p.terms[k].coef);
-    double b = 0.0;
-    for (size_t g = 0; g < p.gates.size(); ++g)
-      for (int k = p.gates[g].term_begin; k < p.gates[g].term_end; ++k) b += std::fabs(th[g] * p.terms[k].coef);
-    b += bi;
-    int steps = std::max(1, (int)std::ceil(b / 0.5));
-    double r = b / steps;
-    int K = 1;
-    double term = r;
-    while (term > std::ldexp(1.0, -60) && K < 60) {
-      ++K;
-      term *= r / K;
+}
+
+// out = exp(t T(theta)) in.  work: ucc_work_doubles.
+void ucc_expm_impl(const Space& s, const Program& p, const double* theta_dev, double t, const double* in,
+                   double* out, double* work, hipStream_t st) {
+  const UccWork u = carve_ucc(s, p, work);
+  const std::vector<double> th = theta_host(p, theta_dev, st);
+  term_weights(p, theta_dev, u.w, st);
+  const double nu = t_norm(s, p, th, u, st);
+  expm_apply(s, p, u, nu, t, in, out, st);
+}
+
+// 8-point Gauss-Legendre nodes and weights on [0, 1].
+static const double kGLx[8] = {0.019855071751231856, 0.10166676129318664, 0.2372337950418355,  0.4082826787521751,
+                               0.5917173212478249,   0.7627662049581645,  0.8983332387068134,  0.9801449282487681};
+static const double kGLw[8] = {0.05061426814518813, 0.11119051722668724, 0.15685332293894363, 0.18134189168918100,
+                               0.18134189168918100, 0.15685332293894363, 0.11119051722668724, 0.05061426814518813};
+
+// Analytic dE/dtheta for E = <psi|H|psi>, psi = exp(T) psi0, lam = H psi:
+//   d psi / d theta_i = int_0^1 e^{sT} T_i e^{(1-s)T} psi0 ds, and with
+//   e^{sT}^T = e^{-sT} (T real antisymmetric)
+//   grad_i = 2 int_0^1 <e^{-sT} lam | T_i | e^{-sT} psi> ds,
+// by composite 8-point Gauss-Legendre over m subintervals with 2 nu / m <= 1
+// (the integrand's frequencies are bounded by 2 ||T||_2; the quadrature
+// error is then below 1e-14 relative).  psi and lam are propagated from node
+// to node (C, D); every node adds all gates' dots in one pass.
+static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, double nu, const double* psi,
+                          const double* lam, double* grad_dev, hipStream_t st) {
+  const int ng = (int)p.gates.size();
+  const int m = std::max(1, (int)std::ceil(2.0 * nu));
+  copy_vec(psi, u.C, s.dim, st);
+  copy_vec(lam, u.D, s.dim, st);
+  k_zero<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(grad_dev, ng);
+  SQ_HIP(hipGetLastError());
+  const int nbx = (s.nsb + kUB - 1) / kUB;
+  const int nb = nbx * s.nsa;
+  const dim3 dgrid(nbx, s.nsa, gate_groups(s, ng));
+  double prev = 0.0;
+  for (int j = 0; j < m; ++j) {
+    for (int q = 0; q < 8; ++q) {
+      const double node = (j + kGLx[q]) / m;
+      const double ds = node - prev;
+      expm_apply(s, p, u, nu, -ds, u.C, u.C, st);
+      expm_apply(s, p, u, nu, -ds, u.D, u.D, st);
+      prev = node;
+      k_gate_dots<<<dgrid, kUB, 0, st>>>(sv(s), p.d_terms, p.d_gates, ng, u.C, u.D, u.dpart);
+      k_gate_accum<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(u.dpart, nb, ng, 2.0 * kGLw[q] / m, grad_dev);
+      SQ_HIP(hipGetLastError());
     }
-    const double a = 1.0 / steps;
-    // [top; bot] = [0; psi0]
-    k_zero<<<(unsigned)((s.dim + kUB - 1) / kUB), kUB, 0, st>>>(top, s.dim);
-    copy_vec(psi0, bot, s.dim, st);
-    for (int step = 0; step < steps; ++step) {
-      copy_vec(top, tA, s.dim, st);
-      copy_vec(bot, bA, s.dim, st);
-      double *tc = tA, *tn = tB, *bc = bA, *bn = bB;
-      for (int k = 1; k <= K; ++k) {
-        // new top = a/k (T top + T_i bot); new bot = a/k T bot
-        k_tapply_w<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, w, 0, nt, tc, gi.term_begin, gi.term_end, bc, a / k, tn,
-                                         top);
-        k_tapply_w<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, w, 0, nt, bc, 0, 0, nullptr, a / k, bn, bot);
-        SQ_HIP(hipGetLastError());
-        std::swap(tc, tn);
-        std::swap(bc, bn);
-      }
-    }
-    dot_dev(lam, top, s.dim, grad_dev, i, 2.0, partial, st);
   }
 }
 
@@ -225,18 +373,23 @@ int sq_ucc_energy_grad(const sq_space* sp, const sq_program* pr, const sq_hamilt
   return guarded([&] {
     SQ_CHECK(sp && pr && H && psi0 && energy && work && (theta || pr->p.gates.empty()), SQ_ERR_ARG, "NULL argument");
     const Space& s = sp->s;
-    const int ng = (int)pr->p.gates.size();
+    const Program& p = pr->p;
+    const int ng = (int)p.gates.size();
     hipStream_t st = as_stream(stream);
     double* psi = work;
     double* lam = psi + s.dim;
     double* scal = lam + s.dim;            // 64
     double* gdev = scal + 64;              // ng
-    double* partial = gdev + std::max(ng, 1) + 1024;  // 1024
-    double* rest = partial + 1024;
-    ucc_expm_impl(s, pr->p, theta, 1.0, psi0, psi, rest, st);
-    sigma_energy_dev(s, H->h, psi, lam, scal, rest, st);
+    double* rest = gdev + std::max(ng, 1) + 64;
+    const UccWork u = carve_ucc(s, p, rest);
+    const std::vector<double> th = theta_host(p, theta, st);
+    term_weights(p, theta, u.w, st);
+    const double nu = t_norm(s, p, th, u, st);
+    expm_apply(s, p, u, nu, 1.0, psi0, psi, st);
+    sigma_energy_dev(s, H->h, psi, lam, scal, rest, st);  // reuses the UCC work area
     if (grad && ng) {
-      ucc_grad_impl(s, pr->p, theta, psi0, lam, gdev, rest, partial, st);
+      term_weights(p, theta, u.w, st);
+      ucc_grad_impl(s, p, u, nu, psi, lam, gdev, st);
       SQ_HIP(hipMemcpyAsync(grad, gdev, sizeof(double) * ng, hipMemcpyDeviceToHost, st));
     }
     SQ_HIP(hipMemcpyAsync(energy, scal, sizeof(double), hipMemcpyDeviceToHost, st));
@@ -247,10 +400,9 @@ int sq_ucc_energy_grad(const sq_space* sp, const sq_program* pr, const sq_hamilt
 int64_t sq_ucc_work_bytes(const sq_space* sp, const sq_program* pr) {
   if (!sp || !pr) return -1;
   const Space& s = sp->s;
-  int64_t nt = (int64_t)pr->p.terms.size() + 64;
-  int64_t ng = (int64_t)pr->p.gates.size() + 1;
-  int64_t body = std::max<int64_t>(6 * s.dim + nt, sigma_work_bytes(s) / 8);
-  return 8 * (2 * s.dim + 64 + ng + 2048 + body);
+  const int64_t ng = (int64_t)pr->p.gates.size() + 1;
+  const int64_t body = std::max<int64_t>(ucc_work_doubles(s, pr->p), sigma_work_bytes(s) / 8);
+  return 8 * (2 * s.dim + 64 + ng + 64 + body);
 }
 
 }  // extern "C"

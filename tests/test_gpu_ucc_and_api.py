@@ -52,6 +52,44 @@ def test_ucc_state_and_gradient_vs_oracle(exc):
     assert np.abs(g_fd - g).max() < 1e-6
 
 
+def test_ucc_synthetic_6o_state_energy_gradient():
+    """UCCSD on a synthetic (6e,6o) Hamiltonian (rank 12, variance 0.3^2/6,
+    theta ~ N(0, 0.1^2) seed 3, 54 parameters, CI dimension 400): the state
+    from the norm-planned Taylor series against the oracle's dense
+    exponential (1e-12), the energy (1e-10), and the quadrature gradient
+    against central differences of the oracle energy (h = 1e-5, 1e-8)."""
+    from oracle import fermion as F
+    from oracle.apply import expectation_value
+    from oracle.ci import get_indexing
+    from oracle.ucc import construct_ucc_state
+
+    from slowquant_amd.synthetic import SyntheticIntegrals, synthetic_hamiltonian
+    from slowquant_amd.ucc_wavefunction import WaveFunctionUCC
+
+    h, g = synthetic_hamiltonian(6, 12, np.random.default_rng(3))
+    wf = WaveFunctionUCC((6, 6), np.eye(6), SyntheticIntegrals(h, g, 6), "SD")
+    assert wf.ucc_layout.n_params == 54
+    theta = np.random.default_rng(3).normal(0, 0.1, wf.ucc_layout.n_params)
+    wf.thetas = theta.tolist()
+    ci = get_indexing(0, 6, 0, 3, 3)
+    ref = construct_ucc_state(wf.csf_coeffs, ci, theta, wf.ucc_layout)
+    assert np.abs(wf.ci_coeffs - ref).max() < 1e-12
+    H = F.hamiltonian_0i_0a(wf.h_mo, wf.g_mo, 0, 6)
+
+    def energy(th):
+        psi = construct_ucc_state(wf.csf_coeffs, ci, th, wf.ucc_layout)
+        return expectation_value(psi, [H], psi, ci, do_folding=False)
+
+    e, grad = wf.energy_and_gradient()
+    assert abs(e - energy(theta)) < 1e-10
+    hstep = 1e-5
+    for i in (0, 17, 53):
+        v = np.zeros_like(theta)
+        v[i] = hstep
+        fd = (energy(theta + v) - energy(theta - v)) / (2 * hstep)
+        assert abs(fd - grad[i]) < 1e-8, i
+
+
 def test_ucc_h4_sdtq_known_answer():
     """tests/test_unitary_coupled_cluster.py:589-609: UCCSDTQ(4,4) on the H4
     1.5 x 1.8 Angstrom rectangle, E_elec = -3.714153922167 (tol 1e-8)."""
