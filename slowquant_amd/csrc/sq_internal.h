@@ -68,6 +68,18 @@ struct Term {
   SpinOps b;
 };
 
+// A term's adjoint on a determinant in mask form, for terms whose spin parts
+// touch each orbital at most once (flags bit 0): it applies iff the string
+// has every creator orbital (s) and no annihilator orbital (c); the image is
+// string ^ s ^ c and the sign parity is popc(string & x) + flags bit 1,
+// summed over both spin parts.  Other terms (flags bit 0 clear) are applied
+// operator by operator.
+struct TermFast {
+  uint32_t sa, ca, xa, sb, cb, xb;
+  uint32_t flags, pad;
+  double coef;
+};
+
 // Device-side gate record.
 struct GateDev {
   int32_t kind;          // internal class, see GateClass
@@ -125,6 +137,9 @@ struct Program {
   // (offset -1: the spin part is the identity, every string).
   std::vector<int4> rot_lists;
   int2* d_lists = nullptr;
+  // Per term, its adjoint in mask form (built on first use by the UCC
+  // kernels, sq_ucc.hip).
+  mutable struct TermFast* d_term_fast = nullptr;
 };
 
 // Strings a spin part applies to, as (string index, image | parity << 31)

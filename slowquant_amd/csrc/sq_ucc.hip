@@ -45,8 +45,27 @@ __device__ __forceinline__ bool ops_adjoint(const SpinOps& o, uint32_t& m, int& 
 // part[g][I] = sum over the terms t = g, g + G, ... of w[t] <I|O_t|J> cur[J]
 // (w = nullptr: unit weights).  Splitting the terms over G groups keeps
 // small spaces' grids full; k_tapply_sum adds the groups in a fixed order.
-__global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const double* __restrict__ w, int nt, int G,
-                              const double* __restrict__ cur, double* __restrict__ part) {
+// Adjoint of term t on strings (a0, b0): images (a, b) and sign parity;
+// false if it does not apply.
+__device__ __forceinline__ bool term_adjoint(const TermFast& F, const Term* __restrict__ terms, int t, uint32_t a0,
+                                             uint32_t b0, uint32_t& a, uint32_t& b, int& par) {
+  if (F.flags & 1u) {
+    if ((a0 & F.sa) != F.sa || (a0 & F.ca) || (b0 & F.sb) != F.sb || (b0 & F.cb)) return false;
+    a = a0 ^ F.sa ^ F.ca;
+    b = b0 ^ F.sb ^ F.cb;
+    par = __popc(a0 & F.xa) + __popc(b0 & F.xb) + (int)(F.flags >> 1);
+    return true;
+  }
+  const Term& T = terms[t];
+  a = a0;
+  b = b0;
+  par = 0;
+  return ops_adjoint(T.a, a, par) && ops_adjoint(T.b, b, par);
+}
+
+__global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const TermFast* __restrict__ fast,
+                              const double* __restrict__ w, int nt, int G, const double* __restrict__ cur,
+                              double* __restrict__ part) {
   const int ia = blockIdx.y;
   const int ib = blockIdx.x * blockDim.x + threadIdx.x;
   if (ib >= v.nsb) return;
@@ -54,13 +73,13 @@ __global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const double
   const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
   double val = 0.0;
   for (int t = g; t < nt; t += G) {
-    const Term& T = terms[t];
-    uint32_t a = a0, b = b0;
-    int par = 0;
-    if (!ops_adjoint(T.a, a, par) || !ops_adjoint(T.b, b, par)) continue;
+    const TermFast F = fast[t];
+    uint32_t a, b;
+    int par;
+    if (!term_adjoint(F, terms, t, a0, b0, a, b, par)) continue;
     const int ra = v.arank[a], rb = v.brank[b];
     if (ra < 0 || rb < 0) continue;
-    const double c = (w ? w[t] : 1.0) * T.coef;
+    const double c = (w ? w[t] : 1.0) * F.coef;
     val += ((par & 1) ? -c : c) * cur[(size_t)ra * v.nsb + rb];
   }
   const size_t dim = (size_t)v.nsa * v.nsb;
@@ -83,8 +102,9 @@ __global__ void k_tapply_sum(const double* __restrict__ part, int G, int64_t dim
 // block's I of v[I] (T_i u)[I] with T_i the gate's generator (unit weight).
 // Block (x, y, z) takes the gates i = z, z + gridDim.z, ...; every (b, i) is
 // written by exactly one block and summed over b in a fixed order later.
-__global__ void k_gate_dots(SV sv, const Term* __restrict__ terms, const GateDev* __restrict__ gates, int ng,
-                            const double* __restrict__ u, const double* __restrict__ v, double* __restrict__ dpart) {
+__global__ void k_gate_dots(SV sv, const Term* __restrict__ terms, const TermFast* __restrict__ fast,
+                            const GateDev* __restrict__ gates, int ng, const double* __restrict__ u,
+                            const double* __restrict__ v, double* __restrict__ dpart) {
   __shared__ double red[kUB / 64];
   const int ia = blockIdx.y;
   const int ib = blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,13 +116,13 @@ __global__ void k_gate_dots(SV sv, const Term* __restrict__ terms, const GateDev
     double val = 0.0;
     if (ok && vI != 0.0) {
       for (int t = gates[i].term_begin; t < gates[i].term_end; ++t) {
-        const Term& T = terms[t];
-        uint32_t a = a0, bb = b0;
-        int par = 0;
-        if (!ops_adjoint(T.a, a, par) || !ops_adjoint(T.b, bb, par)) continue;
+        const TermFast F = fast[t];
+        uint32_t a, bb;
+        int par;
+        if (!term_adjoint(F, terms, t, a0, b0, a, bb, par)) continue;
         const int ra = sv.arank[a], rb = sv.brank[bb];
         if (ra < 0 || rb < 0) continue;
-        val += ((par & 1) ? -T.coef : T.coef) * u[(size_t)ra * sv.nsb + rb];
+        val += ((par & 1) ? -F.coef : F.coef) * u[(size_t)ra * sv.nsb + rb];
       }
     }
     double c = vI * val;
@@ -162,6 +182,48 @@ __global__ void k_inv_sqrt_slot(double* __restrict__ s, int from, int to) {
 
 static SV sv(const Space& s) { return SV{s.nsa, s.nsb, s.d_astr, s.d_bstr, s.d_arank, s.d_brank}; }
 
+// Mask form of one spin part's adjoint (ops applied in order as in
+// ops_adjoint: a creator needs its orbital occupied and clears it, an
+// annihilator needs it empty and sets it).  Op j adds popc(m_j & below_j)
+// with m_j = m ^ (bits of the ops before j); mod 2 that is popc(m & below_j)
+// + popc(flipped_before_j & below_j), so the parity is popc(m & x) + c0 with
+// x = XOR of the below_j masks and c0 a constant.  False if an orbital
+// repeats.
+static bool part_fast(const SpinOps& o, uint32_t& set, uint32_t& clear, uint32_t& x, int& c0) {
+  set = clear = x = 0;
+  c0 = 0;
+  uint32_t flipped = 0;
+  for (int j = 0; j < o.n; ++j) {
+    const uint32_t bit = 1u << o.orb[j];
+    if (flipped & bit) return false;
+    x ^= bit - 1u;
+    c0 += __builtin_popcount(flipped & (bit - 1u));
+    flipped |= bit;
+    (o.cre[j] ? set : clear) |= bit;
+  }
+  return true;
+}
+
+static const TermFast* term_fast(const Program& p) {
+  if (!p.d_term_fast) {
+    std::vector<TermFast> f(std::max<size_t>(p.terms.size(), 1), TermFast{0, 0, 0, 0, 0, 0, 0u, 0u, 0.0});
+    for (size_t t = 0; t < p.terms.size(); ++t) {
+      TermFast& F = f[t];
+      F.coef = p.terms[t].coef;
+      int ca0 = 0, cb0 = 0;
+      if (part_fast(p.terms[t].a, F.sa, F.ca, F.xa, ca0) && part_fast(p.terms[t].b, F.sb, F.cb, F.xb, cb0))
+        F.flags = 1u | ((uint32_t)((ca0 + cb0) & 1) << 1);
+      else
+        F = TermFast{0, 0, 0, 0, 0, 0, 0u, 0u, p.terms[t].coef};
+    }
+    TermFast* d = nullptr;
+    SQ_HIP(hipMalloc(&d, sizeof(TermFast) * f.size()));
+    SQ_HIP(hipMemcpy(d, f.data(), sizeof(TermFast) * f.size(), hipMemcpyHostToDevice));
+    p.d_term_fast = d;
+  }
+  return p.d_term_fast;
+}
+
 // Term groups of the split T application: enough (I, group) threads to fill
 // the GPU for small spaces, at most 64 groups.
 static int term_groups(const Space& s) {
@@ -208,7 +270,8 @@ static void tapply(const Space& s, const Program& p, const double* w, const doub
   const int G = term_groups(s);
   const dim3 grid((s.nsb + kUB - 1) / kUB, s.nsa, G);
   SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
-          k_tapply_part<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, w, (int)p.terms.size(), G, cur, part));
+          k_tapply_part<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), w, (int)p.terms.size(), G, cur,
+                                              part));
   k_tapply_sum<<<(unsigned)((s.dim + kUB - 1) / kUB), kUB, 0, st>>>(part, G, s.dim, f, nxt, acc);
   SQ_HIP(hipGetLastError());
 }
@@ -344,7 +407,7 @@ static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, do
       expm_apply(s, p, u, nu, -ds, u.C, u.C, st);
       expm_apply(s, p, u, nu, -ds, u.D, u.D, st);
       prev = node;
-      k_gate_dots<<<dgrid, kUB, 0, st>>>(sv(s), p.d_terms, p.d_gates, ng, u.C, u.D, u.dpart);
+      k_gate_dots<<<dgrid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), p.d_gates, ng, u.C, u.D, u.dpart);
       k_gate_accum<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(u.dpart, nb, ng, 2.0 * kGLw[q] / m, grad_dev);
       SQ_HIP(hipGetLastError());
     }
