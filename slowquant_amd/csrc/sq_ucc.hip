@@ -384,14 +384,15 @@ static const double kGLw[8] = {0.05061426814518813, 0.11119051722668724, 0.15685
 //   d psi / d theta_i = int_0^1 e^{sT} T_i e^{(1-s)T} psi0 ds, and with
 //   e^{sT}^T = e^{-sT} (T real antisymmetric)
 //   grad_i = 2 int_0^1 <e^{-sT} lam | T_i | e^{-sT} psi> ds,
-// by composite 8-point Gauss-Legendre over m subintervals with 2 nu / m <= 1
-// (the integrand's frequencies are bounded by 2 ||T||_2; the quadrature
-// error is then below 1e-14 relative).  psi and lam are propagated from node
+// by composite 8-point Gauss-Legendre over m subintervals of length h with
+// h * 2 nu <= 4 (the integrand is entire with frequencies bounded by
+// 2 ||T||_2 = omega; the 8-point rule's error is below
+// h^17 (8!)^4 / (17 (16!)^3) (h omega)^16 max|f|, i.e. < 3e-14 relative).  psi and lam are propagated from node
 // to node (C, D); every node adds all gates' dots in one pass.
 static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, double nu, const double* psi,
                           const double* lam, double* grad_dev, hipStream_t st) {
   const int ng = (int)p.gates.size();
-  const int m = std::max(1, (int)std::ceil(2.0 * nu));
+  const int m = std::max(1, (int)std::ceil(0.5 * nu));
   copy_vec(psi, u.C, s.dim, st);
   copy_vec(lam, u.D, s.dim, st);
   k_zero<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(grad_dev, ng);
