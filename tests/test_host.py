@@ -225,3 +225,60 @@ def test_wavefunction_construction_on_cpu():
     wf2 = WaveFunctionUPS((2, 2), np.eye(12), gen, "fUCCSD", {})
     assert (wf2.num_inactive_orbs, wf2.num_active_orbs, wf2.num_virtual_orbs) == (5, 2, 5)
     assert len(wf2.kappa_idx) == 12 * 11 // 2 - (5 * 4 // 2) * 2 - 1
+
+
+def test_term_adjoint_mask_form_matches_operator_loop():
+    """The mask form of a term's adjoint used by the UCC kernels (sq_ucc.hip,
+    part_fast / term_adjoint): for a spin part touching each orbital at most
+    once, applying its operators in order (a creator needs its orbital
+    occupied and clears it, an annihilator needs it empty and sets it, each
+    adding popc(m & below(orbital)) to the parity) is the same as: applies iff
+    m holds every creator orbital and no annihilator orbital; image m ^ s ^ c;
+    parity popc(m & x) + c0, x the XOR of the below-masks, c0 the sum of
+    popc(flipped-so-far & below).  Brute force over random strings."""
+    import random
+
+    def loop(ops, m):
+        par = 0
+        for orb, cre in ops:
+            bit = 1 << orb
+            par += bin(m & (bit - 1)).count("1")
+            if cre:
+                if not m & bit:
+                    return None
+                m &= ~bit
+            else:
+                if m & bit:
+                    return None
+                m |= bit
+        return m, par
+
+    def mask_form(ops):
+        s = c = x = c0 = flipped = 0
+        for orb, cre in ops:
+            bit = 1 << orb
+            x ^= bit - 1
+            c0 += bin(flipped & (bit - 1)).count("1")
+            flipped |= bit
+            if cre:
+                s |= bit
+            else:
+                c |= bit
+        return s, c, x, c0
+
+    rng = random.Random(7)
+    applied = 0
+    for _ in range(50000):
+        orbs = rng.sample(range(14), rng.randint(0, 4))
+        ops = [(o, rng.randint(0, 1)) for o in orbs]
+        m = rng.getrandbits(14)
+        ref = loop(ops, m)
+        s, c, x, c0 = mask_form(ops)
+        passes = (m & s) == s and not m & c
+        assert passes == (ref is not None)
+        if ref is None:
+            continue
+        applied += 1
+        assert ref[0] == m ^ s ^ c
+        assert ref[1] % 2 == (bin(m & x).count("1") + c0) % 2
+    assert applied > 10000
