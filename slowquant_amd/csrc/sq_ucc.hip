@@ -66,10 +66,11 @@ __device__ __forceinline__ bool term_adjoint(const TermFast& F, const Term* __re
 __global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const TermFast* __restrict__ fast,
                               const double* __restrict__ w, int nt, int G, const double* __restrict__ cur,
                               double* __restrict__ part) {
-  const int ia = blockIdx.y;
-  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ib >= v.nsb) return;
-  const int g = blockIdx.z;
+  const size_t dim = (size_t)v.nsa * v.nsb;
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // determinants flattened: every lane works
+  if (I >= dim) return;
+  const int ia = (int)(I / (uint32_t)v.nsb), ib = (int)(I - (size_t)ia * v.nsb);
+  const int g = blockIdx.y;
   const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
   double val = 0.0;
   for (int t = g; t < nt; t += G) {
@@ -82,8 +83,7 @@ __global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const TermFa
     const double c = (w ? w[t] : 1.0) * F.coef;
     val += ((par & 1) ? -c : c) * cur[(size_t)ra * v.nsb + rb];
   }
-  const size_t dim = (size_t)v.nsa * v.nsb;
-  part[(size_t)g * dim + (size_t)ia * v.nsb + ib] = val;
+  part[(size_t)g * dim + I] = val;
 }
 
 // nxt[I] = f * sum_g part[g][I]; acc[I] += nxt[I] when acc != nullptr.
@@ -100,19 +100,22 @@ __global__ void k_tapply_sum(const double* __restrict__ part, int G, int64_t dim
 
 // Per gate i, per block b of amplitudes: dpart[b*ng + i] = sum over the
 // block's I of v[I] (T_i u)[I] with T_i the gate's generator (unit weight).
-// Block (x, y, z) takes the gates i = z, z + gridDim.z, ...; every (b, i) is
-// written by exactly one block and summed over b in a fixed order later.
+// Block (b, y) covers determinants [256 b, 256 b + 256) and takes the gates
+// i = y, y + gridDim.y, ...; every (b, i) is written by exactly one block and
+// summed over b in a fixed order later.
 __global__ void k_gate_dots(SV sv, const Term* __restrict__ terms, const TermFast* __restrict__ fast,
                             const GateDev* __restrict__ gates, int ng, const double* __restrict__ u,
                             const double* __restrict__ v, double* __restrict__ dpart) {
   __shared__ double red[kUB / 64];
-  const int ia = blockIdx.y;
-  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool ok = ib < sv.nsb;
-  const int b = blockIdx.y * gridDim.x + blockIdx.x;
-  const uint32_t a0 = sv.astr[ia], b0 = ok ? sv.bstr[ib] : 0u;
-  const double vI = ok ? v[(size_t)ia * sv.nsb + ib] : 0.0;
-  for (int i = blockIdx.z; i < ng; i += gridDim.z) {
+  const size_t dim = (size_t)sv.nsa * sv.nsb;
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // determinants flattened
+  const bool ok = I < dim;
+  const int ia = ok ? (int)(I / (uint32_t)sv.nsb) : 0;
+  const int ib = ok ? (int)(I - (size_t)ia * sv.nsb) : 0;
+  const int b = blockIdx.x;
+  const uint32_t a0 = sv.astr[ia], b0 = sv.bstr[ib];
+  const double vI = ok ? v[I] : 0.0;
+  for (int i = blockIdx.y; i < ng; i += gridDim.y) {
     double val = 0.0;
     if (ok && vI != 0.0) {
       for (int t = gates[i].term_begin; t < gates[i].term_end; ++t) {
@@ -224,10 +227,10 @@ static const TermFast* term_fast(const Program& p) {
   return p.d_term_fast;
 }
 
-// Term groups of the split T application: enough (I, group) threads to fill
-// the GPU for small spaces, at most 64 groups.
+// Term groups of the split T application: enough (determinant, group)
+// threads to fill the GPU for small spaces, at most 64 groups.
 static int term_groups(const Space& s) {
-  const int64_t G = 65536 / std::max<int64_t>(s.dim, 1);
+  const int64_t G = 262144 / std::max<int64_t>(s.dim, 1);
   return (int)std::max<int64_t>(1, std::min<int64_t>(64, G));
 }
 
@@ -236,7 +239,7 @@ struct UccWork {
   double *A, *B, *C, *D, *w, *part, *dpart, *partial, *scal;
 };
 
-static int64_t dot_blocks(const Space& s) { return (int64_t)((s.nsb + kUB - 1) / kUB) * s.nsa; }
+static int64_t dot_blocks(const Space& s) { return (s.dim + kUB - 1) / kUB; }
 
 static int gate_groups(const Space& s, int ng) {
   const int64_t nb = dot_blocks(s);
@@ -268,7 +271,7 @@ static UccWork carve_ucc(const Space& s, const Program& p, double* work) {
 static void tapply(const Space& s, const Program& p, const double* w, const double* cur, double f, double* nxt,
                    double* acc, double* part, hipStream_t st) {
   const int G = term_groups(s);
-  const dim3 grid((s.nsb + kUB - 1) / kUB, s.nsa, G);
+  const dim3 grid((unsigned)((s.dim + kUB - 1) / kUB), G);
   SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
           k_tapply_part<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), w, (int)p.terms.size(), G, cur,
                                               part));
@@ -397,9 +400,8 @@ static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, do
   copy_vec(lam, u.D, s.dim, st);
   k_zero<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(grad_dev, ng);
   SQ_HIP(hipGetLastError());
-  const int nbx = (s.nsb + kUB - 1) / kUB;
-  const int nb = nbx * s.nsa;
-  const dim3 dgrid(nbx, s.nsa, gate_groups(s, ng));
+  const int nb = (int)dot_blocks(s);
+  const dim3 dgrid(nb, gate_groups(s, ng));
   double prev = 0.0;
   for (int j = 0; j < m; ++j) {
     for (int q = 0; q < 8; ++q) {
