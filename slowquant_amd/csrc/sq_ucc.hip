@@ -42,9 +42,6 @@ __device__ __forceinline__ bool ops_adjoint(const SpinOps& o, uint32_t& m, int& 
   return true;
 }
 
-// part[g][I] = sum over the terms t = g, g + G, ... of w[t] <I|O_t|J> cur[J]
-// (w = nullptr: unit weights).  Splitting the terms over G groups keeps
-// small spaces' grids full; k_tapply_sum adds the groups in a fixed order.
 // Adjoint of term t on strings (a0, b0): images (a, b) and sign parity;
 // false if it does not apply.
 __device__ __forceinline__ bool term_adjoint(const TermFast& F, const Term* __restrict__ terms, int t, uint32_t a0,
@@ -63,15 +60,61 @@ __device__ __forceinline__ bool term_adjoint(const TermFast& F, const Term* __re
   return ops_adjoint(T.a, a, par) && ops_adjoint(T.b, b, par);
 }
 
-__global__ void k_tapply_part(SV v, const Term* __restrict__ terms, const TermFast* __restrict__ fast,
-                              const double* __restrict__ w, int nt, int G, const double* __restrict__ cur,
-                              double* __restrict__ part) {
+// part[y][I] = sum over the terms t = g, g + G, ... of w[t] <I|O_t|J> cur[J]
+// summed over the groups g = y*nw + wave of one block (w = nullptr: unit
+// weights).  A block is nw = min(G, kTG) waves over the same 64 determinants
+// (lane = determinant, flattened across rows), one term group per wave; the
+// waves' sums are added in LDS in wave order.  Splitting the terms over G groups
+// keeps small spaces' grids full; k_tapply_sum adds the part rows in order.
+constexpr int kTG = 8;
+
+__global__ __launch_bounds__(64 * kTG) void k_tapply_part(SV v, const Term* __restrict__ terms,
+                                                         const TermFast* __restrict__ fast,
+                                                         const double* __restrict__ w, int nt, int G,
+                                                         const double* __restrict__ cur, double* __restrict__ part) {
+  __shared__ double red[kTG][64];
   const size_t dim = (size_t)v.nsa * v.nsb;
-  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;  // determinants flattened: every lane works
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  const size_t I = (size_t)blockIdx.x * 64 + lane;
+  const int g = blockIdx.y * nw + wave;
+  double val = 0.0;
+  if (I < dim && g < G) {
+    const int ia = (int)(I / (uint32_t)v.nsb), ib = (int)(I - (size_t)ia * v.nsb);
+    const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
+    for (int t = g; t < nt; t += G) {
+      const TermFast F = fast[t];
+      uint32_t a, b;
+      int par;
+      if (!term_adjoint(F, terms, t, a0, b0, a, b, par)) continue;
+      const int ra = v.arank[a], rb = v.brank[b];
+      if (ra < 0 || rb < 0) continue;
+      const double c = (w ? w[t] : 1.0) * F.coef;
+      val += ((par & 1) ? -c : c) * cur[(size_t)ra * v.nsb + rb];
+    }
+  }
+  red[wave][lane] = val;
+  __syncthreads();
+  if (wave == 0 && I < dim) {
+    double s = 0.0;
+    for (int q = 0; q < nw; ++q) s += red[q][lane];
+    part[(size_t)blockIdx.y * dim + I] = s;
+  }
+}
+
+// part[g][I] for one term group g = blockIdx.y per block (all waves of a
+// block on the same term stream, so they share its scalar-cache lines); used
+// when there are too few groups to fold (large spaces).
+__global__ __launch_bounds__(kUB) void k_tapply_group(SV v, const Term* __restrict__ terms,
+                                                     const TermFast* __restrict__ fast,
+                                                     const double* __restrict__ w, int nt, int G,
+                                                     const double* __restrict__ cur, double* __restrict__ part) {
+  const size_t dim = (size_t)v.nsa * v.nsb;
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (I >= dim) return;
   const int ia = (int)(I / (uint32_t)v.nsb), ib = (int)(I - (size_t)ia * v.nsb);
-  const int g = blockIdx.y;
   const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
+  const int g = blockIdx.y;
   double val = 0.0;
   for (int t = g; t < nt; t += G) {
     const TermFast F = fast[t];
@@ -271,11 +314,20 @@ static UccWork carve_ucc(const Space& s, const Program& p, double* work) {
 static void tapply(const Space& s, const Program& p, const double* w, const double* cur, double f, double* nxt,
                    double* acc, double* part, hipStream_t st) {
   const int G = term_groups(s);
-  const dim3 grid((unsigned)((s.dim + kUB - 1) / kUB), G);
-  SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
-          k_tapply_part<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), w, (int)p.terms.size(), G, cur,
-                                              part));
-  k_tapply_sum<<<(unsigned)((s.dim + kUB - 1) / kUB), kUB, 0, st>>>(part, G, s.dim, f, nxt, acc);
+  int rows = G;  // part rows
+  if (G >= kTG) {  // small space: fold kTG groups per block in LDS
+    rows = (G + kTG - 1) / kTG;
+    const dim3 grid((unsigned)((s.dim + 63) / 64), rows);
+    SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
+            k_tapply_part<<<grid, 64 * kTG, 0, st>>>(sv(s), p.d_terms, term_fast(p), w, (int)p.terms.size(), G,
+                                                      cur, part));
+  } else {
+    const dim3 grid((unsigned)((s.dim + kUB - 1) / kUB), G);
+    SQ_PROF(PC_TAPPLY, st, 24.0 * (double)s.dim, 0.0,
+            k_tapply_group<<<grid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), w, (int)p.terms.size(), G, cur,
+                                                 part));
+  }
+  k_tapply_sum<<<(unsigned)((s.dim + kUB - 1) / kUB), kUB, 0, st>>>(part, rows, s.dim, f, nxt, acc);
   SQ_HIP(hipGetLastError());
 }
 
