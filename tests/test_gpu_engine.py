@@ -192,6 +192,61 @@ def test_sigma_parity(n, na, nb):
     assert abs(e - float(psi_ref @ expect)) < 1e-10
 
 
+@pytest.mark.parametrize("n,na,nb", [(2, 2, 2), (3, 1, 0), (4, 2, 0)])
+def test_degenerate_spaces(n, na, nb):
+    """Spaces with one determinant (both spins full) or one beta string (no
+    beta electrons): tUPS state, energy, adjoint and reference-sweep
+    gradients and RDMs against the oracle; with one determinant every gate
+    acts trivially (gradient 0)."""
+    from oracle.apply import expectation_value
+    from oracle.ci import get_indexing
+    from oracle.rdm import rdm1, rdm2
+    from oracle.ups import construct_ups_state, gradient_reference
+
+    E = _engine()
+    lay = _tups(n, 1)
+    h, g = _hamiltonian(n, 2 * n, 21)
+    theta = np.random.default_rng(6).normal(0, 0.3, lay.n_params)
+    ci = get_indexing(0, n, 0, na, nb)
+    Hop = _oracle_H(h, g, n)
+    csf = np.zeros(len(ci.idx2det))
+    csf[0] = 1.0
+    psi_ref = construct_ups_state(csf, ci, theta, lay)
+    e_ref = expectation_value(psi_ref, [Hop], psi_ref, ci, do_folding=False)
+    g_ref = gradient_reference(Hop, csf, ci, theta, lay)
+    sp = E.Space(n, na, nb)
+    assert sp.dim == len(ci.idx2det)
+    prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    H = E.Hamiltonian(sp, h, g)
+    psi0 = sp.from_reference(csf)
+    assert np.abs(sp.to_reference_numpy(prog.state_prep(theta, psi0)) - psi_ref).max() < 1e-12
+    for method in ("adjoint", "reference"):
+        e, grad = prog.energy_grad(H, theta, psi0, method)
+        assert abs(e - e_ref) < 1e-10, method
+        assert np.abs(grad - g_ref).max() < 1e-10, method
+    if sp.dim == 1:
+        assert np.abs(g_ref).max() < 1e-14
+    r1, r2 = E.rdm12(sp, sp.from_reference(psi_ref))
+    e1 = rdm1(psi_ref, ci)
+    assert np.abs(r1 - e1).max() < 1e-12
+    assert np.abs(r2 - rdm2(psi_ref, ci, e1)).max() < 1e-12
+
+
+def test_empty_program():
+    """A program with no gates: the state is psi0 and the gradient is empty."""
+    E = _engine()
+    h, g = _hamiltonian(4, 8, 3)
+    sp = E.Space(4, 2, 2)
+    H = E.Hamiltonian(sp, h, g)
+    prog = E.Program(sp, [], [])
+    psi0 = sp.from_reference(_rand_state(sp.dim, 2))
+    out = prog.state_prep(np.zeros(0), psi0)
+    assert np.array_equal(out.cpu().numpy(), psi0.cpu().numpy())
+    e, grad = prog.energy_grad(H, np.zeros(0), psi0, "adjoint")
+    _sig, e_ref = H.sigma(psi0)
+    assert abs(e - e_ref) < 1e-12 and grad.shape == (0,)
+
+
 @pytest.mark.parametrize("n,layers", [(4, 1), (6, 1)])
 def test_gradients_parity(n, layers):
     from oracle.ci import get_indexing
