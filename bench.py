@@ -11,9 +11,12 @@ gradient dots).  Inputs are resident in HBM before timing.
 
 The JSON line also carries:
   plain           evals/s of per-parameter derivative propagation (O(N^2))
-  roofline        dominant kernel: algorithmic bytes or flops per launch over
-                  its average duration, timed with HIP events on the engine's
-                  stream in a separate profiled pass over the same workload
+  kernels         per-class kernel table (HIP events on each kernel's
+                  stream) from a profiled pass over the same workload
+  roofline        dominant kernel of that table: algorithmic bytes or flops
+                  per launch over its average duration, timed with HIP events
+                  on its stream over the timed steps themselves (only that
+                  kernel is bracketed there: one event pair per launch)
   rotation        the forward sweep by per-gate rotation kernels (HBM
                   fraction they reach) and by the fused tile stages (the
                   per-gate sweep's bytes over the fused sweep's time)
@@ -136,13 +139,6 @@ def rotation_profile(sp, lay, prog, theta, psi0):
             if fu_ms > 0 else None,
         },
     }
-
-
-def kernel_profile(sp, prog, H, theta, psi0, reps):
-    """Average per-launch duration of each kernel class via HIP events."""
-    from slowquant_amd import profiling
-
-    return profiling.profile_energy_grad(sp, prog, H, theta, psi0, reps)
 
 
 def cpu_baseline(h, g, theta, lay, budget_s=20.0):
@@ -442,6 +438,8 @@ def main() -> int:
     if args.workload == "ucc":
         print(json.dumps(bench_ucc(args)), flush=True)
         return 0
+    from slowquant_amd import profiling
+
     lay, h, g, theta, sp, prog, H, psi0 = build_workload()
     th = np.asarray(theta)
 
@@ -449,14 +447,17 @@ def main() -> int:
         return prog.energy_grad(H, th, psi0, "adjoint")
 
     e_adj, g_adj = step()
-    t_adj = time_steps(step, args.steps, args.warmup)
+    # per-class kernel table from a profiled pass; the timed steps then
+    # bracket only its dominant kernel with events (for the roofline)
+    kernels = profiling.profile(step, 3)
+    dom_name = profiling.dominant(kernels)["name"]
+    t_adj, timed = profiling.timed_profile(step, args.steps, args.warmup, (dom_name,))
     t_plain = grad_diff = None
     if args.plain_steps > 0:
         t_plain = time_steps(lambda: prog.energy_grad(H, th, psi0, "plain"), args.plain_steps, 0)
         e_pl, g_pl = prog.energy_grad(H, th, psi0, "plain")
         grad_diff = float(np.abs(g_adj - g_pl).max())
-    prof = kernel_profile(sp, prog, H, th, psi0, reps=3)
-    dom = prof["dominant"]
+    dom = profiling.dominant(timed)
     roof = {
         "bound": dom["bound"],
         "achieved": dom["achieved"],
@@ -492,7 +493,7 @@ def main() -> int:
         "max_grad_diff_adjoint_vs_plain": grad_diff,
         "energy": e_adj,
         "roofline": roof,
-        "kernels": prof["kernels"],
+        "kernels": kernels,
         "rotation": rotation_profile(sp, lay, prog, th, psi0),
     }
     if not args.no_cpu:

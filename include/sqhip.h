@@ -222,6 +222,9 @@ int sq_operator_apply(const sq_space* s, const sq_operator* op, const double* in
  * stream; collect returns, per kernel class, the summed duration (ms), the
  * launch count and the algorithmic bytes and flops of those launches. */
 int sq_profile_enable(int on);
+/* Kernel classes timed while enabled: bit c of class_mask for class c
+ * (default all).  Timing one class adds one event pair per launch of it. */
+int sq_profile_select(uint32_t class_mask);
 int sq_profile_nclasses(void);
 const char* sq_profile_class_name(int cls);
 int sq_profile_collect(int n_classes, double* total_ms, int64_t* count, double* bytes, double* flops);

@@ -101,6 +101,7 @@ _SIGNATURES = {
     "sq_batch_run_graph": ([vp, vp], i32),
     "sq_batch_results": ([vp, vp, vp, vp], i32),
     "sq_profile_enable": ([i32], i32),
+    "sq_profile_select": ([ctypes.c_uint32], i32),
     "sq_profile_nclasses": ([], i32),
     "sq_profile_class_name": ([i32], ctypes.c_char_p),
     "sq_profile_collect": ([i32, vp, vp, vp, vp], i32),

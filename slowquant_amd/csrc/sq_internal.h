@@ -176,12 +176,12 @@ enum ProfClass {
   kProfClasses
 };
 bool prof_on();
-void prof_begin(hipStream_t st);
+void prof_begin(int cls, hipStream_t st);
 void prof_end(int cls, hipStream_t st, double bytes, double flops);
 
 #define SQ_PROF(cls, st, bytes, flops, ...)          \
   do {                                               \
-    ::sq::prof_begin(st);                            \
+    ::sq::prof_begin((cls), (st));                   \
     __VA_ARGS__;                                     \
     SQ_HIP(hipGetLastError());                       \
     ::sq::prof_end((cls), (st), (bytes), (flops));   \

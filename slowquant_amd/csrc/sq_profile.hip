@@ -18,6 +18,7 @@ struct Rec {
 };
 struct Prof {
   bool on = false;
+  uint32_t mask = 0xffffffffu;  // kernel classes to time (sq_profile_select)
   std::vector<Rec> recs;
   std::vector<hipEvent_t> pool;
   std::mutex mu;
@@ -45,9 +46,9 @@ const char* kClassNames[kProfClasses] = {"rotate", "rotate_adjoint", "t_apply", 
 
 bool prof_on() { return prof().on; }
 
-void prof_begin(hipStream_t st) {
+void prof_begin(int cls, hipStream_t st) {
   Prof& P = prof();
-  if (!P.on) return;
+  if (!P.on || !((P.mask >> cls) & 1u)) return;
   std::lock_guard<std::mutex> g(P.mu);
   t_start = P.get();
   SQ_HIP(hipEventRecord(t_start, st));
@@ -55,7 +56,7 @@ void prof_begin(hipStream_t st) {
 
 void prof_end(int cls, hipStream_t st, double bytes, double flops) {
   Prof& P = prof();
-  if (!P.on) return;
+  if (!P.on || !((P.mask >> cls) & 1u)) return;
   std::lock_guard<std::mutex> g(P.mu);
   hipEvent_t b = P.get();
   SQ_HIP(hipEventRecord(b, st));
@@ -103,6 +104,14 @@ int sq_profile_collect(int n_classes, double* total_ms, int64_t* count, double* 
 }
 
 const char* sq_profile_class_name(int cls) { return (cls >= 0 && cls < kProfClasses) ? kClassNames[cls] : ""; }
+
+int sq_profile_select(uint32_t class_mask) {
+  return guarded([&] {
+    Prof& P = prof();
+    std::lock_guard<std::mutex> g(P.mu);
+    P.mask = class_mask;
+  });
+}
 
 int sq_profile_nclasses(void) { return kProfClasses; }
 

@@ -60,6 +60,44 @@ def profile(fn, reps: int = 3) -> dict:
     return kernels
 
 
+def class_index(name: str) -> int:
+    L = N.lib()
+    for c in range(L.sq_profile_nclasses()):
+        if L.sq_profile_class_name(c).decode() == name:
+            return c
+    raise ValueError(f"unknown kernel class {name}")
+
+
+def timed_profile(fn, steps: int, warmup: int, classes: tuple[str, ...] | None = None) -> tuple[float, dict]:
+    """Seconds per call of fn() over ``steps`` calls after ``warmup`` untimed
+    ones, with the launches of the kernel ``classes`` (all when None) in the
+    timed calls bracketed by HIP events on their own stream; returns
+    (seconds per call, per-class kernel table of the timed classes)."""
+    import time
+
+    L = N.lib()
+    mask = 0xFFFFFFFF if classes is None else sum(1 << class_index(c) for c in classes)
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    N.check(L.sq_profile_select(mask))
+    N.check(L.sq_profile_enable(1))
+    try:
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            fn()
+        torch.cuda.synchronize()
+        secs = (time.perf_counter() - t0) / steps
+        kernels = collect()
+    finally:
+        N.check(L.sq_profile_enable(0))
+        N.check(L.sq_profile_select(0xFFFFFFFF))
+    for k in kernels.values():
+        k["launches_per_call"] = k["launches"] / steps
+        k["ms_per_call"] = k["total_ms"] / steps
+    return secs, kernels
+
+
 def dominant(kernels: dict) -> dict:
     name = max(kernels, key=lambda k: kernels[k]["total_ms"])
     k = kernels[name]
