@@ -422,7 +422,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1:
         print(
-            json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+            json.dumps({"error": "this engine runs on one GPU; a run over several GPUs is not supported",
                         "n_gpus": args.gpus}),
             flush=True,
         )
