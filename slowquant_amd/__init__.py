@@ -1,6 +1,6 @@
 """MI355X-native state-vector engine for SlowQuant's UPS and UCC wave functions.
 
-Drop-in classes: WaveFunctionUPS, WaveFunctionUCC (same constructors and
+Drop-in classes: WaveFunctionUPS, WaveFunctionSAUPS, WaveFunctionUCC (same constructors and
 entry points as the reference); module functions with the reference's
 signatures live in slowquant_amd.operator_state_algebra.  All state-vector
 arithmetic runs in the in-tree HIP library libsqhip.so (gfx950).
@@ -17,6 +17,10 @@ def __getattr__(name):
         from .ups_wavefunction import WaveFunctionUPS
 
         return WaveFunctionUPS
+    if name == "WaveFunctionSAUPS":
+        from .sa_ups_wavefunction import WaveFunctionSAUPS
+
+        return WaveFunctionSAUPS
     if name == "WaveFunctionUCC":
         from .ucc_wavefunction import WaveFunctionUCC
 
@@ -24,4 +28,5 @@ def __getattr__(name):
     raise AttributeError(name)
 
 
-__all__ = ["CI_Info", "FermionicOperator", "UccStructure", "UpsStructure", "WaveFunctionUCC", "WaveFunctionUPS", "get_indexing"]
+__all__ = ["CI_Info", "FermionicOperator", "UccStructure", "UpsStructure", "WaveFunctionSAUPS", "WaveFunctionUCC", "WaveFunctionUPS",
+           "get_indexing"]

@@ -84,6 +84,35 @@ class _Integrals:
         self.kinetic_energy_matrix = T
         self.nuclear_attraction_matrix = V
         self.electron_repulsion_tensor = self._eri(coords, alpha, d)
+        self._coords = np.asarray(coords, dtype=np.float64)
+        self._alpha = alpha
+        self._d = d
+
+    def get_multipole_matrix(self, carts: np.ndarray) -> np.ndarray:
+        """<a| x^i y^j z^k |b> about the origin for carts = (i, j, k) with
+        i + j + k <= 1 (overlap or one dipole component; the front end's
+        electric-dipole integrals, integral_manager.py:95-104).  For 1s
+        primitives the product a(r) b(r) is a Gaussian centred at
+        P = (alpha_a A + alpha_b B) / (alpha_a + alpha_b), so <a|r|b> = s_ab P."""
+        carts = np.asarray(carts, dtype=int)
+        if carts.shape != (3,) or carts.min() < 0 or carts.sum() > 1:
+            raise ValueError("HydrogenSlowQuant provides overlap and dipole integrals only")
+        if carts.sum() == 0:
+            return self.overlap_matrix.copy()
+        k = int(np.argmax(carts))
+        coords, alpha, d = self._coords, self._alpha, self._d
+        n = len(coords)
+        M = np.zeros((n, n))
+        for i in range(n):
+            for j in range(n):
+                Rab2 = float(np.sum((coords[i] - coords[j]) ** 2))
+                for a in range(3):
+                    for b in range(3):
+                        p = alpha[a] + alpha[b]
+                        s_ab = (np.pi / p) ** 1.5 * math.exp(-alpha[a] * alpha[b] / p * Rab2)
+                        P = (alpha[a] * coords[i, k] + alpha[b] * coords[j, k]) / p
+                        M[i, j] += d[a] * d[b] * s_ab * P
+        return M
 
     @staticmethod
     def _eri(coords, alpha, d):

@@ -23,7 +23,7 @@ def two_electron_integral_transform(C: np.ndarray, int2e: np.ndarray) -> np.ndar
 
 
 class IntegralManager:
-    __slots__ = ("_eri", "_h_ao", "_kin", "_nuc", "int_obj")
+    __slots__ = ("_dip", "_eri", "_h_ao", "_kin", "_nuc", "int_obj")
 
     def __init__(self, integral_obj) -> None:
         self.int_obj = copy.deepcopy(integral_obj)
@@ -31,6 +31,7 @@ class IntegralManager:
         self._nuc = None
         self._eri = None
         self._h_ao = None
+        self._dip = None
 
     def _is_slowquant(self) -> bool:
         return hasattr(self.int_obj, "molecule") and hasattr(self.int_obj, "integral")
@@ -89,6 +90,20 @@ class IntegralManager:
         if self._is_mole():
             return self.int_obj.energy_nuc()
         raise self._unknown()
+
+    @property
+    def electric_dipole(self) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+        """AO dipole integrals (x, y, z) (integral_manager.py:94-111)."""
+        if self._dip is None:
+            if self._is_slowquant():
+                self._dip = tuple(self.int_obj.integral.get_multipole_matrix(np.array(c))
+                                  for c in ((1, 0, 0), (0, 1, 0), (0, 0, 1)))
+            elif self._is_mole():
+                x, y, z = self.int_obj.intor("int1e_r", comp=3)
+                self._dip = (x, y, z)
+            else:
+                raise self._unknown()
+        return self._dip
 
     @property
     def h_ao(self) -> np.ndarray:

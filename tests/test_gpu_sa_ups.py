@@ -1,0 +1,154 @@
+"""GPU checks of WaveFunctionSAUPS (state-averaged UPS).
+
+Known answers from the reference's own tests (tests/test_unitary_product_state.py:156-242:
+H2 and H3+ in STO-3G, excitation energies to 1e-6, oscillator strengths to
+1e-3), and parity with the oracle on linear H4 (4e,4o), 1.5 Angstrom, three
+reference states: states 1e-12, averaged energy and subspace energies 1e-10,
+averaged gradient 1e-10 (and 1e-8 against central differences), averaged
+RDMs 1e-12.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+R = 2 ** (-1 / 2)
+H4_STATES = ([[1], [R, -R], [1]], [["11110000"], ["11100100", "11011000"], ["11001100"]])
+
+
+def _molecule(geom, charge=0):
+    from slowquant_amd.hchain import HydrogenSlowQuant
+
+    sq = HydrogenSlowQuant()
+    sq.set_molecule(geom, distance_unit="angstrom", molecular_charge=charge)
+    sq.set_basis_set("STO-3G")
+    sq.init_hartree_fock()
+    sq.hartree_fock.run_restricted_hartree_fock()
+    return sq
+
+
+def test_saups_h2_3states_known_answer():
+    """All three singlet-space states in the subspace, so the subspace
+    diagonalisation reproduces FCI (tests/test_unitary_product_state.py:156-198)."""
+    from slowquant_amd.sa_ups_wavefunction import WaveFunctionSAUPS
+
+    sq = _molecule("H 0.0 0.0 0.0; H 0.0 0.0 0.735")
+    wf = WaveFunctionSAUPS((2, 2), sq.hartree_fock.mo_coeff, sq,
+                           ([[1], [R, -R], [1]], [["1100"], ["1001", "0110"], ["0011"]]),
+                           "tUPS", {"n_layers": 1, "skip_last_singles": True}, include_active_kappa=True)
+    wf.run_wf_optimization_1step("BFGS", True)
+    assert abs(wf.excitation_energies[0] - 0.974553) < 1e-6
+    assert abs(wf.excitation_energies[1] - 1.632364) < 1e-6
+    osc = wf.get_oscillator_strenghts()
+    assert abs(osc[0] - 0.8706) < 1e-3
+    assert abs(osc[1] - 0.0) < 1e-3
+
+
+def test_saups_h3_3states_known_answer():
+    """H3+ (2e,3o), a subspace smaller than the CI space, two-step
+    optimisation (tests/test_unitary_product_state.py:201-242)."""
+    from slowquant_amd.sa_ups_wavefunction import WaveFunctionSAUPS
+
+    sq = _molecule("H -0.45 -0.3897114317 0.0; H 0.45 -0.3897114317 0.0; H 0.0 0.3897114317 0.0", charge=1)
+    wf = WaveFunctionSAUPS((2, 3), sq.hartree_fock.mo_coeff, sq,
+                           ([[1], [R, -R], [R, -R]], [["110000"], ["100100", "011000"], ["100001", "010010"]]),
+                           "tUPS", {"n_layers": 2, "skip_last_singles": True}, include_active_kappa=True)
+    wf.run_wf_optimization_2step("BFGS", True)
+    assert abs(wf.excitation_energies[0] - 0.838466) < 1e-6
+    assert abs(wf.excitation_energies[1] - 0.838466) < 1e-6
+    osc = wf.get_oscillator_strenghts()
+    assert abs(osc[0] - 0.7569) < 1e-3
+    assert abs(osc[1] - 0.7569) < 1e-3
+
+
+def _h4_sa(layers=1):
+    from slowquant_amd.hchain import linear_h_chain
+    from slowquant_amd.sa_ups_wavefunction import WaveFunctionSAUPS
+
+    sq = linear_h_chain(4, 1.5)
+    return WaveFunctionSAUPS((4, 4), sq.hartree_fock.mo_coeff, sq, H4_STATES, "tUPS", {"n_layers": layers})
+
+
+def _oracle(wf):
+    from oracle import fermion as F
+    from oracle.ci import get_indexing
+
+    H = F.hamiltonian_0i_0a(wf.h_mo, wf.g_mo, wf.num_inactive_orbs, wf.num_active_orbs)
+    H = H.folded(wf.num_inactive_orbs, wf.num_active_orbs, wf.num_virtual_orbs)
+    ci = get_indexing(wf.num_inactive_orbs, wf.num_active_orbs, wf.num_virtual_orbs, wf.num_active_elec_alpha,
+                      wf.num_active_elec_beta)
+    return H, ci
+
+
+def test_sa_h4_against_oracle():
+    from oracle.apply import expectation_value
+    from oracle.rdm import rdm1, rdm2
+    from oracle.ups import construct_ups_state, gradient_reference
+
+    wf = _h4_sa()
+    theta = np.random.default_rng(5).normal(0.0, 0.2, wf.ups_layout.n_params)
+    H, ci = _oracle(wf)
+    psis = [construct_ups_state(c, ci, theta, wf.ups_layout) for c in wf.csf_coeffs]
+    wf.thetas = theta.tolist()
+    assert np.abs(wf.ci_coeffs - np.array(psis)).max() < 1e-12
+    M = np.array([[expectation_value(a, [H], b, ci, do_folding=False) for b in psis] for a in psis])
+    e_states = np.diag(M)
+    assert abs(wf.sa_energy - e_states.mean()) < 1e-10
+    wf._old_opt_parameters = np.zeros(len(theta)) + 1e20
+    wf._E_opt_old = 0.0
+    all_e = wf._calc_energy_optimization(theta.tolist(), True, False, return_all_states=True)
+    assert np.abs(all_e - e_states).max() < 1e-10
+    assert np.abs(wf.energy_states - np.linalg.eigvalsh(0.5 * (M + M.T))).max() < 1e-10
+    g_ref = np.mean([gradient_reference(H, c, ci, theta, wf.ups_layout) for c in wf.csf_coeffs], axis=0)
+    for method in ("adjoint", "reference", "plain"):
+        wf.gradient_method = method
+        g = wf._calc_gradient_optimization(theta.tolist(), True, False)
+        assert np.abs(g - g_ref).max() < 1e-10, method
+    r1s = [rdm1(p, ci) for p in psis]
+    r1 = np.mean(r1s, axis=0)
+    r2 = np.mean([rdm2(p, ci, a) for p, a in zip(psis, r1s)], axis=0)
+    assert np.abs(wf.rdm1 - r1).max() < 1e-12
+    assert np.abs(wf.rdm2 - r2).max() < 1e-12
+    # the trace of the averaged 1-RDM counts the active electrons
+    assert abs(np.trace(wf.rdm1) - wf.num_active_elec) < 1e-12
+    # RotoSolve's shifted energies are sums over the states
+    diffs = [-0.3, 0.1, 0.7]
+    vals = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, 2)
+    for d, v in zip(diffs, vals):
+        t = theta.copy()
+        t[2] = d
+        ref = sum(expectation_value(q, [H], q, ci, do_folding=False)
+                  for q in (construct_ups_state(c, ci, t, wf.ups_layout) for c in wf.csf_coeffs))
+        assert abs(v - ref) < 1e-10
+
+
+def test_sa_h4_gradient_vs_central_differences():
+    wf = _h4_sa(layers=2)
+    theta = np.random.default_rng(6).normal(0.0, 0.1, wf.ups_layout.n_params)
+
+    def energy(t):
+        wf._old_opt_parameters = np.zeros(len(t)) + 1e20
+        return wf._calc_energy_optimization(list(t), True, False)
+
+    g = wf._calc_gradient_optimization(theta.tolist(), True, False)
+    h = 1e-5
+    fd = np.zeros_like(theta)
+    for i in range(len(theta)):
+        tp = theta.copy()
+        tp[i] += h
+        tm = theta.copy()
+        tm[i] -= h
+        fd[i] = (energy(tp) - energy(tm)) / (2 * h)
+    assert np.abs(fd - g).max() < 1e-8
+
+
+def test_sa_h4_rotosolve_lowers_the_averaged_energy():
+    wf = _h4_sa()
+    e0 = wf.sa_energy
+    wf.run_wf_optimization_1step("RotoSolve", False, maxiter=3)
+    assert wf.sa_energy < e0 - 1e-3
+    # the subspace energies average to the state-averaged energy
+    assert abs(np.mean(wf.energy_states) - wf.sa_energy) < 1e-10
