@@ -184,6 +184,14 @@ int sq_batch_destroy(sq_batch* b);
 int64_t sq_batch_num_params(const sq_batch* b);
 /* theta: host array, instances' parameters concatenated in instance order. */
 int sq_batch_set_theta(sq_batch* b, const double* theta, void* stream);
+/* Start state of every instance (construct_ups_state_SA from arbitrary
+ * reference states, operator_state_algebra.py:1415-1864, as the SA-UPS
+ * RotoSolve batch uses it, sa_ups_wavefunction.py:1005-1082): psi0 is a host
+ * array of n_inst device pointers, each to its instance's state in the engine
+ * convention (space dim doubles), or NULL for the HF determinant (the
+ * default).  Read at every launch and graph replay: the states must stay
+ * allocated while the batch runs. */
+int sq_batch_set_start_states(sq_batch* b, const double* const* psi0, void* stream);
 /* Kernel shape picked by the autotuner: rotation pairs of determinants per
  * thread in flight (1, 2 or 4) and sigma GEMM 16-column tiles per wave (1 or
  * 2); no counterpart in the reference (its batching has no kernel shapes).

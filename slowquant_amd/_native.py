@@ -96,6 +96,7 @@ _SIGNATURES = {
     "sq_batch_destroy": ([vp], i32),
     "sq_batch_num_params": ([vp], i64),
     "sq_batch_set_theta": ([vp, vp, vp], i32),
+    "sq_batch_set_start_states": ([vp, vp, vp], i32),
     "sq_batch_configure": ([vp, i32, i32], i32),
     "sq_batch_run": ([vp, vp], i32),
     "sq_batch_run_graph": ([vp, vp], i32),

@@ -75,6 +75,7 @@ struct BatchDev {
   int64_t dmax;
   double* energy;          // [ninst]
   double* grad;            // concatenated gradients
+  const double* const* psi0;  // [ninst] start states (engine convention), nullptr: HF determinant
 };
 
 // Visit every (alpha row, beta column) of a space with 32-bit index math:
@@ -174,15 +175,20 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
     const int dim = S.nsa * nsb;  // < 2^30, checked at creation
     double* G = lam + B.dmax;
     const double* th = B.theta + I.theta0;
-    // psi = HF determinant (alpha string 0, beta string 0) in the engine's
-    // sign convention: (-1)^{sum_{p in alpha} #{q in beta : q < p}}.
-    for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) psi[x] = 0.0;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      uint32_t a = S.astr[0], b = S.bstr[0];
-      int par = 0;
-      for (uint32_t aa = a; aa; aa &= aa - 1) par += __popc(b & ((1u << (__ffs(aa) - 1)) - 1u));
-      psi[0] = (par & 1) ? -1.0 : 1.0;
+    const double* start = B.psi0[B.wg_list[li]];
+    if (start) {
+      for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) psi[x] = start[x];
+    } else {
+      // psi = HF determinant (alpha string 0, beta string 0) in the engine's
+      // sign convention: (-1)^{sum_{p in alpha} #{q in beta : q < p}}.
+      for (int64_t x = threadIdx.x; x < dim; x += blockDim.x) psi[x] = 0.0;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t a = S.astr[0], b = S.bstr[0];
+        int par = 0;
+        for (uint32_t aa = a; aa; aa &= aa - 1) par += __popc(b & ((1u << (__ffs(aa) - 1)) - 1u));
+        psi[0] = (par & 1) ? -1.0 : 1.0;
+      }
     }
     __syncthreads();
     // forward sweep
@@ -321,6 +327,7 @@ struct Batch {
   std::vector<void*> allocs;
   BatchDev dev{};
   double* d_theta = nullptr;
+  const double** d_psi0 = nullptr;  // start-state pointer table [inst]
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int32_t> grad_off;
@@ -492,6 +499,8 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       SQ_HIP(hipMalloc(&B.d_theta, sizeof(double) * std::max(theta_off, 1)));
       B.allocs.push_back(B.d_theta);
       B.dev.theta = B.d_theta;
+      B.d_psi0 = bupload(B, std::vector<const double*>(n_inst, nullptr));
+      B.dev.psi0 = B.d_psi0;
     } catch (...) {
       delete bo;
       throw;
@@ -513,6 +522,20 @@ int sq_batch_set_theta(sq_batch* bo, const double* theta, void* stream) {
     SQ_CHECK(bo && theta, SQ_ERR_ARG, "NULL argument");
     SQ_HIP(hipMemcpyAsync(bo->b.d_theta, theta, sizeof(double) * bo->b.ngrad, hipMemcpyHostToDevice,
                           as_stream(stream)));
+  });
+}
+
+/* psi0: host array of n_inst device pointers, each to its instance's start
+ * state (engine convention, space dim doubles) or NULL for the HF
+ * determinant.  The pointers are read at every launch (graph replays
+ * included): the states must stay allocated while the batch runs. */
+int sq_batch_set_start_states(sq_batch* bo, const double* const* psi0, void* stream) {
+  return guarded([&] {
+    SQ_CHECK(bo && psi0, SQ_ERR_ARG, "NULL argument");
+    Batch& B = bo->b;
+    hipStream_t st = as_stream(stream);
+    SQ_HIP(hipMemcpyAsync(B.d_psi0, psi0, sizeof(double*) * B.inst.size(), hipMemcpyHostToDevice, st));
+    SQ_HIP(hipStreamSynchronize(st));
   });
 }
 

@@ -320,7 +320,7 @@ def ucc_energy_grad(prog: Program, H: Hamiltonian, theta, psi0: torch.Tensor, wa
 class Batch:
     """Many independent instances (space, rotation program, Hamiltonian) whose
     energies and adjoint gradients come from one persistent launch, started
-    from each space's HF determinant.  ``run(theta, graph=True)`` replays a
+    from each space's HF determinant or from given states (set_start_states).  ``run(theta, graph=True)`` replays a
     captured hipGraph of that launch."""
 
     def __init__(self, instances, n_workgroups: int = 0) -> None:
@@ -353,6 +353,29 @@ class Batch:
             raise ValueError(f"expected {self.n_params} parameters, got {th.size}")
         self._theta_host = th  # kept alive until the asynchronous copy has run
         N.check(N.lib().sq_batch_set_theta(self._h, th.ctypes.data, _stream()))
+
+    def set_start_states(self, states) -> None:
+        """Start state per instance: a CUDA float64 tensor of the instance's
+        space dimension (engine convention, e.g. Space.from_reference) or None
+        for the HF determinant.  The tensors are kept referenced here, since
+        every later launch reads them."""
+        states = list(states)
+        if len(states) != len(self.instances):
+            raise ValueError(f"expected {len(self.instances)} start states, got {len(states)}")
+        kept, ptrs = [], []
+        for (sp, _, _), x in zip(self.instances, states):
+            if x is None:
+                ptrs.append(None)
+                continue
+            if not (torch.is_tensor(x) and x.is_cuda and x.dtype == torch.float64 and x.is_contiguous()):
+                raise TypeError("start states must be contiguous float64 CUDA tensors")
+            if x.numel() != sp.dim:
+                raise ValueError(f"start state has {x.numel()} amplitudes, its space has {sp.dim}")
+            kept.append(x)
+            ptrs.append(x.data_ptr())
+        arr = (ctypes.c_void_p * len(ptrs))(*ptrs)
+        N.check(N.lib().sq_batch_set_start_states(self._h, arr, _stream()))
+        self._starts = kept
 
     def configure(self, rot_pairs: int = 2, gemm_tiles: int = 1) -> None:
         """Kernel shape: rotation pairs per thread in flight (1, 2, 4) and sigma

@@ -342,14 +342,38 @@ class WaveFunctionSAUPS(WaveFunctionBase):
         does not divide by K; RotoSolve only reconstructs the curve's shape,
         so the scale does not move its minimum)."""
         th = np.asarray(parameters, dtype=np.float64)
-        H, prog, csfs = self._hamiltonian(), self._prog(), self._csfs_dev()
-        out = []
-        for d in theta_diffs:
-            t = th.copy()
-            t[theta_idx] = d
-            out.append(float(sum(H.sigma(prog.state_prep(t, c))[1] for c in csfs)))
+        n, K = len(theta_diffs), self.num_states
+        thetas = np.repeat(th[None, :], n, axis=0)
+        thetas[:, theta_idx] = np.asarray(theta_diffs, dtype=np.float64)
+        batch = self._shifted_batch(n)
+        if batch is not None:
+            e, _ = batch.run(list(np.repeat(thetas, K, axis=0)))
+            out = [float(v) for v in e.reshape(n, K).sum(axis=1)]
+        else:
+            H, prog, csfs = self._hamiltonian(), self._prog(), self._csfs_dev()
+            out = [float(sum(H.sigma(prog.state_prep(t, c))[1] for c in csfs)) for t in thetas]
         self.num_energy_evals += self.num_states
         return out
+
+    def _shifted_batch(self, n: int):
+        """Cached batch of n x K instances (shift-major, state-minor), each
+        started from its reference state, or None where the batched kernel
+        does not apply (non-rotation gates, more than 12 orbitals)."""
+        from . import engine
+
+        H, prog = self._hamiltonian(), self._prog()
+        cache = getattr(self, "_roto_batches", None)
+        if cache is None or cache.get("key") != (id(H), id(prog)):
+            cache = {"key": (id(H), id(prog)), "H": H}
+            self._roto_batches = cache
+        if n not in cache:
+            try:
+                b = engine.Batch([(self._space(), prog, H)] * (n * self.num_states))
+                b.set_start_states(self._csfs_dev() * n)
+                cache[n] = b
+            except (ValueError, RuntimeError):
+                cache[n] = None
+        return cache[n]
 
     def run_wf_optimization_2step(self, optimizer_name: str, orbital_optimization: bool = False, tol: float = 1e-10,
                                   maxiter: int = 1000, is_silent_subiterations: bool = False) -> None:

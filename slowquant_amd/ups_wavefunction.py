@@ -256,9 +256,8 @@ class WaveFunctionUPS(WaveFunctionBase):
         """Energies with theta[theta_idx] replaced by each of theta_diffs
         (ups_wavefunction.py:1144-1194, which propagates the shifted states as
         one batch).  All shifted instances run in one persistent batched
-        launch when the layout and reference allow it (rotation gates, HF
-        reference, at most 12 active orbitals), else one engine energy per
-        shift."""
+        launch when the layout allows it (rotation gates, at most 12 active
+        orbitals), else one engine energy per shift."""
         th = np.asarray(parameters, dtype=np.float64)
         thetas = np.repeat(th[None, :], len(theta_diffs), axis=0)
         thetas[:, theta_idx] = np.asarray(theta_diffs, dtype=np.float64)
@@ -268,11 +267,10 @@ class WaveFunctionUPS(WaveFunctionBase):
 
     def _shifted_batch(self, n: int):
         """Cached batch of n instances of this wave function's program and
-        Hamiltonian, or None where the batched kernel does not apply."""
+        Hamiltonian (started from the perfect-pairing reference when that is
+        the CSF), or None where the batched kernel does not apply."""
         from . import engine
 
-        if self._pp:
-            return None
         H, prog = self._hamiltonian(), self._prog()
         cache = getattr(self, "_roto_batches", None)
         if cache is None or cache.get("key") != (id(H), id(prog)):
@@ -280,7 +278,10 @@ class WaveFunctionUPS(WaveFunctionBase):
             self._roto_batches = cache
         if n not in cache:
             try:
-                cache[n] = engine.Batch([(self._space(), prog, H)] * n)
+                b = engine.Batch([(self._space(), prog, H)] * n)
+                if self._pp:
+                    b.set_start_states([self._csf_dev()] * n)
+                cache[n] = b
             except (ValueError, RuntimeError):
                 cache[n] = None
         return cache[n]

@@ -132,3 +132,58 @@ def test_batch_autotune_and_errors():
     H = inst[0][2]
     with pytest.raises(ValueError):
         E.Batch([(sp, prog, H)], 1)
+
+
+def test_batch_start_states():
+    """Instances started from given states (random unit vectors, engine
+    convention) equal Program.energy_grad from the same states, eagerly and
+    over graph replays that see a changed start table; None restores the HF
+    start.  Argument errors are raised on the host."""
+    import torch
+
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, loop_energy_grad
+
+    specs = _specs()[:6]
+    inst, thetas = build_instances(specs)
+    b = E.Batch(inst, 4)
+    gen = torch.Generator().manual_seed(21)
+
+    def unit_states():
+        out = []
+        for sp, _, _ in inst:
+            x = torch.randn(sp.dim, generator=gen, dtype=torch.float64)
+            out.append((x / x.norm()).cuda())
+        return out
+
+    def expected(starts):
+        e, g = [], []
+        for (sp, prog, H), th, x in zip(inst, thetas, starts):
+            ei, gi = prog.energy_grad(H, th, E.hf_state(sp) if x is None else x)
+            e.append(ei)
+            g.append(gi)
+        return e, g
+
+    starts = unit_states()
+    b.set_start_states(starts)
+    e, g = b.run(thetas)
+    _close(e, g, *expected(starts))
+    for rep in range(2):
+        e, g = b.run(thetas, graph=True)
+        _close(e, g, *expected(starts))
+        starts = unit_states()
+        b.set_start_states(starts)
+        torch.cuda.synchronize()
+    mixed = [None if i % 2 else s for i, s in enumerate(starts)]
+    b.set_start_states(mixed)
+    e, g = b.run(thetas, graph=True)
+    _close(e, g, *expected(mixed))
+    b.set_start_states([None] * len(inst))
+    e, g = b.run(thetas)
+    _close(e, g, *loop_energy_grad(inst, thetas))
+    with pytest.raises(ValueError):
+        b.set_start_states(starts[:-1])
+    with pytest.raises(ValueError):
+        b.set_start_states([starts[1]] + starts[1:])  # wrong dimension for instance 0
+    with pytest.raises(TypeError):
+        b.set_start_states([starts[0].cpu()] + starts[1:])

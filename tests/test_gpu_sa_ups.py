@@ -114,15 +114,24 @@ def test_sa_h4_against_oracle():
     assert np.abs(wf.rdm2 - r2).max() < 1e-12
     # the trace of the averaged 1-RDM counts the active electrons
     assert abs(np.trace(wf.rdm1) - wf.num_active_elec) < 1e-12
-    # RotoSolve's shifted energies are sums over the states
+    # RotoSolve's shifted energies are sums over the states, from one batched
+    # launch of shifts x states instances and from the per-state engine loop
     diffs = [-0.3, 0.1, 0.7]
+    assert wf._shifted_batch(len(diffs)) is not None
     vals = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, 2)
-    for d, v in zip(diffs, vals):
+    batched = wf._shifted_batch
+    wf._shifted_batch = lambda n: None
+    try:
+        vals_loop = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, 2)
+    finally:
+        wf._shifted_batch = batched
+    for d, v, w in zip(diffs, vals, vals_loop):
         t = theta.copy()
         t[2] = d
         ref = sum(expectation_value(q, [H], q, ci, do_folding=False)
                   for q in (construct_ups_state(c, ci, t, wf.ups_layout) for c in wf.csf_coeffs))
         assert abs(v - ref) < 1e-10
+        assert abs(w - ref) < 1e-10
 
 
 def test_sa_h4_gradient_vs_central_differences():

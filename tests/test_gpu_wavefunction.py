@@ -193,6 +193,38 @@ def test_rotosolve_shifted_energies_against_oracle():
         assert abs(O.reconstructed_f([theta[idx]], e, R)[0] - oracle_e(theta)) < 1e-9, idx
 
 
+def test_rotosolve_perfect_pairing_reference_batched():
+    """tUPS with the perfect-pairing reference (ups_wavefunction.py:218-241):
+    the shifted energies come from the batched launch started from that CSF
+    and equal the per-shift engine loop and the oracle to 1e-10."""
+    from oracle.apply import expectation_value
+    from oracle.ups import construct_ups_state
+    from slowquant_amd import optimizers as O
+
+    _sq, wf = _h4_wf(opts={"n_layers": 2, "do_pp": True})
+    assert wf._pp
+    theta = np.random.default_rng(8).normal(0.0, 0.3, wf.ups_layout.n_params)
+    H, ci = _oracle_parts(wf)
+    idx = 3
+    R = wf.ups_layout.grad_param_R[wf.ups_layout.param_names[idx]]
+    diffs = O.shift_points(R)
+    assert wf._shifted_batch(len(diffs)) is not None
+    e = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, idx)
+    batched = wf._shifted_batch
+    wf._shifted_batch = lambda n: None
+    try:
+        e_loop = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, idx)
+    finally:
+        wf._shifted_batch = batched
+    for d, a, b in zip(diffs, e, e_loop):
+        t = theta.copy()
+        t[idx] = d
+        psi = construct_ups_state(wf.csf_coeffs, ci, t, wf.ups_layout)
+        ref = expectation_value(psi, [H], psi, ci, do_folding=False)
+        assert abs(a - ref) < 1e-10
+        assert abs(b - ref) < 1e-10
+
+
 def test_rotosolve_optimisation():
     """RotoSolve through run_wf_optimization_2step: H2 (2e,2o) tUPS reaches the
     lowest eigenvalue of the oracle's Hamiltonian matrix on the CI space; H4
