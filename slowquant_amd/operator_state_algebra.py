@@ -120,23 +120,40 @@ def _fold(op: FermionicOperator, ci: CI_Info, do_folding: bool) -> FermionicOper
     return op
 
 
-def _apply_list(operators, psi: torch.Tensor, ci: CI_Info, sp: E.Space, thetas, wf_struct, do_folding, do_unsafe):
+def _compile_list(operators, ci: CI_Info, sp: E.Space, thetas, wf_struct, do_folding, do_unsafe, ups_only=False):
+    """Device steps psi -> psi for an operator list, rightmost first; the
+    argument errors of the reference are raised before any device work.
+    ups_only: "U"/"Ud" accept only a UpsStructure (the _SA functions,
+    operator_state_algebra.py:684-700)."""
+    steps = []
     for op in operators[::-1]:
         if isinstance(op, str):
             if op not in ("U", "Ud"):
                 raise ValueError(f"Unknown str operator, expected ('U', 'Ud') got {op}")
+            if ups_only and not isinstance(wf_struct, UpsStructure):
+                raise TypeError(f"Got unknown wave function structure type, {type(wf_struct)}")
             if thetas is None:
                 raise ValueError("theta must be different from None")
             th = np.asarray(thetas, dtype=np.float64)
+            dag = op == "Ud"
             if isinstance(wf_struct, UpsStructure):
-                psi = _program(sp, wf_struct).state_prep(th, psi, dagger=(op == "Ud"))
+                prog = _program(sp, wf_struct)
+                steps.append(lambda psi, prog=prog, th=th, dag=dag: prog.state_prep(th, psi, dagger=dag))
             elif isinstance(wf_struct, UccStructure):
-                psi = E.ucc_expm(_ucc_program(sp, wf_struct), th, psi, -1.0 if op == "Ud" else 1.0)
+                prog = _ucc_program(sp, wf_struct)
+                steps.append(lambda psi, prog=prog, th=th, dag=dag: E.ucc_expm(prog, th, psi, -1.0 if dag else 1.0))
             else:
                 raise TypeError(f"Got unknown wave function structure type, {type(wf_struct)}")
         else:
             folded = _fold(op, ci, do_folding)
-            psi = E.Operator(sp, folded.operators.items(), drop_outside=do_unsafe).apply(psi)
+            O = E.Operator(sp, folded.operators.items(), drop_outside=do_unsafe)
+            steps.append(O.apply)
+    return steps
+
+
+def _apply_list(operators, psi: torch.Tensor, ci: CI_Info, sp: E.Space, thetas, wf_struct, do_folding, do_unsafe):
+    for step in _compile_list(operators, ci, sp, thetas, wf_struct, do_folding, do_unsafe):
+        psi = step(psi)
     return psi
 
 
@@ -171,3 +188,79 @@ def build_operator_matrix(op: FermionicOperator, ci_info: CI_Info, do_unsafe: bo
         ref = sp.from_reference(eye)
         M[:, j] = _to_host(sp, O.apply(ref))
     return M
+
+
+# ---- state-averaged (leading state axis) forms -------------------------------
+# operator_state_algebra.py:282-410, 633-867, 1415-1864, 2312-2976: the same
+# operations on every row of a (n_states, dim) array.  Each row is propagated
+# on the device with the single-state kernels; programs and operators are
+# built once per call.
+
+
+def _rows(sp: E.Space, state) -> np.ndarray:
+    a = np.asarray(state, dtype=np.float64)
+    if a.ndim != 2 or a.shape[1] != sp.dim:
+        raise ValueError(f"states have shape {a.shape}, expected (n_states, {sp.dim})")
+    return a
+
+
+def propagate_state_SA(operators: list, state, ci_info: CI_Info, thetas: Sequence[float] | None = None,
+                       wf_struct: UpsStructure | None = None, do_folding: bool = True,
+                       do_unsafe: bool = False) -> np.ndarray:
+    """propagate_state on every row (operator_state_algebra.py:633-781);
+    "U"/"Ud" take a UpsStructure only, as there."""
+    if len(operators) == 0:
+        return np.array(state, dtype=np.float64, copy=True)
+    sp = _space(ci_info)
+    st = _rows(sp, state)
+    steps = _compile_list(operators, ci_info, sp, thetas, wf_struct, do_folding, do_unsafe, ups_only=True)
+    out = np.empty_like(st)
+    for k, row in enumerate(st):
+        psi = sp.from_reference(row)
+        for step in steps:
+            psi = step(psi)
+        out[k] = _to_host(sp, psi)
+    return out
+
+
+def expectation_value_SA(bra, operators: list, ket, ci_info: CI_Info, thetas: Sequence[float] | None = None,
+                         wf_struct: UpsStructure | None = None, do_folding: bool = True) -> float:
+    """(1/n_states) sum_k <bra_k|O|ket_k> (operator_state_algebra.py:827-867)."""
+    op_ket = propagate_state_SA(operators, ket, ci_info, thetas, wf_struct, do_folding=do_folding)
+    b = np.asarray(bra, dtype=np.float64)
+    return float(np.einsum("kd,kd->", b, op_ket) / len(b))
+
+
+def construct_ups_state_SA(state, ci_info: CI_Info, thetas: Sequence[float], ups_struct: UpsStructure,
+                           dagger: bool = False) -> np.ndarray:
+    """U(theta) (or U^+) on every row (operator_state_algebra.py:1415-1864)."""
+    sp = _space(ci_info)
+    st = _rows(sp, state)
+    if len(thetas) != ups_struct.n_params:
+        raise ValueError(f"Expected {ups_struct.n_params} thetas got {len(thetas)}")
+    prog = _program(sp, ups_struct)
+    th = np.asarray(thetas, dtype=np.float64)
+    return np.stack([_to_host(sp, prog.state_prep(th, sp.from_reference(r), dagger=dagger)) for r in st])
+
+
+def propagate_unitary_SA(state, idx: int, ci_info: CI_Info, thetas: Sequence[float],
+                         ups_struct: UpsStructure) -> np.ndarray:
+    """Gate idx on every row (operator_state_algebra.py:2312-2754)."""
+    sp = _space(ci_info)
+    st = _rows(sp, state)
+    prog = _program(sp, ups_struct)
+    th = np.asarray(thetas, dtype=np.float64)
+    out = np.empty_like(st)
+    for k, r in enumerate(st):
+        psi = sp.from_reference(r)
+        prog.apply_gate(idx, th, psi)
+        out[k] = _to_host(sp, psi)
+    return out
+
+
+def get_grad_action_SA(state, idx: int, ci_info: CI_Info, ups_struct: UpsStructure) -> np.ndarray:
+    """T_idx on every row (operator_state_algebra.py:2868-2976)."""
+    sp = _space(ci_info)
+    st = _rows(sp, state)
+    prog = _program(sp, ups_struct)
+    return np.stack([_to_host(sp, prog.grad_action(idx, sp.from_reference(r))) for r in st])
