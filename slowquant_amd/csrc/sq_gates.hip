@@ -768,6 +768,7 @@ int sq_program_destroy(sq_program* pr) {
     if (pr->p.d_terms) (void)hipFree(pr->p.d_terms);
     if (pr->p.d_lists) (void)hipFree(pr->p.d_lists);
     if (pr->p.d_term_fast) (void)hipFree(pr->p.d_term_fast);
+    if (pr->p.ucc_csr) ucc_csr_free(pr->p.ucc_csr);
     if (pr->p.tiles) destroy_tile_plan(pr->p.tiles);
     delete pr;
   });

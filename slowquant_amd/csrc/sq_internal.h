@@ -140,7 +140,14 @@ struct Program {
   // Per term, its adjoint in mask form (built on first use by the UCC
   // kernels, sq_ucc.hip).
   mutable struct TermFast* d_term_fast = nullptr;
+  // T's sparsity pattern as rows of (source determinant, term | sign) entries
+  // (built on first use by the UCC kernels, sq_ucc.hip; freed by
+  // ucc_csr_free).
+  mutable struct UccCsr* ucc_csr = nullptr;
+  mutable bool ucc_csr_tried = false;
 };
+
+void ucc_csr_free(struct UccCsr* c);
 
 // Strings a spin part applies to, as (string index, image | parity << 31)
 // entries appended to lists (shared per (strings, operator) through cache);

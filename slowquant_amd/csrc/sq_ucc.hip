@@ -10,8 +10,14 @@
 // estimate with margin, capped by the triangle-inequality bound).
 // The analytic gradient integrates <e^{-sT} lam | T_i | e^{-sT} psi> over
 // s in [0, 1] by composite Gauss-Legendre (lam = H psi), all gates per node.
+// T's sparsity pattern does not depend on theta: it is built once per program
+// (rows of (source determinant, term, sign)) and every T application is then
+// one weighted sparse matrix-vector product; patterns above 2^29 entries fall
+// back to kernels that rescan the terms per determinant.
 
 #include <cmath>
+#include <cstdlib>
+#include <vector>
 
 #include "sq_internal.h"
 
@@ -127,6 +133,82 @@ __global__ __launch_bounds__(kUB) void k_tapply_group(SV v, const Term* __restri
     val += ((par & 1) ? -c : c) * cur[(size_t)ra * v.nsb + rb];
   }
   part[(size_t)g * dim + I] = val;
+}
+
+// T's sparsity pattern, row I = the terms whose adjoint maps I to a
+// determinant J of the space, in term order: count, then fill the entries
+// (J, t | parity << 31) between rowp[I] and rowp[I + 1].
+__device__ __forceinline__ bool term_entry(const SV& v, const Term* __restrict__ terms,
+                                           const TermFast* __restrict__ fast, int t, uint32_t a0, uint32_t b0,
+                                           uint32_t& J, int& par) {
+  uint32_t a, b;
+  if (!term_adjoint(fast[t], terms, t, a0, b0, a, b, par)) return false;
+  const int ra = v.arank[a], rb = v.brank[b];
+  if (ra < 0 || rb < 0) return false;
+  J = (uint32_t)ra * (uint32_t)v.nsb + (uint32_t)rb;
+  return true;
+}
+
+__global__ void k_csr_count(SV v, const Term* __restrict__ terms, const TermFast* __restrict__ fast, int nt,
+                            int32_t* __restrict__ cnt) {
+  const size_t dim = (size_t)v.nsa * v.nsb;
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= dim) return;
+  const int ia = (int)(I / (uint32_t)v.nsb), ib = (int)(I - (size_t)ia * v.nsb);
+  const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
+  int c = 0;
+  for (int t = 0; t < nt; ++t) {
+    uint32_t J;
+    int par;
+    if (term_entry(v, terms, fast, t, a0, b0, J, par)) ++c;
+  }
+  cnt[I] = c;
+}
+
+__global__ void k_csr_fill(SV v, const Term* __restrict__ terms, const TermFast* __restrict__ fast, int nt,
+                           const int64_t* __restrict__ rowp, uint2* __restrict__ ent) {
+  const size_t dim = (size_t)v.nsa * v.nsb;
+  const size_t I = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (I >= dim) return;
+  const int ia = (int)(I / (uint32_t)v.nsb), ib = (int)(I - (size_t)ia * v.nsb);
+  const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
+  int64_t k = rowp[I];
+  const int64_t e = rowp[I + 1];
+  for (int t = 0; t < nt && k < e; ++t) {
+    uint32_t J;
+    int par;
+    if (term_entry(v, terms, fast, t, a0, b0, J, par)) ent[k++] = make_uint2(J, (uint32_t)t | ((uint32_t)(par & 1) << 31));
+  }
+}
+
+// nxt[I] = f sum_{entries (J, t, sign) of row I} sign w[t] coef[t] cur[J] and
+// acc[I] += nxt[I] (acc != nullptr), one wave per row: lane l takes entries
+// l, l + 64, ... and the wave sums in a fixed shuffle order.
+constexpr int kCsrRows = 4;  // rows (waves) per block
+
+__global__ __launch_bounds__(64 * kCsrRows) void k_tapply_csr(int64_t dim, const int64_t* __restrict__ rowp,
+                                                              const uint2* __restrict__ ent,
+                                                              const double* __restrict__ coef,
+                                                              const double* __restrict__ w,
+                                                              const double* __restrict__ cur, double f,
+                                                              double* __restrict__ nxt, double* __restrict__ acc) {
+  const int lane = threadIdx.x & 63;
+  const int64_t I = (int64_t)blockIdx.x * kCsrRows + (threadIdx.x >> 6);
+  if (I >= dim) return;  // uniform per wave; no block barrier below
+  const int64_t b = rowp[I], e = rowp[I + 1];
+  double val = 0.0;
+  for (int64_t k = b + lane; k < e; k += 64) {
+    const uint2 x = ent[k];
+    const uint32_t t = x.y & 0x7fffffffu;
+    const double c = (w ? w[t] : 1.0) * coef[t] * cur[x.x];
+    val += (x.y >> 31) ? -c : c;
+  }
+  for (int off = 32; off > 0; off >>= 1) val += __shfl_down(val, off, 64);
+  if (lane == 0) {
+    const double r = f * val;
+    nxt[I] = r;
+    if (acc) acc[I] += r;
+  }
 }
 
 // nxt[I] = f * sum_g part[g][I]; acc[I] += nxt[I] when acc != nullptr.
@@ -270,6 +352,71 @@ static const TermFast* term_fast(const Program& p) {
   return p.d_term_fast;
 }
 
+struct UccCsr {
+  int64_t nnz = 0;
+  int64_t* rowp = nullptr;  // [dim + 1]
+  uint2* ent = nullptr;     // [nnz]
+  double* coef = nullptr;   // [terms]
+};
+
+void ucc_csr_free(UccCsr* c) {
+  if (!c) return;
+  if (c->rowp) (void)hipFree(c->rowp);
+  if (c->ent) (void)hipFree(c->ent);
+  if (c->coef) (void)hipFree(c->coef);
+  delete c;
+}
+
+// Largest pattern kept (entries of 8 bytes); above it, and with SQ_UCC_CSR=0,
+// T is applied by the term-scan kernels below.
+constexpr int64_t kCsrMaxEntries = (int64_t)1 << 29;
+
+// The program's sparsity pattern of T on space s, built on first use
+// (synchronous); nullptr where it is not used.
+static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
+  if (p.ucc_csr_tried) return p.ucc_csr;
+  p.ucc_csr_tried = true;
+  const char* env = getenv("SQ_UCC_CSR");
+  if (env && env[0] == '0') return nullptr;
+  const int nt = (int)p.terms.size();
+  if (s.dim == 0 || nt == 0 || s.dim >= ((int64_t)1 << 31)) return nullptr;
+  const TermFast* fast = term_fast(p);
+  const unsigned nb = (unsigned)((s.dim + kUB - 1) / kUB);
+  int32_t* cnt = nullptr;
+  SQ_HIP(hipMalloc(&cnt, sizeof(int32_t) * (size_t)s.dim));
+  std::vector<int32_t> hc((size_t)s.dim);
+  k_csr_count<<<nb, kUB, 0, st>>>(sv(s), p.d_terms, fast, nt, cnt);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(cnt);
+  SQ_HIP(e);
+  std::vector<int64_t> rowp((size_t)s.dim + 1, 0);
+  for (size_t i = 0; i < hc.size(); ++i) rowp[i + 1] = rowp[i] + hc[i];
+  const int64_t nnz = rowp.back();
+  if (nnz == 0 || nnz > kCsrMaxEntries) return nullptr;
+  UccCsr* c = new UccCsr;
+  c->nnz = nnz;
+  std::vector<double> coef((size_t)nt);
+  for (int t = 0; t < nt; ++t) coef[t] = p.terms[t].coef;
+  e = hipMalloc(&c->rowp, sizeof(int64_t) * rowp.size());
+  if (e == hipSuccess) e = hipMalloc(&c->ent, sizeof(uint2) * (size_t)nnz);
+  if (e == hipSuccess) e = hipMalloc(&c->coef, sizeof(double) * coef.size());
+  if (e == hipSuccess) e = hipMemcpy(c->rowp, rowp.data(), sizeof(int64_t) * rowp.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(c->coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    k_csr_fill<<<nb, kUB, 0, st>>>(sv(s), p.d_terms, fast, nt, c->rowp, c->ent);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) {
+    ucc_csr_free(c);
+    SQ_HIP(e);
+  }
+  p.ucc_csr = c;
+  return c;
+}
+
 // Term groups of the split T application: enough (determinant, group)
 // threads to fill the GPU for small spaces, at most 64 groups.
 static int term_groups(const Space& s) {
@@ -313,6 +460,13 @@ static UccWork carve_ucc(const Space& s, const Program& p, double* work) {
 // nxt = f T cur (w: term weights, nullptr for unit weights); acc += nxt.
 static void tapply(const Space& s, const Program& p, const double* w, const double* cur, double f, double* nxt,
                    double* acc, double* part, hipStream_t st) {
+  if (const UccCsr* c = ucc_csr(s, p, st)) {
+    SQ_PROF(PC_TAPPLY, st, 16.0 * (double)s.dim + 8.0 * (double)c->nnz, 0.0,
+            k_tapply_csr<<<(unsigned)((s.dim + kCsrRows - 1) / kCsrRows), 64 * kCsrRows, 0, st>>>(
+                s.dim, c->rowp, c->ent, c->coef, w, cur, f, nxt, acc));
+    SQ_HIP(hipGetLastError());
+    return;
+  }
   const int G = term_groups(s);
   int rows = G;  // part rows
   if (G >= kTG) {  // small space: fold kTG groups per block in LDS
