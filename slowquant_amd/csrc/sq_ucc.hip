@@ -266,6 +266,34 @@ __global__ void k_gate_dots(SV sv, const Term* __restrict__ terms, const TermFas
   }
 }
 
+// gsum[i] += f sum over gate i's pattern entries (I, J, t, sign) of
+// sign coef[t] v[I] u[J], i.e. f <v|T_i u>; one block per gate, its entries
+// (rows ascending) strided over the threads and summed in a fixed order.
+__global__ __launch_bounds__(kUB) void k_gate_dots_csr(const int64_t* __restrict__ gptr,
+                                                       const uint4* __restrict__ gent,
+                                                       const double* __restrict__ coef,
+                                                       const double* __restrict__ u,
+                                                       const double* __restrict__ v, double f,
+                                                       double* __restrict__ gsum) {
+  __shared__ double red[kUB / 64];
+  const int i = blockIdx.x;
+  const int64_t e = gptr[i + 1];
+  double val = 0.0;
+  for (int64_t k = gptr[i] + threadIdx.x; k < e; k += blockDim.x) {
+    const uint4 x = gent[k];
+    const double c = coef[x.z & 0x7fffffffu] * v[x.x] * u[x.y];
+    val += (x.z >> 31) ? -c : c;
+  }
+  for (int off = 32; off > 0; off >>= 1) val += __shfl_down(val, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
+    gsum[i] += f * tot;
+  }
+}
+
 // gsum[i] += f * sum_b dpart[b*ng + i] (fixed order).
 __global__ void k_gate_accum(const double* __restrict__ dpart, int nb, int ng, double f, double* __restrict__ gsum) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -357,6 +385,11 @@ struct UccCsr {
   int64_t* rowp = nullptr;  // [dim + 1]
   uint2* ent = nullptr;     // [nnz]
   double* coef = nullptr;   // [terms]
+  // The same entries grouped by gate (rows ascending inside a gate) as
+  // (I, J, t | sign << 31, 0), for the gradient's per-gate dots; nullptr
+  // above kGateMaxEntries.
+  int64_t* gptr = nullptr;  // [gates + 1]
+  uint4* gent = nullptr;    // [nnz]
 };
 
 void ucc_csr_free(UccCsr* c) {
@@ -364,7 +397,42 @@ void ucc_csr_free(UccCsr* c) {
   if (c->rowp) (void)hipFree(c->rowp);
   if (c->ent) (void)hipFree(c->ent);
   if (c->coef) (void)hipFree(c->coef);
+  if (c->gptr) (void)hipFree(c->gptr);
+  if (c->gent) (void)hipFree(c->gent);
   delete c;
+}
+
+constexpr int64_t kGateMaxEntries = (int64_t)1 << 28;
+
+// Regroup the row pattern by gate on the host (stable: rows stay ascending).
+static void build_gate_major(const Space& s, const Program& p, const std::vector<int64_t>& rowp, UccCsr* c) {
+  const int ng = (int)p.gates.size();
+  if (ng == 0 || c->nnz > kGateMaxEntries) return;
+  std::vector<uint2> ent((size_t)c->nnz);
+  SQ_HIP(hipMemcpy(ent.data(), c->ent, sizeof(uint2) * ent.size(), hipMemcpyDeviceToHost));
+  std::vector<int32_t> t2g(p.terms.size(), -1);
+  for (int g = 0; g < ng; ++g)
+    for (int t = p.gates[g].term_begin; t < p.gates[g].term_end; ++t) t2g[t] = g;
+  std::vector<int64_t> gptr((size_t)ng + 1, 0);
+  for (const uint2& x : ent) {
+    const int g = t2g[x.y & 0x7fffffffu];
+    if (g >= 0) ++gptr[(size_t)g + 1];
+  }
+  for (int g = 0; g < ng; ++g) gptr[(size_t)g + 1] += gptr[g];
+  std::vector<uint4> gent((size_t)gptr[ng]);
+  std::vector<int64_t> pos(gptr.begin(), gptr.end() - 1);
+  for (int64_t I = 0; I < s.dim; ++I)
+    for (int64_t k = rowp[I]; k < rowp[I + 1]; ++k) {
+      const uint2 x = ent[(size_t)k];
+      const int g = t2g[x.y & 0x7fffffffu];
+      if (g >= 0) gent[(size_t)pos[g]++] = make_uint4((uint32_t)I, x.x, x.y, 0u);
+    }
+  hipError_t e = hipMalloc(&c->gptr, sizeof(int64_t) * gptr.size());
+  if (e == hipSuccess) e = hipMalloc(&c->gent, sizeof(uint4) * std::max<size_t>(gent.size(), 1));
+  if (e == hipSuccess) e = hipMemcpy(c->gptr, gptr.data(), sizeof(int64_t) * gptr.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && !gent.empty())
+    e = hipMemcpy(c->gent, gent.data(), sizeof(uint4) * gent.size(), hipMemcpyHostToDevice);
+  SQ_HIP(e);
 }
 
 // Largest pattern kept (entries of 8 bytes); above it, and with SQ_UCC_CSR=0,
@@ -412,6 +480,12 @@ static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
   if (e != hipSuccess) {
     ucc_csr_free(c);
     SQ_HIP(e);
+  }
+  try {
+    build_gate_major(s, p, rowp, c);
+  } catch (...) {
+    ucc_csr_free(c);
+    throw;
   }
   p.ucc_csr = c;
   return c;
@@ -610,6 +684,7 @@ static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, do
   SQ_HIP(hipGetLastError());
   const int nb = (int)dot_blocks(s);
   const dim3 dgrid(nb, gate_groups(s, ng));
+  const UccCsr* csr = ucc_csr(s, p, st);
   double prev = 0.0;
   for (int j = 0; j < m; ++j) {
     for (int q = 0; q < 8; ++q) {
@@ -618,8 +693,14 @@ static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, do
       expm_apply(s, p, u, nu, -ds, u.C, u.C, st);
       expm_apply(s, p, u, nu, -ds, u.D, u.D, st);
       prev = node;
-      k_gate_dots<<<dgrid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), p.d_gates, ng, u.C, u.D, u.dpart);
-      k_gate_accum<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(u.dpart, nb, ng, 2.0 * kGLw[q] / m, grad_dev);
+      if (csr && csr->gent) {
+        k_gate_dots_csr<<<(unsigned)ng, kUB, 0, st>>>(csr->gptr, csr->gent, csr->coef, u.C, u.D,
+                                                     2.0 * kGLw[q] / m, grad_dev);
+      } else {
+        k_gate_dots<<<dgrid, kUB, 0, st>>>(sv(s), p.d_terms, term_fast(p), p.d_gates, ng, u.C, u.D, u.dpart);
+        k_gate_accum<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(u.dpart, nb, ng, 2.0 * kGLw[q] / m,
+                                                                       grad_dev);
+      }
       SQ_HIP(hipGetLastError());
     }
   }
