@@ -17,12 +17,16 @@ The JSON line also carries:
                   per launch over its average duration, timed with HIP events
                   on its stream over the timed steps themselves (only that
                   kernel is bracketed there: one event pair per launch)
-  rotation        the forward sweep by per-gate rotation kernels (HBM
-                  fraction they reach) and by the fused tile stages (the
-                  per-gate sweep's bytes over the fused sweep's time)
+  rotation        the forward sweep by per-gate rotation kernels and by
+                  the fused tile stages: each kernel's algorithmic bytes per
+                  launch over its average duration, as a fraction of HBM peak
+  batch512        the 512-instance batched workload (one hipGraph-replayed
+                  persistent launch) in instances/s, against a loop of
+                  per-instance engine calls
   cpu_baseline    the reference algorithm (C port of its Numba scan kernel,
                   Python driver as in the reference) on a bounded sample,
-                  extrapolated to one evaluation
+                  extrapolated to one evaluation, on all host cores (value)
+                  and on one thread (single_thread)
 """
 
 from __future__ import annotations
@@ -93,9 +97,9 @@ def measured_traffic(kernel_class):
 def rotation_profile(sp, lay, prog, theta, psi0):
     """The forward sweep U(theta) psi0 two ways: per-gate rotation kernels (a
     program built with SQ_TILES=0: one HBM pass per rotation string over the
-    amplitudes it touches) and the fused tile stages.  The per-gate sweep's
-    bytes over the fused sweep's time is the bandwidth an unfused sweep would
-    need to match it."""
+    amplitudes it touches) and the fused tile stages; for each, the kernel's
+    algorithmic bytes over its duration as a fraction of HBM peak, and the
+    ratio of the two sweep times."""
     from slowquant_amd import engine as E
     from slowquant_amd import profiling
 
@@ -133,16 +137,57 @@ def rotation_profile(sp, lay, prog, theta, psi0):
             "launches_per_sweep": tf.get("launches_per_call"),
             "avg_us": tf.get("avg_us"),
             "achieved_GBs": tf.get("achieved_GBs"),
+            "frac_of_hbm": None if not tf else tf["achieved_GBs"] / HBM_PEAK_GBS,
             "sweep_ms": fu_ms,
-            "per_gate_bytes_over_fused_time_GBs": pg_bytes / (fu_ms * 1e-3) / 1e9 if fu_ms > 0 else None,
-            "per_gate_bytes_over_fused_time_frac_of_hbm": (pg_bytes / (fu_ms * 1e-3) / 1e9) / HBM_PEAK_GBS
-            if fu_ms > 0 else None,
+            "speedup_over_per_gate_sweep": pg_ms / fu_ms if fu_ms > 0 else None,
         },
     }
 
 
+def host_cpu() -> dict:
+    """Host description for the CPU baseline: os.cpu_count() and the lscpu model."""
+    model = None
+    try:
+        import subprocess
+
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, ValueError):
+        pass
+    return {"os_cpu_count": os.cpu_count(), "lscpu_model": model}
+
+
+def cpu_threads() -> int:
+    """All host cores for the threaded variant, capped at the 16 threads one
+    GPU's share of the host provides."""
+    return max(1, min(16, os.cpu_count() or 1))
+
+
 def cpu_baseline(h, g, theta, lay, budget_s=20.0):
-    """Reference algorithm on host cores: C port of the Numba scan, bounded sample."""
+    """Reference algorithm on host cores: C port of the Numba scan, bounded
+    sample, timed on 1 thread (the reference's default, Numba forced to one
+    thread at operator_state_algebra.py:23-30) and on all host cores (its
+    threaded scan, apply_operator_threaded, :139-219)."""
+    nt = cpu_threads()
+    one = _cpu_sample(h, g, theta, lay, 0.5 * budget_s, 1)
+    allc = _cpu_sample(h, g, theta, lay, 0.5 * budget_s, nt)
+    return {
+        "value": 1.0 / allc["seconds_per_eval"],
+        "unit": "evals/s",
+        "cores": nt,
+        "kind": "port",
+        "sample": allc["sample"],
+        "seconds_per_eval": allc["seconds_per_eval"],
+        "single_thread": {"value": 1.0 / one["seconds_per_eval"], "unit": "evals/s", "cores": 1,
+                          "seconds_per_eval": one["seconds_per_eval"], "sample": one["sample"]},
+        "host": host_cpu(),
+    }
+
+
+def _cpu_sample(h, g, theta, lay, budget_s, nthreads):
     from oracle import cscan
     from oracle.ci import get_indexing
     from oracle.fermion import hamiltonian_active_fast
@@ -157,7 +202,7 @@ def cpu_baseline(h, g, theta, lay, budget_s=20.0):
     rng = np.random.default_rng(1)
     psi = rng.normal(size=len(ci.idx2det))
     psi /= np.linalg.norm(psi)
-    # H.psi sample: first strings of H, 1 thread (the reference's default)
+    # H.psi sample: the first strings of H
     sample = 0
     t_h = 0.0
     items = list(H.operators.items())
@@ -166,7 +211,7 @@ def cpu_baseline(h, g, theta, lay, budget_s=20.0):
         chunk = items[sample : sample + 200]
         t0 = time.perf_counter()
         for key, f in chunk:
-            cscan.apply_string_c(hs, psi, key, f, out, 1)
+            cscan.apply_string_c(hs, psi, key, f, out, nthreads)
         t_h += time.perf_counter() - t0
         sample += len(chunk)
     t_hpsi = t_h / sample * n_str
@@ -186,8 +231,8 @@ def cpu_baseline(h, g, theta, lay, budget_s=20.0):
             gens = [generator(t, idx)]
         v = psi
         for T in gens:
-            t1 = cscan.propagate_state_c([T], v, hs)
-            t2 = cscan.propagate_state_c([T, T], v, hs)
+            t1 = cscan.propagate_state_c([T], v, hs, nthreads)
+            t2 = cscan.propagate_state_c([T, T], v, hs, nthreads)
             v = v + np.sin(theta[k]) * t1 + (1 - np.cos(theta[k])) * t2
         gate_t += time.perf_counter() - t0
         gate_n += 1
@@ -199,14 +244,10 @@ def cpu_baseline(h, g, theta, lay, budget_s=20.0):
     N = len(theta)
     per_eval = 2 * (t_hpsi + t_build) + (5 + 1.0 / 3.0) * N * t_gate
     return {
-        "value": 1.0 / per_eval,
-        "unit": "evals/s",
-        "cores": 1,
-        "kind": "port",
         "sample": (
             f"extrapolated: {sample} of {n_str} H strings ({t_h:.1f} s) and {gate_n} of {N} gates "
             f"({gate_t:.1f} s) of the reference algorithm (C port of the Numba scan with a hash map, Python "
-            f"driver), 1 thread; one eval = 2 H.psi + 2 H builds + 5.33 N gate applications"
+            f"driver), {nthreads} thread(s); one eval = 2 H.psi + 2 H builds + 5.33 N gate applications"
         ),
         "seconds_per_eval": per_eval,
     }
@@ -413,6 +454,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 512-instance batch key of the default line")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", choices=("tups12", "batch512", "tups14", "ucc"), default="tups12",
                     help="tups12: the headline (12e,12o) tUPS metric; batch512: 512 batched instances; "
@@ -465,6 +507,8 @@ def main() -> int:
         "unit": dom["unit"],
         "frac": dom["achieved"] / dom["peak"],
         "traffic": measured_traffic(dom["name"]),
+        "traffic_source": "profiles/tups12_traffic.txt (rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE per dispatch, "
+                          "scaled by a device copy of known size; committed, not measured in this run)",
         "kernel": dom["name"],
     }
     out = {
@@ -496,6 +540,12 @@ def main() -> int:
         "kernels": kernels,
         "rotation": rotation_profile(sp, lay, prog, th, psi0),
     }
+    if not args.no_batch:
+        # the 512-instance batched workload, timed in the same run
+        bargs = argparse.Namespace(instances=args.instances, steps=10, warmup=2)
+        bb = bench_batch(bargs)
+        out["batch512"] = {k: bb[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "loop",
+                                              "batch_over_loop", "max_diff_batch_vs_loop", "config")}
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(h, g, th, lay, args.cpu_budget)
     print(json.dumps(out), flush=True)

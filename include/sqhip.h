@@ -163,7 +163,8 @@ int64_t sq_ucc_work_bytes(const sq_space* s, const sq_program* p);
 int sq_ucc_expm(const sq_space* s, const sq_program* p, const double* theta, double t, const double* in,
                 double* out, void* work, void* stream);
 /* energy = <psi|H|psi>, psi = exp(T(theta)) psi0, and (if grad != NULL) the
- * analytic dE/dtheta from the block-triangular exponential. */
+ * analytic dE/dtheta_i = 2 int_0^1 <e^{-sT} lam|T_i|e^{-sT} psi> ds (lam = H psi),
+ * by composite 8-point Gauss-Legendre quadrature over s. */
 int sq_ucc_energy_grad(const sq_space* s, const sq_program* p, const sq_hamiltonian* H, const double* theta,
                        const double* psi0, double* energy, double* grad, void* work, void* stream);
 

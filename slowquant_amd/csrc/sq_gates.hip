@@ -821,6 +821,7 @@ int sq_operator_destroy(sq_operator* op) {
 int sq_operator_apply(const sq_space* sp, const sq_operator* op, const double* in, double* out, void* stream) {
   return guarded([&] {
     SQ_CHECK(sp && op && in && out && in != out, SQ_ERR_ARG, "invalid arguments (in and out must differ)");
+    SQ_CHECK(op->space == &sp->s, SQ_ERR_ARG, "operator built for another space");
     const Space& s = sp->s;
     if (s.dim == 0) return;
     k_tapply<<<grid2d(s), kBlk, 0, as_stream(stream)>>>(view(s), op->d_terms, 0, (int)op->terms.size(), in, out,
@@ -833,6 +834,7 @@ int sq_state_prep(const sq_space* sp, const sq_program* pr, const double* theta,
                   double* psi_out, int dagger, double* work, void* stream) {
   return guarded([&] {
     SQ_CHECK(sp && pr && psi_in && psi_out, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(pr->p.space == &sp->s, SQ_ERR_ARG, "program built for another space");
     const Space& s = sp->s;
     const Program& p = pr->p;
     SQ_CHECK(theta || p.gates.empty(), SQ_ERR_ARG, "theta is NULL");
@@ -855,6 +857,7 @@ int sq_apply_gate(const sq_space* sp, const sq_program* pr, int gate, const doub
                   void* stream) {
   return guarded([&] {
     SQ_CHECK(sp && pr && psi && theta, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(pr->p.space == &sp->s, SQ_ERR_ARG, "program built for another space");
     SQ_CHECK(gate >= 0 && gate < (int)pr->p.gates.size(), SQ_ERR_ARG, "gate index out of range");
     apply_gate_impl(sp->s, pr->p, gate, theta, false, psi, work, as_stream(stream));
   });
@@ -863,6 +866,7 @@ int sq_apply_gate(const sq_space* sp, const sq_program* pr, int gate, const doub
 int sq_grad_action(const sq_space* sp, const sq_program* pr, int gate, const double* in, double* out, void* stream) {
   return guarded([&] {
     SQ_CHECK(sp && pr && in && out && in != out, SQ_ERR_ARG, "invalid arguments (in and out must differ)");
+    SQ_CHECK(pr->p.space == &sp->s, SQ_ERR_ARG, "program built for another space");
     SQ_CHECK(gate >= 0 && gate < (int)pr->p.gates.size(), SQ_ERR_ARG, "gate index out of range");
     grad_action_impl(sp->s, pr->p, gate, in, out, as_stream(stream));
   });

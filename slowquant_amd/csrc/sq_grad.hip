@@ -184,6 +184,8 @@ int64_t sq_work_bytes(const sq_space* sp, int what) {
            const double* psi0, double* energy, double* grad, double* work, void* stream) {                    \
     return guarded([&] {                                                                                        \
       SQ_CHECK(sp && pr && H && psi0 && energy && work, SQ_ERR_ARG, "NULL argument");                          \
+      SQ_CHECK(pr->p.space == &sp->s && H->h.space == &sp->s, SQ_ERR_ARG,                                       \
+               "program / Hamiltonian built for another space");                                                \
       SQ_CHECK(theta || pr->p.gates.empty(), SQ_ERR_ARG, "theta is NULL");                                      \
       SQ_CHECK(grad || pr->p.gates.empty(), SQ_ERR_ARG, "grad is NULL");                                        \
       SQ_CHECK((int)pr->p.gates.size() <= 65536, SQ_ERR_ARG, "too many gates");                                 \
@@ -199,6 +201,7 @@ int sq_adjoint_reverse(const sq_space* sp, const sq_program* pr, const double* t
                        double* grad, double* work, void* stream) {
   return guarded([&] {
     SQ_CHECK(sp && pr && psi && lam && work, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(pr->p.space == &sp->s, SQ_ERR_ARG, "program built for another space");
     SQ_CHECK((theta && grad) || pr->p.gates.empty(), SQ_ERR_ARG, "theta or grad is NULL");
     SQ_CHECK((int)pr->p.gates.size() <= 65536, SQ_ERR_ARG, "too many gates");
     adjoint_reverse(sp->s, pr->p, theta, psi, lam, grad, work, as_stream(stream));

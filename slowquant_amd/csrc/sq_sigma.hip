@@ -794,6 +794,7 @@ int sq_sigma(const sq_space* sp, const sq_hamiltonian* H, const double* psi, dou
   return guarded([&] {
     SQ_CHECK(sp && H && psi && sigma && work, SQ_ERR_ARG, "NULL argument");
     SQ_CHECK(psi != sigma, SQ_ERR_ARG, "psi and sigma must differ");
+    SQ_CHECK(H->h.space == &sp->s, SQ_ERR_ARG, "Hamiltonian built for another space");
     sigma_impl(sp->s, H->h, psi, sigma, energy, work, as_stream(stream));
   });
 }

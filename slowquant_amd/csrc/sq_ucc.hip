@@ -447,7 +447,9 @@ static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
   const char* env = getenv("SQ_UCC_CSR");
   if (env && env[0] == '0') return nullptr;
   const int nt = (int)p.terms.size();
-  if (s.dim == 0 || nt == 0 || s.dim >= ((int64_t)1 << 31)) return nullptr;
+  // k_tapply_csr runs one 64-lane wave per determinant: 64 * dim work-items
+  // must stay below the 2^32 limit of one grid dimension
+  if (s.dim == 0 || nt == 0 || 64 * s.dim >= ((int64_t)1 << 32)) return nullptr;
   const TermFast* fast = term_fast(p);
   const unsigned nb = (unsigned)((s.dim + kUB - 1) / kUB);
   int32_t* cnt = nullptr;
@@ -718,6 +720,7 @@ int sq_ucc_expm(const sq_space* sp, const sq_program* pr, const double* theta, d
   return guarded([&] {
     SQ_CHECK(sp && pr && in && out && work && (theta || pr->p.gates.empty()), SQ_ERR_ARG, "NULL argument");
     SQ_CHECK(in != out, SQ_ERR_ARG, "in and out must differ");
+    SQ_CHECK(pr->p.space == &sp->s, SQ_ERR_ARG, "program built for another space");
     ucc_expm_impl(sp->s, pr->p, theta, t, in, out, work, as_stream(stream));
   });
 }
@@ -727,6 +730,8 @@ int sq_ucc_energy_grad(const sq_space* sp, const sq_program* pr, const sq_hamilt
   double* work = static_cast<double*>(work_v);
   return guarded([&] {
     SQ_CHECK(sp && pr && H && psi0 && energy && work && (theta || pr->p.gates.empty()), SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(pr->p.space == &sp->s && H->h.space == &sp->s, SQ_ERR_ARG,
+             "program / Hamiltonian built for another space");
     const Space& s = sp->s;
     const Program& p = pr->p;
     const int ng = (int)p.gates.size();

@@ -154,6 +154,7 @@ class WaveFunctionSAUPS(WaveFunctionBase):
         """Moves the expansion point (sa_ups_wavefunction.py:296-311)."""
         self._h_mo = None
         self._g_mo = None
+        self._dev_ham = None  # built from the integrals just dropped
         self._sa_energy = None
         self._state_energies = None
         self._kappa = list(k).copy()

@@ -3,7 +3,8 @@
 
 The reference builds a dense d x d T matrix and calls expm_multiply; here
 exp(T(theta))|CSF> is a scaled Taylor series of engine operator applications.
-``gradient_method``: "analytic" (default; block-triangular exponential) or
+``gradient_method``: "analytic" (default; Gauss-Legendre quadrature of
+<e^{-sT} lam|T_i|e^{-sT} psi> over s in [0, 1]) or
 "finite_difference" (the reference's forward difference with step
 sqrt(eps), ucc_wavefunction.py:1069-1097, evaluated with the same engine).
 """

@@ -124,6 +124,7 @@ class WaveFunctionBase:
     def kappa(self, k: list[float]) -> None:
         self._h_mo = None
         self._g_mo = None
+        self._dev_ham = None  # built from the integrals just dropped
         self._energy_elec = None
         self._kappa = list(k).copy()
         self._c_mo = self.c_mo

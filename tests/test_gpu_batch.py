@@ -2,7 +2,7 @@
 and, for the small spaces, the oracle.
 
 Instances: tUPS with 1-3 layers over (4e,4o), (6e,6o), (8e,8o), (10e,10o)
-with seeded synthetic Hamiltonians (rank 16, variance 0.3^2/n) and
+with seeded synthetic Hamiltonians (rank 2n, variance 0.3^2/n) and
 theta ~ N(0, 0.1^2).  Tolerances: energy 1e-10, gradient 1e-10 (analytic vs
 analytic); the oracle comparison uses the same bounds.
 """
@@ -187,3 +187,39 @@ def test_batch_start_states():
         b.set_start_states([starts[1]] + starts[1:])  # wrong dimension for instance 0
     with pytest.raises(TypeError):
         b.set_start_states([starts[0].cpu()] + starts[1:])
+
+
+def test_batch512_workload():
+    """The 512-instance workload exactly as benchmarked (draw_specs: rng seed 7,
+    spaces uniform over (4e,4o)..(10e,10o), 1-3 layers, instance seeds 1000+k,
+    rank 2n): autotuned, run through the captured hipGraph, every energy and
+    gradient equal to the per-instance loop (1e-10), and the (4e,4o) and (6e,6o)
+    instances against the oracle (1e-10)."""
+    from oracle import fermion as F
+    from oracle.apply import expectation_value
+    from oracle.ci import get_indexing
+    from oracle.ups import construct_ups_state, gradient_reference
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import build_instances, draw_specs, instance_problem, loop_energy_grad
+
+    specs = draw_specs(512)
+    assert len(specs) == 512 and {s.n_orb for s in specs} == {4, 6, 8, 10}
+    inst, thetas = build_instances(specs)
+    b = E.autotune_batch(inst, np.concatenate(thetas))
+    e, g = b.run(thetas, graph=True)
+    e2, g2 = b.run(thetas, graph=True)  # a second replay gives the same numbers
+    _close(e2, g2, e, g, 0.0 + 1e-14)
+    e_loop, g_loop = loop_energy_grad(inst, thetas)
+    _close(e, g, e_loop, g_loop)
+    print(f"batch512: workgroups {b.n_workgroups}, shape {b.shape}", flush=True)
+    for k, s in enumerate(specs):
+        if s.n_orb > 6:
+            continue
+        layout, h, gg, theta = instance_problem(s)
+        Hf = F.hamiltonian_0i_0a(h, gg, 0, s.n_orb)
+        ci = get_indexing(0, s.n_orb, 0, s.n_alpha, s.n_beta)
+        csf = np.zeros(len(ci.idx2det))
+        csf[0] = 1.0
+        psi = construct_ups_state(csf, ci, theta, layout)
+        assert abs(e[k] - expectation_value(psi, [Hf], psi, ci, do_folding=False)) < 1e-10
+        assert np.abs(g[k] - gradient_reference(Hf, csf, ci, theta, layout)).max() < 1e-10

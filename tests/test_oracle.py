@@ -329,3 +329,73 @@ def test_apply_string_out_of_space_raises():
     psi = np.ones(len(ci.idx2det))
     with pytest.raises(KeyError):
         apply_string(psi, ci, ((0, True), (1, False)), 1.0, np.zeros_like(psi))
+
+
+def _nonredundant_kappa_idx(ni, na, nv, active_kappa=False):
+    """Orbital pairs (p < q) the wave functions optimise (ups_wavefunction.py:99-140):
+    every pair except inactive-inactive, virtual-virtual and (unless
+    active_kappa) active-active."""
+    inact = set(range(ni))
+    act = set(range(ni, ni + na))
+    virt = set(range(ni + na, ni + na + nv))
+    out = []
+    for p in range(ni + na + nv):
+        for q in range(p + 1, ni + na + nv):
+            if (p in inact and q in inact) or (p in virt and q in virt):
+                continue
+            if not active_kappa and p in act and q in act:
+                continue
+            out.append((p, q))
+    return out
+
+
+def test_orbital_gradient_against_central_differences():
+    """get_orbital_gradient (density_matrix.py:181-230, restated loop by loop)
+    against central differences of E(kappa) = <psi|H(C expm(-kappa))|psi>
+    (h = 1e-5, tol 1e-8) for tUPS on linear H6 STO-3G with a (2e,2o) active
+    space (two inactive, two virtual orbitals), at displaced orbitals; and
+    get_electronic_energy(Gamma) equal to <psi|H_folded|psi>."""
+    from oracle import orbital as O
+    from oracle.rdm import rdm1, rdm2
+    from oracle.ups import construct_ups_state
+    from slowquant_amd.hchain import linear_h_chain
+    from slowquant_amd.util import UpsStructure
+
+    sq = linear_h_chain(6, 0.8)
+    ni, na, nv = 2, 2, 2
+    h_ao = sq.integral.kinetic_energy_matrix + sq.integral.nuclear_attraction_matrix
+    g_ao = sq.integral.electron_repulsion_tensor
+    lay = UpsStructure()
+    lay.create_tiled(na, {"n_layers": 1, "do_tups": True})
+    rng = np.random.default_rng(5)
+    theta = rng.normal(0.0, 0.1, lay.n_params)
+    idx = _nonredundant_kappa_idx(ni, na, nv)
+    c0 = O.c_mo(sq.hartree_fock.mo_coeff, rng.normal(0.0, 0.05, len(idx)), idx)
+
+    def state(ci):
+        csf = np.zeros(len(ci.idx2det))
+        csf[0] = 1.0
+        return construct_ups_state(csf, ci, theta, lay)
+
+    zero = np.zeros(len(idx))
+    e0, h, g, psi, ci = O.energy_at(c0, h_ao, g_ao, zero, idx, state, ni, na, nv, 1, 1)
+    r1 = rdm1(psi, ci)
+    r2 = rdm2(psi, ci, r1)
+    assert abs(O.get_electronic_energy(h, g, ni, na, r1, r2) - e0) < 1e-12
+    grad = O.get_orbital_gradient(h, g, idx, ni, na, r1, r2)
+    hstep = 1e-5
+    fd = np.zeros(len(idx))
+    for i in range(len(idx)):
+        kp = zero.copy()
+        kp[i] = hstep
+        km = zero.copy()
+        km[i] = -hstep
+        ep = O.energy_at(c0, h_ao, g_ao, kp, idx, state, ni, na, nv, 1, 1)[0]
+        em = O.energy_at(c0, h_ao, g_ao, km, idx, state, ni, na, nv, 1, 1)[0]
+        fd[i] = (ep - em) / (2 * hstep)
+    assert np.abs(grad).max() > 1e-3  # displaced orbitals: a non-trivial gradient
+    assert np.abs(grad - fd).max() < 1e-8
+    # the drop-in's vectorised host version agrees with the loops
+    from slowquant_amd.density_matrix import get_orbital_gradient
+
+    assert np.abs(get_orbital_gradient(h, g, idx, ni, na, r1, r2) - grad).max() < 1e-12
