@@ -117,7 +117,11 @@ int64_t sq_work_bytes(const sq_space* s, int what);
 
 /* sigma = H psi and, if energy != NULL, energy = <psi|H|psi> (host double)
  * (propagate_state([H]) + expectation_value, operator_state_algebra.py:472-824).
- * D build (E_pq psi gathers) -> G = g.D (FP64 MFMA) -> sigma gather. */
+ * Spaces of up to 4096 beta strings: Ma C (the alpha-string Hamiltonian times
+ * psi as a string matrix, FP64 MFMA) and one row kernel per group of alpha
+ * rows that contracts g with E_pq psi on FP64 MFMA and applies the beta
+ * excitations with the intermediate kept in LDS.  Larger spaces: D build
+ * (E_pq psi gathers) -> G = g.D (FP64 MFMA) -> sigma gather. */
 int sq_sigma(const sq_space* s, const sq_hamiltonian* H, const double* psi, double* sigma, double* energy,
              double* work, void* stream);
 

@@ -14,6 +14,7 @@ SOURCES = [
     "sq_space.hip",
     "sq_gates.hip",
     "sq_sigma.hip",
+    "sq_rowsigma.hip",
     "sq_grad.hip",
     "sq_profile.hip",
     "sq_tiles.hip",

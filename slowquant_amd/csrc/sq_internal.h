@@ -113,10 +113,19 @@ struct Space {
   int32_t* d_bexc_full = nullptr;
   int2* d_gta = nullptr;  // sigma GEMM operand tables [K4 + 16][nsa], [K4 + 16][nsb] (sq_sigma.hip)
   int2* d_gtb = nullptr;
+  // row sigma tables (sq_rowsigma.hip): operand rows [K8 + 8][nsa], [K8 + 16][nsb],
+  // beta neighbour lists by partner column with per-tile offsets
+  int2* d_rta = nullptr;
+  int2* d_rtb = nullptr;
+  uint32_t* d_nbr = nullptr;
+  int32_t* d_nbr_off = nullptr;
+  uint8_t* d_nbr_tile = nullptr;
   std::vector<uint32_t> h_astr, h_bstr;
 };
 
 void build_gemm_tables(Space& s);
+void build_row_tables(Space& s);
+void free_row_tables(Space& s);
 
 struct TilePlan;
 TilePlan* build_tile_plan(const Space& sp, const struct Program& P);
@@ -162,7 +171,17 @@ struct Hamiltonian {
   int kpad;            // padded K (= M) of the packed GEMM, multiple of 16
   double e_core;
   double* d_A = nullptr;  // [kpad][kpad] transposed coefficient matrix (see sq_sigma.hip)
+  double* d_Ma = nullptr;  // [nsa][nsa] alpha-string Hamiltonian + e_core (sq_rowsigma.hip)
 };
+
+// Row sigma (sq_rowsigma.hip)
+bool row_sigma_applies(const Space& s);
+int64_t row_sigma_work_doubles(const Space& s);
+int64_t row_sigma_used_doubles(const Space& s);
+void build_alpha_ham(const Space& s, Hamiltonian& H);
+int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* work,
+                  double** partial_out, hipStream_t st);
+void row_sigma_configure(int rows_per_block, int nsplit);
 
 // Kernel timing classes (sq_profile.hip).
 enum ProfClass {
@@ -180,6 +199,8 @@ enum ProfClass {
   PC_RDM_GRAM,
   PC_BATCH,
   PC_OTHER,
+  PC_SIGMA_ROWS,
+  PC_ALPHA_GEMM,
   kProfClasses
 };
 bool prof_on();

@@ -42,7 +42,8 @@ thread_local hipEvent_t t_start = nullptr;
 
 const char* kClassNames[kProfClasses] = {"rotate", "rotate_adjoint", "t_apply", "d_build", "sigma_gemm_mfma",
                                          "sigma_gather", "reduce", "copy", "tile_forward", "tile_reverse",
-                                         "perm", "rdm_gram_mfma", "batch", "other"};
+                                         "perm", "rdm_gram_mfma", "batch", "other",
+                                         "sigma_rows_mfma", "alpha_gemm_mfma"};
 
 bool prof_on() { return prof().on; }
 

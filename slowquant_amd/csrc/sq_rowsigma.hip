@@ -1,0 +1,588 @@
+// sigma = H psi by alpha rows, with no intermediate in HBM (replaces the
+// H application of propagate_state([H]) / expectation_value,
+// operator_state_algebra.py:472-824, for H = hamiltonian_0i_0a,
+// operators.py:476-529).
+//
+// Spin split.  With E+_k = E+a_k + E+b_k over packed pairs k = (p >= q),
+// A(k,k') = 1/2 g(k,k') and h'_k = h_pq - 1/2 sum_r g_prrq (the alpha and
+// beta parts commute):
+//     H = Ha + Hb + sum_{k,k'} g(k,k') E+b_k E+a_k',
+//     Ha = sum_k h'_k E+a_k + sum_{k,k'} A(k,k') E+a_k E+a_k'   (Hb alike),
+// so
+//     sigma = Ma C + sum_k E+b_k G_k,
+//     G_k   = h'_k psi + sum_k' A(k,k') (2 E+a_k' + E+b_k') psi,
+// where C is psi as an [alpha string][beta string] matrix and Ma (+ e_core
+// on its diagonal) is the matrix of Ha over alpha strings, built once per
+// Hamiltonian (k_alpha_ham).
+//   1. k_alpha_gemm: S = Ma C, FP64 MFMA, split over the contraction index
+//      into nsplit partial products (summed in a fixed order in step 2).
+//   2. k_sigma_rows: one block per group of R alpha rows.  Row I_a of G is
+//      A . B with B[k'][I_b] = 2 ca_k' psi[Pa_k'(I_a), I_b] + cb_k'(I_b)
+//      psi[I_a, Pb_k'(I_b)] (rows npair: psi), generated in registers (the
+//      alpha partner row is uniform over the row, the beta partner is read
+//      from the row staged in LDS) and contracted on FP64 MFMA.  Each wave
+//      computes one 16-column tile of G for all k per round; the tiles of a
+//      round are staged in LDS and every target amplitude I_b of the rows
+//      adds, in a fixed order, the staged G_k[I_a, J_b] of its beta
+//      neighbours (J_b = partner of I_b under E+b_k).  G never leaves LDS.
+// Deterministic: every sum runs in a fixed order.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <vector>
+
+#include "sq_internal.h"
+
+namespace sq {
+
+void reduce_impl(const double* partial, int64_t n, double* out_dev, int slot, bool accumulate, double factor,
+                 hipStream_t st);
+
+namespace {
+
+constexpr int kRowThreads = 512;
+constexpr int kRowWaves = kRowThreads / 64;
+constexpr int kMaxTargets = 8;  // targets (alpha row, beta string) per thread
+constexpr size_t kLdsMax = 160 * 1024 - 256;  // dynamic LDS per block (the static part stays below 256 bytes)
+constexpr int kMaxSplit = 4;     // alpha GEMM partial products (workspace is sized for this many)
+
+__host__ __device__ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
+
+// ---------------------------------------------------------------------------
+// Space tables.
+// rta[k][s] (k < K8 + 8): {partner row * nsb, 2 * coefficient} of E+a_k on
+// alpha string s (diagonal: 2 * occupation; absent / k >= npair: {own, 0}).
+// rtb[k][s] (k < K8 + 16): {partner column, coefficient} of E+b_k on beta
+// string s (diagonal: occupation; k == npair: {own, 1}; absent: {own, 0}).
+__global__ void k_row_tables(int npair, int nstr, int nsb, int alpha, const int32_t* __restrict__ exc,
+                             int2* __restrict__ tab) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = blockIdx.y;
+  if (s >= nstr) return;
+  const int own = alpha ? s * nsb : s;
+  const int f = alpha ? 2 : 1;
+  int2 e = make_int2(own, 0);
+  if (k < npair) {
+    int p = 0;
+    while ((p + 1) * (p + 2) / 2 <= k) ++p;
+    const int q = k - p * (p + 1) / 2;
+    const int32_t x = exc[(size_t)k * nstr + s];
+    if (p == q) e = make_int2(own, f * x);
+    else if (x != 0) e = make_int2(alpha ? (abs(x) - 1) * nsb : abs(x) - 1, x > 0 ? f : -f);
+  } else if (k == npair && !alpha) {
+    e = make_int2(own, 1);
+  }
+  tab[(size_t)k * nstr + s] = e;
+}
+
+// Beta neighbour lists: for target string s, the entries (J, k, sign) with
+// (E+b_k)[s][J] != 0 (J = partner, or s itself with occupation 1 on the
+// diagonal), packed J | k << 14 | negative << 31 and sorted by J; tile[s][c]
+// (c = 0..tpr) counts the entries with J < 16 c.
+__global__ void k_nbr_count(int npair, int nsb, const int32_t* __restrict__ bexc, int32_t* __restrict__ cnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsb) return;
+  int c = 0;
+  for (int k = 0; k < npair; ++k) c += bexc[(size_t)k * nsb + s] != 0;
+  cnt[s] = c;
+}
+
+__global__ void k_nbr_fill(int npair, int nsb, int tpr, const int32_t* __restrict__ bexc,
+                           const int32_t* __restrict__ off, uint32_t* __restrict__ nbr, uint8_t* __restrict__ tile) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsb) return;
+  uint32_t* out = nbr + off[s];
+  int m = 0, k = 0;
+  for (int p = 0; p < 32 && k < npair; ++p) {
+    for (int q = 0; q <= p; ++q, ++k) {
+      const int32_t x = bexc[(size_t)k * nsb + s];
+      if (x == 0) continue;
+      uint32_t e;
+      if (p == q) e = (uint32_t)s | ((uint32_t)k << 14);
+      else e = (uint32_t)(abs(x) - 1) | ((uint32_t)k << 14) | (x < 0 ? 0x80000000u : 0u);
+      // insertion by partner column (stable: equal columns keep k order)
+      int j = m++;
+      while (j > 0 && (out[j - 1] & 0x3FFFu) > (e & 0x3FFFu)) {
+        out[j] = out[j - 1];
+        --j;
+      }
+      out[j] = e;
+    }
+  }
+  int e = 0;
+  for (int c = 0; c <= tpr; ++c) {
+    while (e < m && (int)(out[e] & 0x3FFFu) < 16 * c) ++e;
+    tile[(size_t)s * (tpr + 1) + c] = (uint8_t)e;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ma[J][I] = <I|Ha|J> + e_core delta_IJ (symmetric), one block per column J:
+//   Ha|J> = sum_k E+a_k u_k,  u_k = h'_k |J> + sum_k' A(k,k') E+a_k' |J>,
+// where E+a_k'|J> is J itself (diagonal k', coefficient = occupation) or one
+// single excitation of J (distinct for distinct off-diagonal k'): u_k lives
+// on at most 1 + Na (n - Na) strings ("slots"), and
+//   <I|Ha|J> = sum_k c_k(I) u_k[slot(P_k(I))].
+__global__ void k_alpha_ham(int nsa, int npair, int kpad, const int32_t* __restrict__ aexc,
+                            const double* __restrict__ At, double e_core, double* __restrict__ Ma) {
+  extern __shared__ unsigned char smem_raw[];
+  const int J = blockIdx.x;
+  // LDS: slot of every string (-1: none) | k' -> slot | k' coefficient | u[npair][nslot_max]
+  int16_t* slotmap = reinterpret_cast<int16_t*>(smem_raw);
+  int16_t* kslot = slotmap + rup(nsa, 8);
+  int32_t* kcoef = reinterpret_cast<int32_t*>(kslot + rup(npair, 8));
+  double* u = reinterpret_cast<double*>(kcoef + rup(npair, 2));
+  __shared__ int nslot;
+  for (int i = threadIdx.x; i < nsa; i += blockDim.x) slotmap[i] = -1;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    slotmap[J] = 0;
+    int ns = 1;
+    int k = 0;
+    for (int p = 0; k < npair; ++p)
+      for (int q = 0; q <= p; ++q, ++k) {
+        const int32_t x = aexc[(size_t)k * nsa + J];
+        if (p == q || x == 0) {
+          kslot[k] = 0;
+          kcoef[k] = p == q ? x : 0;
+        } else {
+          const int P = abs(x) - 1;
+          slotmap[P] = (int16_t)ns;
+          kslot[k] = (int16_t)ns++;
+          kcoef[k] = x > 0 ? 1 : -1;
+        }
+      }
+    nslot = ns;
+  }
+  __syncthreads();
+  const int ns = nslot;
+  // u[k][slot] in a fixed k' order
+  for (int i = threadIdx.x; i < npair * ns; i += blockDim.x) {
+    const int k = i / ns, sl = i - k * ns;
+    double v = sl == 0 ? At[(size_t)npair * kpad + k] : 0.0;
+    for (int kp = 0; kp < npair; ++kp)
+      if (kslot[kp] == sl && kcoef[kp] != 0) v += At[(size_t)kp * kpad + k] * (double)kcoef[kp];
+    u[i] = v;
+  }
+  __syncthreads();
+  for (int I = threadIdx.x; I < nsa; I += blockDim.x) {
+    double v = I == J ? e_core : 0.0;
+    int k = 0;
+    for (int p = 0; k < npair; ++p)
+      for (int q = 0; q <= p; ++q, ++k) {
+        const int32_t x = aexc[(size_t)k * nsa + I];
+        if (x == 0) continue;
+        const int K = p == q ? I : abs(x) - 1;
+        const int sl = slotmap[K];
+        if (sl < 0) continue;
+        const double c = p == q ? (double)x : (x > 0 ? 1.0 : -1.0);
+        v += c * u[k * ns + sl];
+      }
+    Ma[(size_t)J * nsa + I] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// out[z][I][c] = sum_{J in split z} Ma[I][J] C[J][c] (Ma nsa x nsa, C nsa x nsb).
+// 256 threads, 64 x 64 output tile; wave w owns rows 32 (w >> 1) and columns
+// 32 (w & 1) as 2 x 2 tiles of v_mfma_f64_16x16x4f64; the contraction runs
+// in 16-deep slices staged in LDS (the next slice loaded into registers
+// during the MFMAs of the current one).
+__global__ __launch_bounds__(256) void k_alpha_gemm(const double* __restrict__ Ma, const double* __restrict__ C,
+                                                    double* __restrict__ out, int nsa, int nsb, int kchunk) {
+  typedef double double4_t __attribute__((ext_vector_type(4)));
+  __shared__ double sM[16][65];  // [k][i]
+  __shared__ double sC[16][64];  // [k][c]
+  const int i0 = blockIdx.y * 64, c0 = blockIdx.x * 64;
+  const int kb = blockIdx.z * kchunk, ke = min(nsa, kb + kchunk);
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, kq = lane >> 4, li = lane & 15;
+  const int wi = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  double4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+  // loaders: Ma slice 64 x 16 (thread: row t/4, columns 4 (t%4) ..), C slice 16 x 64 (row t/16, columns 4 (t%16) ..)
+  const int mr = t >> 2, mc = (t & 3) * 4;
+  const int cr = t >> 4, cc = (t & 15) * 4;
+  double rm[4], rc[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int I = i0 + mr, K = k0 + mc + j;
+      rm[j] = (I < nsa && K < ke) ? Ma[(size_t)I * nsa + K] : 0.0;
+      const int Kc = k0 + cr, c = c0 + cc + j;
+      rc[j] = (Kc < ke && c < nsb) ? C[(size_t)Kc * nsb + c] : 0.0;
+    }
+  };
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      sM[mc + j][mr] = rm[j];
+      sC[cr][cc + j] = rc[j];
+    }
+    __syncthreads();
+    if (k0 + 16 < ke) load(k0 + 16);
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const double a0 = sM[kk + kq][wi + li], a1 = sM[kk + kq][wi + 16 + li];
+      const double b0 = sC[kk + kq][wc + li], b1 = sC[kk + kq][wc + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+  double* o = out + (size_t)blockIdx.z * nsa * nsb;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int I = i0 + wi + a * 16 + kq + 4 * r, c = c0 + wc + b * 16 + li;
+        if (I < nsa && c < nsb) o[(size_t)I * nsb + c] = acc[a][b][r];
+      }
+}
+
+// ---------------------------------------------------------------------------
+// The row kernel (see the top of this file).  LDS: A slice [K8][W] | staged
+// G tiles [waves][W][16] | psi rows [R][nsb] | alpha operand rows [R][K8 + 8].
+template <int MT>
+__global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
+    const double* __restrict__ At, int kpad, int K4, int K8, int mgroups, const double* __restrict__ psi,
+    const double* __restrict__ sin, int nsplit, double* __restrict__ sigma, double* __restrict__ partial,
+    const int2* __restrict__ rta, const int2* __restrict__ rtb, const uint32_t* __restrict__ nbr,
+    const int32_t* __restrict__ nbr_off, const uint8_t* __restrict__ nbr_tile, int nsa, int nsb, int R) {
+  typedef double double4_t __attribute__((ext_vector_type(4)));
+  constexpr int W = MT * 16;
+  extern __shared__ double smem[];
+  __shared__ double red[kRowWaves];
+  const int tpr = (nsb + 15) >> 4;
+  const int RA = K8 + 8;
+  double* sA = smem;
+  double* sG = sA + (size_t)K8 * W;
+  double* sPsi = sG + (size_t)kRowWaves * W * 16;
+  int2* sRA = reinterpret_cast<int2*>(sPsi + (size_t)R * nsb);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, li = lane & 15;
+  const int row0 = blockIdx.x * R;
+  const int nr = min(R, nsa - row0);
+  const int ntg = nr * nsb;
+  const size_t base = (size_t)row0 * nsb;
+  for (int i = tid; i < ntg; i += kRowThreads) sPsi[i] = psi[base + i];
+  for (int i = tid; i < nr * RA; i += kRowThreads) {
+    const int r = i / RA, k = i - r * RA;
+    sRA[i] = rta[(size_t)k * nsa + row0 + r];
+  }
+  double tacc[kMaxTargets];
+#pragma unroll
+  for (int i = 0; i < kMaxTargets; ++i) tacc[i] = 0.0;
+  const int ntiles = nr * tpr;
+  const int nsteps = K8 / 4;
+  for (int mg = 0; mg < mgroups; ++mg) {
+    const int m0 = mg * W;
+    __syncthreads();  // the previous m-group's reads of sA are done
+    for (int i = tid; i < K8 * W; i += kRowThreads) {
+      const int k = i / W, m = m0 + (i - k * W);
+      sA[i] = (k < K4 && m < kpad) ? At[(size_t)k * kpad + m] : 0.0;
+    }
+    __syncthreads();
+    for (int f0 = 0; f0 < ntiles; f0 += kRowWaves) {
+      const int f = f0 + wave;
+      if (f < ntiles) {
+        const int r = f / tpr, c = f - r * tpr;
+        const int ib = c * 16 + li;
+        const uint32_t ibc = (uint32_t)(ib < nsb ? ib : nsb - 1);
+        const double* ps = sPsi + (size_t)r * nsb;
+        const int2* ra = sRA + (size_t)r * RA;
+        double4_t acc[MT];
+#pragma unroll
+        for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+        const uint32_t sb = 4u * (uint32_t)nsb;
+        const uint32_t tb_i = (uint32_t)kq * (uint32_t)nsb + ibc;
+        // slot 0: even steps, slot 1: odd steps
+        int2 tb0 = rtb[tb_i], tb1 = rtb[tb_i + sb];
+        int2 a = ra[kq];
+        double xa0 = psi[(uint32_t)a.x + ibc];
+        int ca0 = a.y;
+        double xb0 = ps[tb0.x];
+        int cb0 = tb0.y;
+        tb0 = rtb[tb_i + 2 * sb];
+        double xa1, xb1;
+        int ca1, cb1;
+        const double* arow = sA + (size_t)kq * W + li;
+        for (int s = 0; s < nsteps; s += 2) {
+          a = ra[(s + 1) * 4 + kq];
+          xa1 = psi[(uint32_t)a.x + ibc];
+          ca1 = a.y;
+          xb1 = ps[tb1.x];
+          cb1 = tb1.y;
+          tb1 = rtb[tb_i + (uint32_t)(s + 3) * sb];
+          __asm__ volatile("" ::: "memory");
+          {
+            const double b = (double)ca0 * xa0 + (double)cb0 * xb0;
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+          }
+          arow += 4 * W;
+          a = ra[(s + 2) * 4 + kq];
+          xa0 = psi[(uint32_t)a.x + ibc];
+          ca0 = a.y;
+          xb0 = ps[tb0.x];
+          cb0 = tb0.y;
+          tb0 = rtb[tb_i + (uint32_t)(s + 4) * sb];
+          __asm__ volatile("" ::: "memory");
+          {
+            const double b = (double)ca1 * xa1 + (double)cb1 * xb1;
+#pragma unroll
+            for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+          }
+          arow += 4 * W;
+        }
+        double* g = sG + (size_t)wave * W * 16 + li;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) g[(t * 16 + kq + 4 * q) * 16] = acc[t][q];
+      }
+      __syncthreads();
+      // every target adds its beta neighbours' staged G_k values, by partner column
+#pragma unroll
+      for (int i = 0; i < kMaxTargets; ++i) {
+        const int tau = tid + i * kRowThreads;
+        if (tau >= ntg) break;
+        const int r = tau / nsb, ib = tau - r * nsb;
+        const int lo = max(f0, r * tpr), hi = min(f0 + kRowWaves, (r + 1) * tpr);
+        if (lo >= hi) continue;
+        const uint8_t* tl = nbr_tile + (size_t)ib * (tpr + 1);
+        const int eb = nbr_off[ib];
+        const int e0 = eb + tl[lo - r * tpr], e1 = eb + tl[hi - r * tpr];
+        const int fb = r * tpr - f0;
+        double v = tacc[i];
+        for (int e = e0; e < e1; ++e) {
+          const uint32_t x = nbr[e];
+          const int jb = (int)(x & 0x3FFFu), k = (int)((x >> 14) & 0xFFu) - m0;
+          if (k < 0 || k >= W) continue;
+          const double gv = sG[((size_t)(fb + (jb >> 4)) * W + k) * 16 + (jb & 15)];
+          v += (x >> 31) ? -gv : gv;
+        }
+        tacc[i] = v;
+      }
+      __syncthreads();
+    }
+  }
+  const size_t dim = (size_t)nsa * nsb;
+  double e = 0.0;
+#pragma unroll
+  for (int i = 0; i < kMaxTargets; ++i) {
+    const int tau = tid + i * kRowThreads;
+    if (tau >= ntg) break;
+    double v = tacc[i];
+    for (int z = 0; z < nsplit; ++z) v += sin[(size_t)z * dim + base + tau];
+    sigma[base + tau] = v;
+    e += v * sPsi[tau];
+  }
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+  if (lane == 0) red[wave] = e;
+  __syncthreads();
+  if (tid == 0) {
+    double tsum = 0.0;
+    for (int w = 0; w < kRowWaves; ++w) tsum += red[w];
+    partial[blockIdx.x] = tsum;
+  }
+}
+
+struct RowPlan {
+  int mt, mgroups, R, ngroups, nsplit, kchunk;
+  size_t lds;
+};
+
+int row_k8(const Space& s) { return rup(s.npair + 1, 8); }
+
+size_t row_lds(const Space& s, int mt, int R) {
+  const int K8 = row_k8(s);
+  const int W = mt * 16;
+  return 8 * ((size_t)K8 * W + (size_t)kRowWaves * W * 16 + (size_t)R * s.nsb + (size_t)R * (K8 + 8));
+}
+
+int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return (e && e[0]) ? atoi(e) : dflt;
+}
+
+// Kernel shape: the m-tile group (largest that fits, then balanced), rows
+// per block R (fewest block-rounds on 256 CUs, one block per CU), the split
+// of the alpha GEMM.  Overrides: sq_sigma_configure().
+struct RowOverride {
+  int R = 0, nsplit = 0;
+};
+RowOverride g_row_override;
+
+bool row_plan(const Space& s, RowPlan* out) {
+  if (s.dim == 0 || s.nsb > 16384 || s.npair + 1 > 255 || s.dim >= ((int64_t)1 << 31)) return false;
+  const int kpad = rup(s.npair + 1, 16);
+  const int mt_total = kpad / 16;
+  int mt = std::min(mt_total, 9);
+  while (mt > 1 && row_lds(s, mt, 1) > kLdsMax) --mt;
+  if (row_lds(s, mt, 1) > kLdsMax) return false;
+  const int mgroups = (mt_total + mt - 1) / mt;
+  mt = (mt_total + mgroups - 1) / mgroups;
+  const int tpr = (s.nsb + 15) / 16;
+  int bestR = 0;
+  int64_t best = -1;
+  for (int R = 1; R <= 64 && R <= s.nsa; ++R) {
+    if (row_lds(s, mt, R) > kLdsMax) break;
+    if ((int64_t)R * s.nsb > (int64_t)kMaxTargets * kRowThreads) break;
+    const int64_t groups = (s.nsa + R - 1) / R;
+    const int64_t cost = ((groups + 255) / 256) * (((int64_t)R * tpr + kRowWaves - 1) / kRowWaves);
+    if (best < 0 || cost < best) {
+      best = cost;
+      bestR = R;
+    }
+  }
+  if (bestR == 0) return false;
+  if (g_row_override.R > 0) {
+    const int R = g_row_override.R;
+    if (R <= s.nsa && row_lds(s, mt, R) <= kLdsMax && (int64_t)R * s.nsb <= (int64_t)kMaxTargets * kRowThreads)
+      bestR = R;
+  }
+  out->mt = mt;
+  out->mgroups = mgroups;
+  out->R = bestR;
+  out->ngroups = (s.nsa + bestR - 1) / bestR;
+  int nsplit = g_row_override.nsplit > 0 ? g_row_override.nsplit : (s.nsa >= 2048 ? 1 : 2);
+  nsplit = std::max(1, std::min({nsplit, kMaxSplit, (s.nsa + 15) / 16}));
+  out->kchunk = rup((s.nsa + nsplit - 1) / nsplit, 16);
+  out->nsplit = (s.nsa + out->kchunk - 1) / out->kchunk;
+  out->lds = row_lds(s, mt, bestR);
+  return true;
+}
+
+bool rows_enabled() {
+  static int v = -1;
+  if (v < 0) v = env_int("SQ_SIGMA_ROWS", 1) != 0 ? 1 : 0;
+  return v == 1;
+}
+
+}  // namespace
+
+void row_sigma_configure(int rows_per_block, int nsplit) {
+  g_row_override.R = std::max(0, rows_per_block);
+  g_row_override.nsplit = std::max(0, nsplit);
+}
+
+bool row_sigma_applies(const Space& s) {
+  RowPlan pl;
+  return rows_enabled() && s.d_rtb != nullptr && row_plan(s, &pl);
+}
+
+// doubles of workspace: alpha GEMM partials [nsplit][dim] | block energy
+// partials [ngroups], sized for the largest shape (nsplit = kMaxSplit,
+// ngroups = nsa) | scalars [64] at the end.
+int64_t row_sigma_work_doubles(const Space& s) {
+  return (int64_t)kMaxSplit * s.dim + s.nsa + 64;
+}
+
+void build_row_tables(Space& s) {
+  RowPlan pl;
+  if (!row_plan(s, &pl)) return;
+  const int K8 = row_k8(s);
+  const int tpr = (s.nsb + 15) / 16;
+  SQ_HIP(hipMalloc(&s.d_rta, sizeof(int2) * (size_t)(K8 + 8) * s.nsa));
+  SQ_HIP(hipMalloc(&s.d_rtb, sizeof(int2) * (size_t)(K8 + 16) * s.nsb));
+  k_row_tables<<<dim3((s.nsa + 255) / 256, K8 + 8), 256>>>(s.npair, s.nsa, s.nsb, 1, s.d_aexc, s.d_rta);
+  SQ_HIP(hipGetLastError());
+  k_row_tables<<<dim3((s.nsb + 255) / 256, K8 + 16), 256>>>(s.npair, s.nsb, s.nsb, 0, s.d_bexc, s.d_rtb);
+  SQ_HIP(hipGetLastError());
+  int32_t* cnt = nullptr;
+  SQ_HIP(hipMalloc(&cnt, sizeof(int32_t) * s.nsb));
+  k_nbr_count<<<(s.nsb + 255) / 256, 256>>>(s.npair, s.nsb, s.d_bexc, cnt);
+  SQ_HIP(hipGetLastError());
+  std::vector<int32_t> h(s.nsb), off(s.nsb + 1, 0);
+  SQ_HIP(hipMemcpy(h.data(), cnt, sizeof(int32_t) * s.nsb, hipMemcpyDeviceToHost));
+  for (int i = 0; i < s.nsb; ++i) off[i + 1] = off[i] + h[i];
+  (void)hipFree(cnt);
+  SQ_HIP(hipMalloc(&s.d_nbr_off, sizeof(int32_t) * (s.nsb + 1)));
+  SQ_HIP(hipMemcpy(s.d_nbr_off, off.data(), sizeof(int32_t) * (s.nsb + 1), hipMemcpyHostToDevice));
+  SQ_HIP(hipMalloc(&s.d_nbr, sizeof(uint32_t) * std::max(off[s.nsb], 1)));
+  SQ_HIP(hipMalloc(&s.d_nbr_tile, (size_t)s.nsb * (tpr + 1)));
+  k_nbr_fill<<<(s.nsb + 255) / 256, 256>>>(s.npair, s.nsb, tpr, s.d_bexc, s.d_nbr_off, s.d_nbr, s.d_nbr_tile);
+  SQ_HIP(hipGetLastError());
+  SQ_HIP(hipDeviceSynchronize());
+}
+
+void free_row_tables(Space& s) {
+  for (void* p : {(void*)s.d_rta, (void*)s.d_rtb, (void*)s.d_nbr, (void*)s.d_nbr_off, (void*)s.d_nbr_tile})
+    if (p) (void)hipFree(p);
+  s.d_rta = s.d_rtb = nullptr;
+  s.d_nbr = nullptr;
+  s.d_nbr_off = nullptr;
+  s.d_nbr_tile = nullptr;
+}
+
+// Ma (+ e_core on the diagonal) for a Hamiltonian whose coefficient matrix
+// is already on the device.
+void build_alpha_ham(const Space& s, Hamiltonian& H) {
+  if (s.d_rtb == nullptr || s.nsa == 0) return;
+  SQ_HIP(hipMalloc(&H.d_Ma, sizeof(double) * (size_t)s.nsa * s.nsa));
+  const int nslot_max = 1 + s.na * (s.n - s.na);
+  const size_t lds = 2 * (size_t)rup(s.nsa, 8) + 2 * (size_t)rup(s.npair, 8) + 4 * (size_t)rup(s.npair, 2) +
+                     8 * (size_t)s.npair * nslot_max;
+  SQ_CHECK(lds <= kLdsMax, SQ_ERR_ARG, "alpha Hamiltonian block does not fit in LDS");
+  allow_lds(k_alpha_ham, lds);
+  k_alpha_ham<<<s.nsa, 256, lds>>>(s.nsa, s.npair, H.kpad, s.d_aexc, H.d_A, H.e_core, H.d_Ma);
+  SQ_HIP(hipGetLastError());
+  SQ_HIP(hipDeviceSynchronize());
+}
+
+// sigma = H psi and the block partials of <psi|sigma> (*partial_out, the
+// count is returned).
+int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* work,
+                  double** partial_out, hipStream_t st) {
+  RowPlan pl;
+  SQ_CHECK(row_plan(s, &pl) && H.d_Ma, SQ_ERR_ARG, "row sigma does not apply to this space");
+  SQ_CHECK(pl.nsplit <= kMaxSplit && pl.ngroups <= s.nsa, SQ_ERR_ARG, "row sigma shape exceeds its workspace");
+  double* sinp = work;
+  double* partial = work + (size_t)pl.nsplit * s.dim;
+  *partial_out = partial;
+  {
+    dim3 grid((s.nsb + 63) / 64, (s.nsa + 63) / 64, pl.nsplit);
+    const double flops = 2.0 * (double)s.nsa * s.nsa * s.nsb;
+    const double bytes = 8.0 * ((double)s.nsa * s.nsa + (double)s.dim * (1 + pl.nsplit));
+    SQ_PROF(PC_ALPHA_GEMM, st, bytes, flops,
+            k_alpha_gemm<<<grid, 256, 0, st>>>(H.d_Ma, psi, sinp, s.nsa, s.nsb, pl.kchunk));
+  }
+  const int K8 = row_k8(s);
+  const int K4 = rup(s.npair + 1, 4);
+  const double flops = 2.0 * (double)s.npair * (s.npair + 1) * (double)s.dim;
+  // algorithmic traffic: psi read (rows and alpha partners through the cache), the alpha partials read, sigma written
+  const double bytes = 8.0 * (double)s.dim * (2 + pl.nsplit);
+  switch (pl.mt) {
+#define SQ_RS(MT)                                                                                               \
+  case MT:                                                                                                      \
+    allow_lds(k_sigma_rows<MT>, pl.lds);                                                                        \
+    SQ_PROF(PC_SIGMA_ROWS, st, bytes, flops,                                                                    \
+            k_sigma_rows<MT><<<pl.ngroups, kRowThreads, pl.lds, st>>>(                                          \
+                H.d_A, H.kpad, K4, K8, pl.mgroups, psi, sinp, pl.nsplit, sigma, partial, s.d_rta, s.d_rtb,      \
+                s.d_nbr, s.d_nbr_off, s.d_nbr_tile, s.nsa, s.nsb, pl.R));                                       \
+    break;
+    SQ_RS(1) SQ_RS(2) SQ_RS(3) SQ_RS(4) SQ_RS(5) SQ_RS(6) SQ_RS(7) SQ_RS(8) SQ_RS(9)
+#undef SQ_RS
+    default:
+      throw Error{SQ_ERR_ARG, "unsupported orbital count for the row sigma"};
+  }
+  return pl.ngroups;
+}
+
+// Doubles of workspace the row sigma uses at its chosen shape (alpha GEMM
+// partials and block partials), or -1 when it does not apply.
+int64_t row_sigma_used_doubles(const Space& s) {
+  RowPlan pl;
+  if (!row_plan(s, &pl)) return -1;
+  return (int64_t)pl.nsplit * s.dim + pl.ngroups;
+}
+
+}  // namespace sq

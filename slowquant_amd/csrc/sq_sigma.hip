@@ -42,8 +42,10 @@ static SigmaDims sigma_dims(const Space& s) {
   return d;
 }
 
-// Work layout (doubles): D [K4*dim] | G [npair*dim] | partials [nsa*ceil(nsb/256)] | scalars [8]
+// Work layout (doubles): D [K4*dim] | G [npair*dim] | partials [nsa*ceil(nsb/256)] | scalars [8];
+// the row path (sq_rowsigma.hip) needs only its alpha GEMM partials and block partials.
 int64_t sigma_work_bytes(const Space& s) {
+  if (row_sigma_applies(s)) return 8 * row_sigma_work_doubles(s);
   SigmaDims d = sigma_dims(s);
   int64_t nb = (int64_t)s.nsa * ((s.nsb + kB - 1) / kB);
   return 8 * ((int64_t)d.K4 * d.dim + (int64_t)d.npair * d.dim + nb + 64);
@@ -391,6 +393,17 @@ void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double*
                 double* work, hipStream_t st) {
   SigmaDims d = sigma_dims(s);
   if (s.dim == 0) return;
+  if (H.d_Ma && row_sigma_applies(s)) {
+    double* partial = nullptr;
+    const int64_t nb = row_sigma(s, H, psi, sigma, work, &partial, st);
+    if (energy_host) {
+      double* scal = work + (row_sigma_work_doubles(s) - 64);
+      reduce_impl(partial, nb, scal, 0, false, 1.0, st);
+      SQ_HIP(hipMemcpyAsync(energy_host, scal, sizeof(double), hipMemcpyDeviceToHost, st));
+      SQ_HIP(hipStreamSynchronize(st));
+    }
+    return;
+  }
   const int32_t* ae = s.d_aexc;
   const int32_t* be = s.d_bexc;
   double* D = work;
@@ -423,6 +436,12 @@ void sigma_impl(const Space& s, const Hamiltonian& H, const double* psi, double*
 void sigma_energy_dev(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* e_dev,
                       double* work, hipStream_t st) {
   SigmaDims d = sigma_dims(s);
+  if (H.d_Ma && row_sigma_applies(s)) {
+    double* partial = nullptr;
+    const int64_t nb = row_sigma(s, H, psi, sigma, work, &partial, st);
+    reduce_impl(partial, nb, e_dev, 0, false, 1.0, st);
+    return;
+  }
   const int32_t* ae = s.d_aexc;
   const int32_t* be = s.d_bexc;
   double* D = work;
@@ -777,6 +796,14 @@ int sq_hamiltonian_create(const sq_space* sp, const double* h, const double* g, 
       delete ho;
       throw Error{SQ_ERR_HIP, hipGetErrorString(e)};
     }
+    try {
+      if (row_sigma_applies(s)) build_alpha_ham(s, ho->h);
+    } catch (...) {
+      if (ho->h.d_Ma) (void)hipFree(ho->h.d_Ma);
+      (void)hipFree(ho->h.d_A);
+      delete ho;
+      throw;
+    }
     *out = ho;
   });
 }
@@ -785,6 +812,7 @@ int sq_hamiltonian_destroy(sq_hamiltonian* H) {
   return guarded([&] {
     if (!H) return;
     if (H->h.d_A) (void)hipFree(H->h.d_A);
+    if (H->h.d_Ma) (void)hipFree(H->h.d_Ma);
     delete H;
   });
 }

@@ -204,6 +204,7 @@ int sq_space_create(int n_orb, int n_alpha, int n_beta, sq_space** out) {
       build_spin(n_orb, n_alpha, s.nsa, &s.d_astr, &s.d_arank, &s.d_aexc, &s.d_aexc_full, s.h_astr);
       build_spin(n_orb, n_beta, s.nsb, &s.d_bstr, &s.d_brank, &s.d_bexc, &s.d_bexc_full, s.h_bstr);
       build_gemm_tables(s);
+      build_row_tables(s);
     } catch (...) {
       sq_space_destroy(sp);
       throw;
@@ -216,6 +217,7 @@ int sq_space_destroy(sq_space* sp) {
   return guarded([&] {
     if (!sp) return;
     Space& s = sp->s;
+    free_row_tables(s);
     for (void* p : {(void*)s.d_astr, (void*)s.d_bstr, (void*)s.d_arank, (void*)s.d_brank, (void*)s.d_aexc,
                     (void*)s.d_bexc, (void*)s.d_aexc_full, (void*)s.d_bexc_full, (void*)s.d_gta, (void*)s.d_gtb})
       if (p) (void)hipFree(p);
