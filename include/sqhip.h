@@ -229,6 +229,31 @@ int sq_operator_destroy(sq_operator* op);
 /* out = O in (device vectors, internal convention, in != out). */
 int sq_operator_apply(const sq_space* s, const sq_operator* op, const double* in, double* out, void* stream);
 
+/* ---- Orbital rotations and integrals ---------------------------------------
+ * Matrices are row-major FP64 device arrays; every contraction is an FP64-MFMA
+ * GEMM.  work: sq_orbital_work_bytes(n_ao, n_mo, n_inactive, n_active, n_kappa)
+ * bytes (the largest of the three calls below for those sizes). */
+int64_t sq_orbital_work_bytes(int n_ao, int n_mo, int n_inactive, int n_active, int n_kappa);
+/* c_mo = C_old expm(-kappa) (ups_wavefunction.py:367-385): c_old and c_new
+ * are n_ao x n device arrays, kappa the n x n antisymmetric HOST matrix of the
+ * rotations moved since the last expansion point.  expm by scaling and
+ * squaring of a degree-14 Taylor polynomial. */
+int sq_orbital_rotation(int n, int n_ao, const double* c_old, const double* kappa, double* c_new, double* work,
+                        void* stream);
+/* one_electron_integral_transform / two_electron_integral_transform
+ * (integralfunctions.py:186-211): h_mo = C^T h_ao C (n_mo x n_mo) and
+ * g_mo[ijkl] = sum C_ai C_bj C_ck C_dl g_ao[abcd] (four quarter transforms);
+ * either pair may be NULL.  n_mo <= 255. */
+int sq_ao2mo(int n_ao, int n_mo, const double* c, const double* h_ao, const double* g_ao, double* h_mo,
+             double* g_mo, void* work, void* stream);
+/* get_orbital_gradient (density_matrix.py:181-230): <0|[kappa_mn, H]|0> for
+ * each (m, n) of kappa_idx (host, 2 * n_kappa ints) from the active-space
+ * rdm1 (n_active^2) and rdm2 (n_active^4) and the MO integrals (device);
+ * grad: host, n_kappa doubles. */
+int sq_orbital_gradient(int n_orb, int n_inactive, int n_active, const double* h_mo, const double* g_mo,
+                        const double* rdm1, const double* rdm2, int n_kappa, const int32_t* kappa_idx, double* grad,
+                        void* work, void* stream);
+
 /* ---- Kernel timing -------------------------------------------------------
  * No counterpart in the reference (its timing is wall-clock prints).  While
  * enabled, every engine kernel launch is bracketed by HIP events on its

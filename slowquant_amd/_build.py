@@ -20,6 +20,7 @@ SOURCES = [
     "sq_tiles.hip",
     "sq_ucc.hip",
     "sq_batch.hip",
+    "sq_orbital.hip",
 ]
 ARCH = os.environ.get("SQ_OFFLOAD_ARCH", "gfx950")
 

@@ -106,6 +106,10 @@ _SIGNATURES = {
     "sq_profile_nclasses": ([], i32),
     "sq_profile_class_name": ([i32], ctypes.c_char_p),
     "sq_profile_collect": ([i32, vp, vp, vp, vp], i32),
+    "sq_orbital_work_bytes": ([i32, i32, i32, i32, i32], i64),
+    "sq_orbital_rotation": ([i32, i32, dp, vp, dp, dp, vp], i32),
+    "sq_ao2mo": ([i32, i32, dp, dp, dp, dp, dp, dp, vp], i32),
+    "sq_orbital_gradient": ([i32, i32, i32, dp, dp, dp, dp, i32, vp, vp, dp, vp], i32),
 }
 
 # Entry points added by later translation units register themselves here.

@@ -201,6 +201,7 @@ enum ProfClass {
   PC_OTHER,
   PC_SIGMA_ROWS,
   PC_ALPHA_GEMM,
+  PC_ORBITAL,
   kProfClasses
 };
 bool prof_on();

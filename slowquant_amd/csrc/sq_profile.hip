@@ -43,7 +43,7 @@ thread_local hipEvent_t t_start = nullptr;
 const char* kClassNames[kProfClasses] = {"rotate", "rotate_adjoint", "t_apply", "d_build", "sigma_gemm_mfma",
                                          "sigma_gather", "reduce", "copy", "tile_forward", "tile_reverse",
                                          "perm", "rdm_gram_mfma", "batch", "other",
-                                         "sigma_rows_mfma", "alpha_gemm_mfma"};
+                                         "sigma_rows_mfma", "alpha_gemm_mfma", "orbital_gemm_mfma"};
 
 bool prof_on() { return prof().on; }
 

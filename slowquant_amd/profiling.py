@@ -11,7 +11,7 @@ from . import _native as N
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP64_MFMA_PEAK_TFLOPS = 78.6  # MI355X dense FP64 matrix
-MFMA_CLASSES = ("sigma_gemm_mfma", "sigma_rows_mfma", "alpha_gemm_mfma", "rdm_gram_mfma")
+MFMA_CLASSES = ("sigma_gemm_mfma", "sigma_rows_mfma", "alpha_gemm_mfma", "rdm_gram_mfma", "orbital_gemm_mfma")
 
 
 def collect() -> dict:

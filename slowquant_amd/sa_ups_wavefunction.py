@@ -26,8 +26,6 @@ from typing import Any
 import numpy as np
 import scipy.linalg
 
-from .density_matrix import get_orbital_gradient
-from .integrals import one_electron_integral_transform
 from .util import UpsStructure
 from .wavefunction_base import WaveFunctionBase
 
@@ -154,6 +152,7 @@ class WaveFunctionSAUPS(WaveFunctionBase):
         """Moves the expansion point (sa_ups_wavefunction.py:296-311)."""
         self._h_mo = None
         self._g_mo = None
+        self.__dict__["_mo_dev_cache"] = {}
         self._dev_ham = None  # built from the integrals just dropped
         self._sa_energy = None
         self._state_energies = None
@@ -264,7 +263,9 @@ class WaveFunctionSAUPS(WaveFunctionBase):
 
         if self._state_ci_coeffs is None:
             self._do_state_ci()
-        o = one_electron_integral_transform(self.c_mo, ao_integral)
+        from . import orbitals
+
+        o = orbitals.ao2mo(self.c_mo, ao_integral, None)[0].cpu().numpy()
         ni, na = self.num_inactive_orbs, self.num_active_orbs
         o_act = np.array(o[ni:ni + na, ni:ni + na])
         o_act[np.abs(o_act) <= 1e-14] = 0.0
@@ -316,8 +317,7 @@ class WaveFunctionSAUPS(WaveFunctionBase):
         if theta_optimization:
             self.thetas = list(parameters[num_kappa:])
         if kappa_optimization:
-            gradient[:num_kappa] = get_orbital_gradient(self.h_mo, self.g_mo, self.kappa_idx, self.num_inactive_orbs,
-                                                        self.num_active_orbs, self.rdm1, self.rdm2)
+            gradient[:num_kappa] = self._orbital_gradient(self.rdm1, self.rdm2)
         if theta_optimization:
             H = self._hamiltonian()
             th = np.asarray(self._thetas, dtype=np.float64)

@@ -15,7 +15,7 @@ from collections.abc import Sequence
 
 import numpy as np
 
-from .density_matrix import get_electronic_energy, get_orbital_gradient
+from .density_matrix import get_electronic_energy
 from .util import UccStructure
 from .wavefunction_base import WaveFunctionBase
 
@@ -172,8 +172,7 @@ class WaveFunctionUCC(WaveFunctionBase):
         if theta_optimization:
             self.thetas = list(parameters[num_kappa:])
         if kappa_optimization:
-            gradient[:num_kappa] = get_orbital_gradient(self.h_mo, self.g_mo, self.kappa_idx, self.num_inactive_orbs,
-                                                        self.num_active_orbs, self.rdm1, self.rdm2)
+            gradient[:num_kappa] = self._orbital_gradient(self.rdm1, self.rdm2)
         if theta_optimization:
             if self.gradient_method == "finite_difference":
                 gradient[num_kappa:] = self._finite_difference_gradient()

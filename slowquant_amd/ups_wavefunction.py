@@ -16,7 +16,7 @@ from typing import Any
 
 import numpy as np
 
-from .density_matrix import get_electronic_energy, get_orbital_gradient
+from .density_matrix import get_electronic_energy
 from .fermionic_operator import FermionicOperator
 from .operators import hamiltonian_0i_0a
 from .util import UpsStructure
@@ -228,8 +228,7 @@ class WaveFunctionUPS(WaveFunctionBase):
         if theta_optimization:
             self.thetas = list(parameters[num_kappa:])
         if kappa_optimization:
-            gradient[:num_kappa] = get_orbital_gradient(self.h_mo, self.g_mo, self.kappa_idx, self.num_inactive_orbs,
-                                                        self.num_active_orbs, self.rdm1, self.rdm2)
+            gradient[:num_kappa] = self._orbital_gradient(self.rdm1, self.rdm2)
         if theta_optimization:
             _, g = self._prog().energy_grad(self._hamiltonian(), np.asarray(self._thetas, dtype=np.float64),
                                             self._csf_dev(), self.gradient_method)

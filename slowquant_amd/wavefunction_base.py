@@ -13,14 +13,13 @@ from __future__ import annotations
 from collections.abc import Sequence
 
 import numpy as np
-import scipy.linalg
 
 import time
 from functools import partial
 
 from .ci_spaces import get_indexing
 from .optimizers import Optimizers
-from .integrals import IntegralManager, one_electron_integral_transform, two_electron_integral_transform
+from .integrals import IntegralManager
 
 
 class WaveFunctionBase:
@@ -124,6 +123,7 @@ class WaveFunctionBase:
     def kappa(self, k: list[float]) -> None:
         self._h_mo = None
         self._g_mo = None
+        self.__dict__["_mo_dev_cache"] = {}
         self._dev_ham = None  # built from the integrals just dropped
         self._energy_elec = None
         self._kappa = list(k).copy()
@@ -132,26 +132,57 @@ class WaveFunctionBase:
 
     @property
     def c_mo(self) -> np.ndarray:
-        kappa_mat = np.zeros_like(self._c_mo)
+        """C_old expm(-kappa) for the kappa moved since the last expansion point
+        (ups_wavefunction.py:367-385), computed on the device."""
         if len(self.kappa) != 0:
             diff = np.array(self.kappa) - np.array(self._kappa_old)
             if np.max(np.abs(diff)) > 0.0:
-                for d, (p, q) in zip(diff, self.kappa_idx):
-                    kappa_mat[p, q] = d
-                    kappa_mat[q, p] = -d
-        return np.matmul(self._c_mo, scipy.linalg.expm(-kappa_mat))
+                from . import orbitals
+
+                kappa_mat = orbitals.kappa_matrix(diff, self.kappa_idx, self._c_mo.shape[1])
+                return orbitals.rotate(self._c_mo, kappa_mat).cpu().numpy()
+        return np.array(self._c_mo, dtype=np.float64, copy=True)
+
+    def _ao_dev(self, which: str):
+        """AO integrals on the device, uploaded once."""
+        import torch
+
+        cache = self.__dict__.setdefault("_ao_dev_cache", {})
+        if which not in cache:
+            src = self.int_gen.h_ao if which == "h" else self.int_gen.electron_electron_repulsion
+            cache[which] = torch.as_tensor(np.ascontiguousarray(src, dtype=np.float64)).to("cuda")
+        return cache[which]
+
+    def _mo_dev(self, which: str):
+        """MO integrals (C^T h C, or the 4-index g transform) on the device, cached until kappa moves."""
+        from . import orbitals
+
+        cache = self.__dict__.setdefault("_mo_dev_cache", {})
+        if cache.get(which) is None:
+            if which == "h":
+                cache["h"] = orbitals.ao2mo(self.c_mo, self._ao_dev("h"), None)[0]
+            else:
+                cache["g"] = orbitals.ao2mo(self.c_mo, None, self._ao_dev("g"))[1]
+        return cache[which]
 
     @property
     def h_mo(self) -> np.ndarray:
         if self._h_mo is None:
-            self._h_mo = one_electron_integral_transform(self.c_mo, self.int_gen.h_ao)
+            self._h_mo = self._mo_dev("h").cpu().numpy()
         return self._h_mo
 
     @property
     def g_mo(self) -> np.ndarray:
         if self._g_mo is None:
-            self._g_mo = two_electron_integral_transform(self.c_mo, self.int_gen.electron_electron_repulsion)
+            self._g_mo = self._mo_dev("g").cpu().numpy()
         return self._g_mo
+
+    def _orbital_gradient(self, rdm1: np.ndarray, rdm2: np.ndarray) -> np.ndarray:
+        """get_orbital_gradient (density_matrix.py:181-230) on the device integrals."""
+        from . import orbitals
+
+        return orbitals.orbital_gradient(self._mo_dev("h"), self._mo_dev("g"), self.kappa_idx, self.num_inactive_orbs,
+                                         self.num_active_orbs, rdm1, rdm2)
 
     # ---- device pieces -----------------------------------------------------
     def _space(self):
