@@ -125,6 +125,17 @@ int64_t sq_work_bytes(const sq_space* s, int what);
 int sq_sigma(const sq_space* s, const sq_hamiltonian* H, const double* psi, double* sigma, double* energy,
              double* work, void* stream);
 
+/* sigma = H psi with no allocation: no counterpart in the reference; for
+ * callers that cannot allocate (code under HIP graph capture, tight serving
+ * loops).  Same kernels as sq_sigma's row path, with its scratch (alpha GEMM
+ * partials and block partials) in a 64 MiB device array that each GPU the
+ * library runs on holds until the process ends; spaces needing more, or
+ * above 4096 beta strings, fail with SQ_ERR_ARG.  energy: DEVICE double or
+ * NULL; nothing is synchronised.  Calls that overlap on one GPU share the
+ * array and race. */
+int sq_sigma_noalloc(const sq_space* s, const sq_hamiltonian* H, const double* psi, double* sigma, double* energy,
+                     void* stream);
+
 /* Energy and full theta gradient of a UPS state at theta (device array):
  * ups_wavefunction._calc_energy_optimization + _calc_gradient_optimization
  * (ups_wavefunction.py:1019-1142) for theta parameters.  grad: host array of

@@ -21,6 +21,7 @@ SOURCES = [
     "sq_ucc.hip",
     "sq_batch.hip",
     "sq_orbital.hip",
+    "sq_noalloc.hip",
 ]
 ARCH = os.environ.get("SQ_OFFLOAD_ARCH", "gfx950")
 

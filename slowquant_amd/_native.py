@@ -80,6 +80,7 @@ _SIGNATURES = {
     "sq_hamiltonian_destroy": ([vp], i32),
     "sq_work_bytes": ([vp, i32], i64),
     "sq_sigma": ([vp, vp, dp, dp, vp, dp, vp], i32),
+    "sq_sigma_noalloc": ([vp, vp, dp, dp, dp, vp], i32),
     "sq_energy_grad_adjoint": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
     "sq_energy_grad_plain": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
     "sq_energy_grad_reference_sweep": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),
