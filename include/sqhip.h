@@ -90,6 +90,11 @@ int sq_state_to_reference(const sq_space* s, const double* internal, double* ref
 int sq_program_create(const sq_space* s, const sq_gate_desc* gates, int n_gates, sq_program** out);
 int sq_program_destroy(sq_program* p);
 int sq_program_ngates(const sq_program* p);
+/* Elements per tile of the fused rotation stages (0: automatic, 256..4096 by
+ * the space's dimension); no counterpart in the reference (its gates have no
+ * tiles).  Synchronises the device and drops the program's tile plan, which
+ * the next call rebuilds. */
+int sq_program_set_tile_target(sq_program* p, int target);
 
 /* construct_ups_state (operator_state_algebra.py:963-1412): psi_out = U(theta) psi_in,
  * or U(theta)^dagger psi_in when dagger != 0.  theta is a DEVICE array of
@@ -124,6 +129,16 @@ int64_t sq_work_bytes(const sq_space* s, int what);
  * (E_pq psi gathers) -> G = g.D (FP64 MFMA) -> sigma gather. */
 int sq_sigma(const sq_space* s, const sq_hamiltonian* H, const double* psi, double* sigma, double* energy,
              double* work, void* stream);
+
+/* Kernel shape of sq_sigma's row path on space s; no counterpart in the
+ * reference (its sigma has no kernel shapes).  rows_per_block: alpha rows per
+ * block (0: automatic); nsplit: partial products of the alpha-string GEMM
+ * (1..4, 0: automatic); deterministic: 1 selects the staged form whose sums
+ * run in a fixed order (bit-identical results run to run), 0 the scatter form
+ * (LDS atomics: equal to rounding, faster), -1 the SQ_SIGMA_DETERMINISTIC
+ * environment default (scatter).  Takes effect at the next call; workspace
+ * sizes do not change. */
+int sq_sigma_configure(sq_space* s, int rows_per_block, int nsplit, int deterministic);
 
 /* sigma = H psi with no allocation: no counterpart in the reference; for
  * callers that cannot allocate (code under HIP graph capture, tight serving

@@ -776,6 +776,18 @@ int sq_program_destroy(sq_program* pr) {
 
 int sq_program_ngates(const sq_program* pr) { return pr ? (int)pr->p.gates.size() : -1; }
 
+int sq_program_set_tile_target(sq_program* pr, int target) {
+  return guarded([&] {
+    SQ_CHECK(pr && target >= 0, SQ_ERR_ARG, "invalid arguments");
+    if (pr->p.tile_target == target) return;
+    SQ_HIP(hipDeviceSynchronize());  // the old plan's buffers may be in use by queued kernels
+    if (pr->p.tiles) destroy_tile_plan(pr->p.tiles);
+    pr->p.tiles = nullptr;
+    pr->p.tiles_built = false;
+    pr->p.tile_target = target;
+  });
+}
+
 int sq_operator_create(const sq_space* sp, const sq_op_string* strings, int n_strings, int drop_outside,
                        sq_operator** out) {
   return guarded([&] {

@@ -120,6 +120,8 @@ struct Space {
   uint32_t* d_nbr = nullptr;
   int32_t* d_nbr_off = nullptr;
   uint8_t* d_nbr_tile = nullptr;
+  // row sigma shape (sq_sigma_configure): 0 / -1 = automatic
+  int sig_R = 0, sig_nsplit = 0, sig_det = -1;
   std::vector<uint32_t> h_astr, h_bstr;
 };
 
@@ -136,6 +138,7 @@ struct Program {
   const Space* space;
   mutable TilePlan* tiles = nullptr;  // built on first use (sq_tiles.hip)
   mutable bool tiles_built = false;
+  int tile_target = 0;  // elements per tile (0: automatic; sq_program_set_tile_target)
   std::vector<GateDev> gates;
   std::vector<Term> terms;
   GateDev* d_gates = nullptr;
@@ -181,7 +184,6 @@ int64_t row_sigma_used_doubles(const Space& s);
 void build_alpha_ham(const Space& s, Hamiltonian& H);
 int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* work,
                   double** partial_out, hipStream_t st);
-void row_sigma_configure(int rows_per_block, int nsplit);
 
 // Kernel timing classes (sq_profile.hip).
 enum ProfClass {

@@ -249,15 +249,76 @@ __global__ __launch_bounds__(256) void k_alpha_gemm(const double* __restrict__ M
 }
 
 // ---------------------------------------------------------------------------
-// The row kernel (see the top of this file).  LDS: A slice [K8][W] | staged
-// G tiles [waves][W][16] | psi rows [R][nsb] | alpha operand rows [R][K8 + 8].
+// One 16-column tile of G for the alpha row whose psi row is ps (LDS) and
+// whose alpha operand entries are ra (LDS): acc[t] holds the 16 x 16 block
+// of rows m0 + 16 t (the A slice sA [K8][W] is in LDS); column ib = c*16 + li
+// (columns past nsb compute garbage that is never used).  The k-steps are
+// unrolled by two with ping-pong registers: step s issues the alpha
+// amplitude and beta table loads of later steps before its MFMAs.
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+template <int MT>
+__device__ __forceinline__ void row_tile(double4_t (&acc)[MT], const double* __restrict__ sA,
+                                         const double* __restrict__ ps, const int2* __restrict__ ra,
+                                         const double* __restrict__ psi, const int2* __restrict__ rtb, int nsb,
+                                         int ib, int kq, int li, int nsteps) {
+  constexpr int W = MT * 16;
+  const uint32_t ibc = (uint32_t)(ib < nsb ? ib : nsb - 1);
+#pragma unroll
+  for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
+  const uint32_t sb = 4u * (uint32_t)nsb;
+  const uint32_t tb_i = (uint32_t)kq * (uint32_t)nsb + ibc;
+  // slot 0: even steps, slot 1: odd steps
+  int2 tb0 = rtb[tb_i], tb1 = rtb[tb_i + sb];
+  int2 a = ra[kq];
+  double xa0 = psi[(uint32_t)a.x + ibc];
+  int ca0 = a.y;
+  double xb0 = ps[tb0.x];
+  int cb0 = tb0.y;
+  tb0 = rtb[tb_i + 2 * sb];
+  double xa1, xb1;
+  int ca1, cb1;
+  const double* arow = sA + (size_t)kq * W + li;
+  for (int s = 0; s < nsteps; s += 2) {
+    a = ra[(s + 1) * 4 + kq];
+    xa1 = psi[(uint32_t)a.x + ibc];
+    ca1 = a.y;
+    xb1 = ps[tb1.x];
+    cb1 = tb1.y;
+    tb1 = rtb[tb_i + (uint32_t)(s + 3) * sb];
+    __asm__ volatile("" ::: "memory");
+    {
+      const double b = (double)ca0 * xa0 + (double)cb0 * xb0;
+#pragma unroll
+      for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+    }
+    arow += 4 * W;
+    a = ra[(s + 2) * 4 + kq];
+    xa0 = psi[(uint32_t)a.x + ibc];
+    ca0 = a.y;
+    xb0 = ps[tb0.x];
+    cb0 = tb0.y;
+    tb0 = rtb[tb_i + (uint32_t)(s + 4) * sb];
+    __asm__ volatile("" ::: "memory");
+    {
+      const double b = (double)ca1 * xa1 + (double)cb1 * xb1;
+#pragma unroll
+      for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
+    }
+    arow += 4 * W;
+  }
+}
+
+// Row kernel, staged form (deterministic): LDS = A slice [K8][W] | staged G
+// tiles [waves][W][16] | psi rows [R][nsb] | alpha operand rows [R][K8 + 8].
+// Per round every wave computes one tile; after a barrier each target adds
+// its beta neighbours' staged values in neighbour-list order.
 template <int MT>
 __global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
     const double* __restrict__ At, int kpad, int K4, int K8, int mgroups, const double* __restrict__ psi,
     const double* __restrict__ sin, int nsplit, double* __restrict__ sigma, double* __restrict__ partial,
     const int2* __restrict__ rta, const int2* __restrict__ rtb, const uint32_t* __restrict__ nbr,
     const int32_t* __restrict__ nbr_off, const uint8_t* __restrict__ nbr_tile, int nsa, int nsb, int R) {
-  typedef double double4_t __attribute__((ext_vector_type(4)));
   constexpr int W = MT * 16;
   extern __shared__ double smem[];
   __shared__ double red[kRowWaves];
@@ -294,54 +355,9 @@ __global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
       const int f = f0 + wave;
       if (f < ntiles) {
         const int r = f / tpr, c = f - r * tpr;
-        const int ib = c * 16 + li;
-        const uint32_t ibc = (uint32_t)(ib < nsb ? ib : nsb - 1);
-        const double* ps = sPsi + (size_t)r * nsb;
-        const int2* ra = sRA + (size_t)r * RA;
         double4_t acc[MT];
-#pragma unroll
-        for (int t = 0; t < MT; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-        const uint32_t sb = 4u * (uint32_t)nsb;
-        const uint32_t tb_i = (uint32_t)kq * (uint32_t)nsb + ibc;
-        // slot 0: even steps, slot 1: odd steps
-        int2 tb0 = rtb[tb_i], tb1 = rtb[tb_i + sb];
-        int2 a = ra[kq];
-        double xa0 = psi[(uint32_t)a.x + ibc];
-        int ca0 = a.y;
-        double xb0 = ps[tb0.x];
-        int cb0 = tb0.y;
-        tb0 = rtb[tb_i + 2 * sb];
-        double xa1, xb1;
-        int ca1, cb1;
-        const double* arow = sA + (size_t)kq * W + li;
-        for (int s = 0; s < nsteps; s += 2) {
-          a = ra[(s + 1) * 4 + kq];
-          xa1 = psi[(uint32_t)a.x + ibc];
-          ca1 = a.y;
-          xb1 = ps[tb1.x];
-          cb1 = tb1.y;
-          tb1 = rtb[tb_i + (uint32_t)(s + 3) * sb];
-          __asm__ volatile("" ::: "memory");
-          {
-            const double b = (double)ca0 * xa0 + (double)cb0 * xb0;
-#pragma unroll
-            for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
-          }
-          arow += 4 * W;
-          a = ra[(s + 2) * 4 + kq];
-          xa0 = psi[(uint32_t)a.x + ibc];
-          ca0 = a.y;
-          xb0 = ps[tb0.x];
-          cb0 = tb0.y;
-          tb0 = rtb[tb_i + (uint32_t)(s + 4) * sb];
-          __asm__ volatile("" ::: "memory");
-          {
-            const double b = (double)ca1 * xa1 + (double)cb1 * xb1;
-#pragma unroll
-            for (int t = 0; t < MT; ++t) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(arow[t * 16], b, acc[t], 0, 0, 0);
-          }
-          arow += 4 * W;
-        }
+        row_tile<MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, c * 16 + li, kq, li,
+                     nsteps);
         double* g = sG + (size_t)wave * W * 16 + li;
 #pragma unroll
         for (int t = 0; t < MT; ++t)
@@ -395,17 +411,104 @@ __global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
   }
 }
 
+// Row kernel, scatter form: LDS = A slice [K8][W] | sigma rows [R][nsb] |
+// psi rows [R][nsb] | alpha operand rows [R][K8 + 8] (small enough for two
+// blocks per CU).  Waves take tiles independently; each output G_k[I_a, J_b]
+// goes to the sigma row at P_k(J_b) (E+b_k is symmetric, so the partner and
+// coefficient are rtb[k][J_b]) by an LDS atomic add.  The order of those
+// adds varies between runs, so results agree to rounding, not bit for bit.
+template <int MT>
+__global__ __launch_bounds__(kRowThreads) void k_sigma_rows_scatter(
+    const double* __restrict__ At, int kpad, int K4, int K8, int mgroups, int npair, const double* __restrict__ psi,
+    const double* __restrict__ sin, int nsplit, double* __restrict__ sigma, double* __restrict__ partial,
+    const int2* __restrict__ rta, const int2* __restrict__ rtb, int nsa, int nsb, int R) {
+  constexpr int W = MT * 16;
+  extern __shared__ double smem[];
+  __shared__ double red[kRowWaves];
+  const int tpr = (nsb + 15) >> 4;
+  const int RA = K8 + 8;
+  double* sA = smem;
+  double* sAcc = sA + (size_t)K8 * W;
+  double* sPsi = sAcc + (size_t)R * nsb;
+  int2* sRA = reinterpret_cast<int2*>(sPsi + (size_t)R * nsb);
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, li = lane & 15;
+  const int row0 = blockIdx.x * R;
+  const int nr = min(R, nsa - row0);
+  const int ntg = nr * nsb;
+  const size_t base = (size_t)row0 * nsb;
+  for (int i = tid; i < ntg; i += kRowThreads) {
+    sPsi[i] = psi[base + i];
+    sAcc[i] = 0.0;
+  }
+  for (int i = tid; i < nr * RA; i += kRowThreads) {
+    const int r = i / RA, k = i - r * RA;
+    sRA[i] = rta[(size_t)k * nsa + row0 + r];
+  }
+  const int ntiles = nr * tpr;
+  const int nsteps = K8 / 4;
+  for (int mg = 0; mg < mgroups; ++mg) {
+    const int m0 = mg * W;
+    __syncthreads();  // the previous m-group's reads of sA are done
+    for (int i = tid; i < K8 * W; i += kRowThreads) {
+      const int k = i / W, m = m0 + (i - k * W);
+      sA[i] = (k < K4 && m < kpad) ? At[(size_t)k * kpad + m] : 0.0;
+    }
+    __syncthreads();
+    for (int f = wave; f < ntiles; f += kRowWaves) {
+      const int r = f / tpr, c = f - r * tpr;
+      const int ib = c * 16 + li;
+      double4_t acc[MT];
+      row_tile<MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
+      if (ib < nsb) {
+        double* acc_row = sAcc + (size_t)r * nsb;
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int m = m0 + t * 16 + kq + 4 * q;
+            if (m < npair) {
+              const int2 tb = rtb[(size_t)m * nsb + ib];
+              if (tb.y != 0) atomicAdd(acc_row + tb.x, (double)tb.y * acc[t][q]);
+            }
+          }
+      }
+    }
+  }
+  __syncthreads();
+  const size_t dim = (size_t)nsa * nsb;
+  double e = 0.0;
+  for (int tau = tid; tau < ntg; tau += kRowThreads) {
+    double v = sAcc[tau];
+    for (int z = 0; z < nsplit; ++z) v += sin[(size_t)z * dim + base + tau];
+    sigma[base + tau] = v;
+    e += v * sPsi[tau];
+  }
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+  if (lane == 0) red[wave] = e;
+  __syncthreads();
+  if (tid == 0) {
+    double tsum = 0.0;
+    for (int w = 0; w < kRowWaves; ++w) tsum += red[w];
+    partial[blockIdx.x] = tsum;
+  }
+}
+
 struct RowPlan {
   int mt, mgroups, R, ngroups, nsplit, kchunk;
+  bool scatter;
   size_t lds;
 };
 
 int row_k8(const Space& s) { return rup(s.npair + 1, 8); }
 
-size_t row_lds(const Space& s, int mt, int R) {
+// Dynamic LDS of one block (bytes): staged form A slice | G tiles | psi rows |
+// alpha operand rows; scatter form A slice | sigma rows | psi rows | alpha
+// operand rows.
+size_t row_lds(const Space& s, int mt, int R, bool scatter) {
   const int K8 = row_k8(s);
-  const int W = mt * 16;
-  return 8 * ((size_t)K8 * W + (size_t)kRowWaves * W * 16 + (size_t)R * s.nsb + (size_t)R * (K8 + 8));
+  const size_t W = (size_t)mt * 16;
+  const size_t mid = scatter ? (size_t)R * s.nsb : (size_t)kRowWaves * W * 16;
+  return 8 * ((size_t)K8 * W + mid + (size_t)R * s.nsb + (size_t)R * (K8 + 8));
 }
 
 int env_int(const char* name, int dflt) {
@@ -413,51 +516,65 @@ int env_int(const char* name, int dflt) {
   return (e && e[0]) ? atoi(e) : dflt;
 }
 
-// Kernel shape: the m-tile group (largest that fits, then balanced), rows
-// per block R (fewest block-rounds on 256 CUs, one block per CU), the split
-// of the alpha GEMM.  Overrides: sq_sigma_configure().
-struct RowOverride {
-  int R = 0, nsplit = 0;
-};
-RowOverride g_row_override;
+// The space's kernel shape choices (sq_sigma_configure) override these
+// defaults: rows per block, alpha GEMM split, and the deterministic (staged)
+// form instead of the scatter form (default: SQ_SIGMA_DETERMINISTIC, else 0).
+bool row_deterministic(const Space& s) {
+  if (s.sig_det >= 0) return s.sig_det != 0;
+  static int v = -1;
+  if (v < 0) v = env_int("SQ_SIGMA_DETERMINISTIC", 0) != 0 ? 1 : 0;
+  return v == 1;
+}
 
+// Kernel shape: the m-tile group (largest that fits, then balanced) and the
+// rows per block R.  Staged form: R with the fewest block-rounds on 256 CUs
+// (one block per CU).  Scatter form: the smallest R that gives every wave a
+// tile (more, smaller blocks; two fit per CU at 12 orbitals).
 bool row_plan(const Space& s, RowPlan* out) {
   if (s.dim == 0 || s.nsb > 16384 || s.npair + 1 > 255 || s.dim >= ((int64_t)1 << 31)) return false;
+  const bool scatter = !row_deterministic(s);
   const int kpad = rup(s.npair + 1, 16);
   const int mt_total = kpad / 16;
   int mt = std::min(mt_total, 9);
-  while (mt > 1 && row_lds(s, mt, 1) > kLdsMax) --mt;
-  if (row_lds(s, mt, 1) > kLdsMax) return false;
+  while (mt > 1 && row_lds(s, mt, 1, scatter) > kLdsMax) --mt;
+  if (row_lds(s, mt, 1, scatter) > kLdsMax) return false;
   const int mgroups = (mt_total + mt - 1) / mt;
   mt = (mt_total + mgroups - 1) / mgroups;
   const int tpr = (s.nsb + 15) / 16;
   int bestR = 0;
-  int64_t best = -1;
-  for (int R = 1; R <= 64 && R <= s.nsa; ++R) {
-    if (row_lds(s, mt, R) > kLdsMax) break;
-    if ((int64_t)R * s.nsb > (int64_t)kMaxTargets * kRowThreads) break;
-    const int64_t groups = (s.nsa + R - 1) / R;
-    const int64_t cost = ((groups + 255) / 256) * (((int64_t)R * tpr + kRowWaves - 1) / kRowWaves);
-    if (best < 0 || cost < best) {
-      best = cost;
-      bestR = R;
+  if (scatter) {
+    bestR = std::max(1, std::min(s.nsa, (kRowWaves + tpr - 1) / tpr));
+    while (bestR > 1 && row_lds(s, mt, bestR, true) > kLdsMax) --bestR;
+  } else {
+    int64_t best = -1;
+    for (int R = 1; R <= 64 && R <= s.nsa; ++R) {
+      if (row_lds(s, mt, R, false) > kLdsMax) break;
+      if ((int64_t)R * s.nsb > (int64_t)kMaxTargets * kRowThreads) break;
+      const int64_t groups = (s.nsa + R - 1) / R;
+      const int64_t cost = ((groups + 255) / 256) * (((int64_t)R * tpr + kRowWaves - 1) / kRowWaves);
+      if (best < 0 || cost < best) {
+        best = cost;
+        bestR = R;
+      }
     }
   }
   if (bestR == 0) return false;
-  if (g_row_override.R > 0) {
-    const int R = g_row_override.R;
-    if (R <= s.nsa && row_lds(s, mt, R) <= kLdsMax && (int64_t)R * s.nsb <= (int64_t)kMaxTargets * kRowThreads)
+  if (s.sig_R > 0) {
+    const int R = s.sig_R;
+    if (R <= s.nsa && row_lds(s, mt, R, scatter) <= kLdsMax &&
+        (scatter || (int64_t)R * s.nsb <= (int64_t)kMaxTargets * kRowThreads))
       bestR = R;
   }
   out->mt = mt;
   out->mgroups = mgroups;
   out->R = bestR;
+  out->scatter = scatter;
   out->ngroups = (s.nsa + bestR - 1) / bestR;
-  int nsplit = g_row_override.nsplit > 0 ? g_row_override.nsplit : (s.nsa >= 2048 ? 1 : 2);
+  int nsplit = s.sig_nsplit > 0 ? s.sig_nsplit : (s.nsa >= 2048 ? 1 : 2);
   nsplit = std::max(1, std::min({nsplit, kMaxSplit, (s.nsa + 15) / 16}));
   out->kchunk = rup((s.nsa + nsplit - 1) / nsplit, 16);
   out->nsplit = (s.nsa + out->kchunk - 1) / out->kchunk;
-  out->lds = row_lds(s, mt, bestR);
+  out->lds = row_lds(s, mt, bestR, scatter);
   return true;
 }
 
@@ -468,11 +585,6 @@ bool rows_enabled() {
 }
 
 }  // namespace
-
-void row_sigma_configure(int rows_per_block, int nsplit) {
-  g_row_override.R = std::max(0, rows_per_block);
-  g_row_override.nsplit = std::max(0, nsplit);
-}
 
 bool row_sigma_applies(const Space& s) {
   RowPlan pl;
@@ -560,6 +672,23 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
   const double flops = 2.0 * (double)s.npair * (s.npair + 1) * (double)s.dim;
   // algorithmic traffic: psi read (rows and alpha partners through the cache), the alpha partials read, sigma written
   const double bytes = 8.0 * (double)s.dim * (2 + pl.nsplit);
+  if (pl.scatter) {
+    switch (pl.mt) {
+#define SQ_RSS(MT)                                                                                              \
+  case MT:                                                                                                      \
+    allow_lds(k_sigma_rows_scatter<MT>, pl.lds);                                                                \
+    SQ_PROF(PC_SIGMA_ROWS, st, bytes, flops,                                                                    \
+            k_sigma_rows_scatter<MT><<<pl.ngroups, kRowThreads, pl.lds, st>>>(                                  \
+                H.d_A, H.kpad, K4, K8, pl.mgroups, s.npair, psi, sinp, pl.nsplit, sigma, partial, s.d_rta,      \
+                s.d_rtb, s.nsa, s.nsb, pl.R));                                                                  \
+    break;
+      SQ_RSS(1) SQ_RSS(2) SQ_RSS(3) SQ_RSS(4) SQ_RSS(5) SQ_RSS(6) SQ_RSS(7) SQ_RSS(8) SQ_RSS(9)
+#undef SQ_RSS
+      default:
+        throw Error{SQ_ERR_ARG, "unsupported orbital count for the row sigma"};
+    }
+    return pl.ngroups;
+  }
   switch (pl.mt) {
 #define SQ_RS(MT)                                                                                               \
   case MT:                                                                                                      \
@@ -586,3 +715,24 @@ int64_t row_sigma_used_doubles(const Space& s) {
 }
 
 }  // namespace sq
+
+extern "C" int sq_sigma_configure(sq_space* sp, int rows_per_block, int nsplit, int deterministic) {
+  return sq::guarded([&] {
+    SQ_CHECK(sp, SQ_ERR_ARG, "NULL space");
+    SQ_CHECK(rows_per_block >= 0 && rows_per_block <= 64 && nsplit >= 0 && nsplit <= 4, SQ_ERR_ARG,
+             "rows_per_block must be in [0, 64] and nsplit in [0, 4] (0: automatic)");
+    sq::Space& s = sp->s;
+    sq::RowPlan pl;
+    const bool before = sq::row_plan(s, &pl);
+    const int old[3] = {s.sig_R, s.sig_nsplit, s.sig_det};
+    s.sig_R = rows_per_block;
+    s.sig_nsplit = nsplit;
+    s.sig_det = deterministic < 0 ? -1 : (deterministic != 0);
+    if (before && !sq::row_plan(s, &pl)) {  // the workspace sizes assume the row path
+      s.sig_R = old[0];
+      s.sig_nsplit = old[1];
+      s.sig_det = old[2];
+      throw sq::Error{SQ_ERR_ARG, "this sigma form does not fit the space"};
+    }
+  });
+}

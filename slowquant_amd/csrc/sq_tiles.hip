@@ -357,7 +357,8 @@ static T* upload(std::vector<void*>& allocs, const std::vector<T>& v) {
 
 // Elements per tile that runs of beta groups are filled up to (SQ_TILE_TARGET
 // overrides); a single group pair above it still makes one tile.
-static int tile_target(int64_t dim) {
+static int tile_target(int64_t dim, int program_target) {
+  if (program_target > 0) return std::min(std::max(program_target, 16), kTileMax);
   const char* env = getenv("SQ_TILE_TARGET");
   if (env) return std::min(std::max(atoi(env), 16), kTileMax);
   // about 200 tiles or more (one or two per CU), 256..4096 elements each
@@ -381,7 +382,7 @@ static uint32_t pair_desc(uint32_t occ, uint32_t bits, int npairs) {
   return d;
 }
 
-static void finalize_stage(const Space& sp, Stage& S) {
+static void finalize_stage(const Space& sp, Stage& S, int program_target) {
   // entries grouped by pair, program order kept within a pair
   std::stable_sort(S.entries.begin(), S.entries.end(),
                    [](const StageEntry& a, const StageEntry& b) { return a.pair < b.pair; });
@@ -394,7 +395,7 @@ static void finalize_stage(const Space& sp, Stage& S) {
   GroupInfo B = group_strings(sp.h_bstr, S.npairs, S.pp);
   // tiles: each alpha group x runs of whole beta groups, up to the target
   // size; a beta group that alone exceeds the target gets its own tile
-  const int target = tile_target(sp.dim);
+  const int target = tile_target(sp.dim, program_target);
   std::vector<int4> tiles;
   const int nag = (int)A.goff.size() - 1;
   const int nbg = (int)B.goff.size() - 1;
@@ -580,7 +581,7 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
     std::vector<StageInfo> info;
     for (auto& S : T->stages) {
       if (!S.tiled) continue;
-      finalize_stage(sp, S);
+      finalize_stage(sp, S, P.tile_target);
       S.index = (int)info.size();
       StageInfo I{};
       I.npairs = S.npairs;

@@ -71,6 +71,11 @@ class Space:
             self._work[what] = torch.empty(max(nbytes // 8, 1), dtype=torch.float64, device="cuda")
         return self._work[what]
 
+    def configure_sigma(self, rows_per_block: int = 0, nsplit: int = 0, deterministic: int = -1) -> None:
+        """Row-sigma kernel shape on this space (0: automatic); deterministic=1
+        selects fixed-order sums, 0 LDS-atomic sums, -1 the default."""
+        N.check(N.lib().sq_sigma_configure(self._h, int(rows_per_block), int(nsplit), int(deterministic)))
+
     def strings(self) -> tuple[np.ndarray, np.ndarray]:
         a = np.zeros(self.n_alpha_strings, dtype=np.uint32)
         b = np.zeros(self.n_beta_strings, dtype=np.uint32)
@@ -123,6 +128,10 @@ class Program:
                 N.lib().sq_program_destroy(h)
         except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
             pass
+
+    def set_tile_target(self, target: int) -> None:
+        """Elements per tile of the fused rotation stages (0: automatic)."""
+        N.check(N.lib().sq_program_set_tile_target(self._h, int(target)))
 
     def theta_tensor(self, theta) -> torch.Tensor:
         if isinstance(theta, torch.Tensor):

@@ -368,6 +368,8 @@ def test_tile_dispatch_order_does_not_change_results(n, layers, monkeypatch):
     are summed in tile order whatever order the tiles ran in."""
     from slowquant_amd.synthetic import synthetic_problem
 
+    from slowquant_amd import _native as N
+
     E = _engine()
     lay = _tups(n, layers)
     h, g, theta = synthetic_problem(n, n, lay.n_params, 0.2, 9)
@@ -375,12 +377,16 @@ def test_tile_dispatch_order_does_not_change_results(n, layers, monkeypatch):
     H = E.Hamiltonian(sp, h, g)
     psi0 = E.hf_state(sp)
     out = []
-    for flag in ("0", "1"):
-        monkeypatch.setenv("SQ_TILE_XCD", flag)
-        prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
-        psi = prog.state_prep(theta, psi0).cpu().numpy()  # the tile plan is built here
-        e, grad = prog.energy_grad(H, theta, psi0, "adjoint")
-        out.append((psi, e, grad))
+    N.check(N.lib().sq_sigma_configure(sp.handle, 0, 0, 1))  # sigma with fixed-order sums
+    try:
+        for flag in ("0", "1"):
+            monkeypatch.setenv("SQ_TILE_XCD", flag)
+            prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+            psi = prog.state_prep(theta, psi0).cpu().numpy()  # the tile plan is built here
+            e, grad = prog.energy_grad(H, theta, psi0, "adjoint")
+            out.append((psi, e, grad))
+    finally:
+        N.check(N.lib().sq_sigma_configure(sp.handle, 0, 0, -1))
     assert np.array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
     assert np.array_equal(out[0][2], out[1][2])
