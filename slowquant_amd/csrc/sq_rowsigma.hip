@@ -51,9 +51,9 @@ __host__ __device__ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
 // ---------------------------------------------------------------------------
 // Space tables.
-// rta[k][s] (k < K8 + 8): {partner row * nsb, 2 * coefficient} of E+a_k on
+// rta[k][s] (k < K8 + 16): {partner row * nsb, 2 * coefficient} of E+a_k on
 // alpha string s (diagonal: 2 * occupation; absent / k >= npair: {own, 0}).
-// rtb[k][s] (k < K8 + 16): {partner column, coefficient} of E+b_k on beta
+// rtb[k][s] (k < K8 + 32): {partner column, coefficient} of E+b_k on beta
 // string s (diagonal: occupation; k == npair: {own, 1}; absent: {own, 0}).
 __global__ void k_row_tables(int npair, int nstr, int nsb, int alpha, const int32_t* __restrict__ exc,
                              int2* __restrict__ tab) {
@@ -310,7 +310,7 @@ __device__ __forceinline__ void row_tile(double4_t (&acc)[MT], const double* __r
 }
 
 // Row kernel, staged form (deterministic): LDS = A slice [K8][W] | staged G
-// tiles [waves][W][16] | psi rows [R][nsb] | alpha operand rows [R][K8 + 8].
+// tiles [waves][W][16] | psi rows [R][nsb] | alpha operand rows [R][K8 + 16].
 // Per round every wave computes one tile; after a barrier each target adds
 // its beta neighbours' staged values in neighbour-list order.
 template <int MT>
@@ -323,7 +323,7 @@ __global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
   extern __shared__ double smem[];
   __shared__ double red[kRowWaves];
   const int tpr = (nsb + 15) >> 4;
-  const int RA = K8 + 8;
+  const int RA = K8 + 16;
   double* sA = smem;
   double* sG = sA + (size_t)K8 * W;
   double* sPsi = sG + (size_t)kRowWaves * W * 16;
@@ -412,27 +412,34 @@ __global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
 }
 
 // Row kernel, scatter form: LDS = A slice [K8][W] | sigma rows [R][nsb] |
-// psi rows [R][nsb] | alpha operand rows [R][K8 + 8] (small enough for two
+// psi rows [R][nsb] | alpha operand rows [R][K8 + 16] (small enough for two
 // blocks per CU).  Waves take tiles independently; each output G_k[I_a, J_b]
 // goes to the sigma row at P_k(J_b) (E+b_k is symmetric, so the partner and
 // coefficient are rtb[k][J_b]) by an LDS atomic add.  The order of those
 // adds varies between runs, so results agree to rounding, not bit for bit.
 template <int MT>
-__global__ __launch_bounds__(kRowThreads) void k_sigma_rows_scatter(
+__global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT <= 5 ? 4 : 2, 8))) void k_sigma_rows_scatter(
     const double* __restrict__ At, int kpad, int K4, int K8, int mgroups, int npair, const double* __restrict__ psi,
     const double* __restrict__ sin, int nsplit, double* __restrict__ sigma, double* __restrict__ partial,
-    const int2* __restrict__ rta, const int2* __restrict__ rtb, int nsa, int nsb, int R) {
+    const int2* __restrict__ rta, const int2* __restrict__ rtb, int nsa, int nsb, int R, int xcd) {
   constexpr int W = MT * 16;
   extern __shared__ double smem[];
   __shared__ double red[kRowWaves];
   const int tpr = (nsb + 15) >> 4;
-  const int RA = K8 + 8;
+  const int RA = K8 + 16;
   double* sA = smem;
   double* sAcc = sA + (size_t)K8 * W;
   double* sPsi = sAcc + (size_t)R * nsb;
   int2* sRA = reinterpret_cast<int2*>(sPsi + (size_t)R * nsb);
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63, kq = lane >> 4, li = lane & 15;
-  const int row0 = blockIdx.x * R;
+  // with xcd, XCD x (workgroups go to the eight XCDs round robin) takes a
+  // contiguous range of row groups, whose alpha partner rows overlap in its L2
+  int blk = blockIdx.x;
+  if (xcd) {
+    const int q = (int)gridDim.x >> 3, rr = (int)gridDim.x & 7, x = blk & 7;
+    blk = x * q + min(x, rr) + (blk >> 3);
+  }
+  const int row0 = blk * R;
   const int nr = min(R, nsa - row0);
   const int ntg = nr * nsb;
   const size_t base = (size_t)row0 * nsb;
@@ -459,19 +466,28 @@ __global__ __launch_bounds__(kRowThreads) void k_sigma_rows_scatter(
       const int ib = c * 16 + li;
       double4_t acc[MT];
       row_tile<MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
-      if (ib < nsb) {
-        double* acc_row = sAcc + (size_t)r * nsb;
+      // partners and coefficients of all 4 MT outputs first (rows m >= npair
+      // and columns past nsb read row 0 / the last column and add zero), then
+      // the LDS adds, with no branch between the loads
+      const uint32_t ibc = (uint32_t)(ib < nsb ? ib : nsb - 1);
+      int2 tb[MT][4];
 #pragma unroll
-        for (int t = 0; t < MT; ++t)
+      for (int t = 0; t < MT; ++t)
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            const int m = m0 + t * 16 + kq + 4 * q;
-            if (m < npair) {
-              const int2 tb = rtb[(size_t)m * nsb + ib];
-              if (tb.y != 0) atomicAdd(acc_row + tb.x, (double)tb.y * acc[t][q]);
-            }
-          }
-      }
+        for (int q = 0; q < 4; ++q) {
+          const int m = m0 + t * 16 + kq + 4 * q;
+          tb[t][q] = rtb[(uint32_t)(m < npair ? m : 0) * (uint32_t)nsb + ibc];
+        }
+      double* acc_row = sAcc + (size_t)r * nsb;
+      const bool col_ok = ib < nsb;
+#pragma unroll
+      for (int t = 0; t < MT; ++t)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int m = m0 + t * 16 + kq + 4 * q;
+          const double c = (col_ok && m < npair) ? (double)tb[t][q].y : 0.0;
+          atomicAdd(acc_row + tb[t][q].x, c * acc[t][q]);
+        }
     }
   }
   __syncthreads();
@@ -499,7 +515,7 @@ struct RowPlan {
   size_t lds;
 };
 
-int row_k8(const Space& s) { return rup(s.npair + 1, 8); }
+int row_k8(const Space& s) { return rup(s.npair + 1, 16); }  // contraction rows, 4 k-steps per iteration
 
 // Dynamic LDS of one block (bytes): staged form A slice | G tiles | psi rows |
 // alpha operand rows; scatter form A slice | sigma rows | psi rows | alpha
@@ -508,7 +524,7 @@ size_t row_lds(const Space& s, int mt, int R, bool scatter) {
   const int K8 = row_k8(s);
   const size_t W = (size_t)mt * 16;
   const size_t mid = scatter ? (size_t)R * s.nsb : (size_t)kRowWaves * W * 16;
-  return 8 * ((size_t)K8 * W + mid + (size_t)R * s.nsb + (size_t)R * (K8 + 8));
+  return 8 * ((size_t)K8 * W + mid + (size_t)R * s.nsb + (size_t)R * (K8 + 16));
 }
 
 int env_int(const char* name, int dflt) {
@@ -603,11 +619,11 @@ void build_row_tables(Space& s) {
   if (!row_plan(s, &pl)) return;
   const int K8 = row_k8(s);
   const int tpr = (s.nsb + 15) / 16;
-  SQ_HIP(hipMalloc(&s.d_rta, sizeof(int2) * (size_t)(K8 + 8) * s.nsa));
-  SQ_HIP(hipMalloc(&s.d_rtb, sizeof(int2) * (size_t)(K8 + 16) * s.nsb));
-  k_row_tables<<<dim3((s.nsa + 255) / 256, K8 + 8), 256>>>(s.npair, s.nsa, s.nsb, 1, s.d_aexc, s.d_rta);
+  SQ_HIP(hipMalloc(&s.d_rta, sizeof(int2) * (size_t)(K8 + 16) * s.nsa));
+  SQ_HIP(hipMalloc(&s.d_rtb, sizeof(int2) * (size_t)(K8 + 32) * s.nsb));
+  k_row_tables<<<dim3((s.nsa + 255) / 256, K8 + 16), 256>>>(s.npair, s.nsa, s.nsb, 1, s.d_aexc, s.d_rta);
   SQ_HIP(hipGetLastError());
-  k_row_tables<<<dim3((s.nsb + 255) / 256, K8 + 16), 256>>>(s.npair, s.nsb, s.nsb, 0, s.d_bexc, s.d_rtb);
+  k_row_tables<<<dim3((s.nsb + 255) / 256, K8 + 32), 256>>>(s.npair, s.nsb, s.nsb, 0, s.d_bexc, s.d_rtb);
   SQ_HIP(hipGetLastError());
   int32_t* cnt = nullptr;
   SQ_HIP(hipMalloc(&cnt, sizeof(int32_t) * s.nsb));
@@ -673,6 +689,7 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
   // algorithmic traffic: psi read (rows and alpha partners through the cache), the alpha partials read, sigma written
   const double bytes = 8.0 * (double)s.dim * (2 + pl.nsplit);
   if (pl.scatter) {
+    static const int xcd = env_int("SQ_ROWS_XCD", 1) != 0 ? 1 : 0;
     switch (pl.mt) {
 #define SQ_RSS(MT)                                                                                              \
   case MT:                                                                                                      \
@@ -680,7 +697,7 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
     SQ_PROF(PC_SIGMA_ROWS, st, bytes, flops,                                                                    \
             k_sigma_rows_scatter<MT><<<pl.ngroups, kRowThreads, pl.lds, st>>>(                                  \
                 H.d_A, H.kpad, K4, K8, pl.mgroups, s.npair, psi, sinp, pl.nsplit, sigma, partial, s.d_rta,      \
-                s.d_rtb, s.nsa, s.nsb, pl.R));                                                                  \
+                s.d_rtb, s.nsa, s.nsb, pl.R, xcd));                                                             \
     break;
       SQ_RSS(1) SQ_RSS(2) SQ_RSS(3) SQ_RSS(4) SQ_RSS(5) SQ_RSS(6) SQ_RSS(7) SQ_RSS(8) SQ_RSS(9)
 #undef SQ_RSS
