@@ -265,7 +265,9 @@ def bench_batch(args) -> dict:
     specs = draw_specs(args.instances)
     inst, thetas = build_instances(specs)
     th_all = np.concatenate(thetas)
-    b = E.autotune_batch(inst, th_all)
+    from slowquant_amd import autotune
+
+    b = autotune.tune_batch(inst, th_all)
     e_b, g_b = b.run(thetas, graph=True)
     e_l, g_l = loop_energy_grad(inst, thetas)
     diff = max(float(np.abs(e_b - e_l).max()), max(float(np.abs(x - y).max()) for x, y in zip(g_b, g_l)))
@@ -320,6 +322,9 @@ def bench_tups14(args) -> dict:
     H = E.Hamiltonian(sp, h, g)
     psi0 = E.hf_state(sp, 0)
     th = np.asarray(theta)
+    from slowquant_amd import autotune
+
+    tuned = autotune.tune(sp, prog, H, th, psi0, reps=2)
 
     def grad_part():
         return prog.energy_grad(H, th, psi0, "adjoint")
@@ -376,9 +381,11 @@ def bench_tups14(args) -> dict:
         "energy": e,
         "checks": {"energy_minus_energy_from_rdms": float(e - e_rdm), "trace_rdm1": float(np.trace(r1)),
                    "max_grad_minus_central_difference": float(max(fd))},
-        "roofline": {name: roof(k, name) for k, name in ((kg, "sigma_gemm_mfma"), (kg, "sigma_gather"),
+        "roofline": {name: roof(k, name) for k, name in ((kg, "sigma_rows_mfma"), (kg, "alpha_gemm_mfma"),
+                                                           (kg, "sigma_gemm_mfma"), (kg, "sigma_gather"),
                                                            (kg, "tile_forward"), (kg, "tile_reverse"),
-                                                           (kr, "rdm_gram_mfma"))},
+                                                           (kr, "rdm_gram_mfma")) if name in k},
+        "autotune": tuned,
         "kernels_energy_gradient": kg,
         "kernels_state_and_rdms": kr,
     }
@@ -484,6 +491,10 @@ def main() -> int:
 
     lay, h, g, theta, sp, prog, H, psi0 = build_workload()
     th = np.asarray(theta)
+    # kernel shapes (tile size, row-sigma blocking) picked by timing, before the timed region
+    from slowquant_amd import autotune
+
+    tuned = autotune.tune(sp, prog, H, th, psi0)
 
     def step():
         return prog.energy_grad(H, th, psi0, "adjoint")
@@ -537,6 +548,7 @@ def main() -> int:
         "max_grad_diff_adjoint_vs_plain": grad_diff,
         "energy": e_adj,
         "roofline": roof,
+        "autotune": tuned,
         "kernels": kernels,
         "rotation": rotation_profile(sp, lay, prog, th, psi0),
     }
