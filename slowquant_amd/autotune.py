@@ -64,7 +64,9 @@ def _store(key: str, value: dict) -> None:
 
 
 def _device_key() -> str:
-    return f"{torch.cuda.get_device_name(0)}|{N.lib().sq_version().decode()}"
+    p = torch.cuda.get_device_properties(0)
+    arch = getattr(p, "gcnArchName", "") or "gpu"
+    return f"{arch}:{p.multi_processor_count}cu|{N.lib().sq_version().decode()}"
 
 
 def program_key(space: E.Space, prog: E.Program) -> str:
