@@ -254,6 +254,11 @@ int sq_operator_create(const sq_space* s, const sq_op_string* strings, int n_str
 int sq_operator_destroy(sq_operator* op);
 /* out = O in (device vectors, internal convention, in != out). */
 int sq_operator_apply(const sq_space* s, const sq_operator* op, const double* in, double* out, void* stream);
+/* build_operator_matrix (operator_state_algebra.py:413-469): the dense
+ * dim x dim matrix M[I][J] = <I|O|J> (device, row-major) in one launch, in the
+ * reference's sign convention when reference_convention != 0. */
+int sq_operator_matrix(const sq_space* s, const sq_operator* op, int reference_convention, double* M,
+                       void* stream);
 
 /* ---- Orbital rotations and integrals ---------------------------------------
  * Matrices are row-major FP64 device arrays; every contraction is an FP64-MFMA

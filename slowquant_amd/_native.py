@@ -92,6 +92,7 @@ _SIGNATURES = {
     "sq_operator_create": ([vp, ctypes.POINTER(OpString), i32, i32, ctypes.POINTER(vp)], i32),
     "sq_operator_destroy": ([vp], i32),
     "sq_operator_apply": ([vp, vp, dp, dp, vp], i32),
+    "sq_operator_matrix": ([vp, vp, i32, dp, vp], i32),
     "sq_ucc_work_bytes": ([vp, vp], i64),
     "sq_ucc_expm": ([vp, vp, dp, ctypes.c_double, dp, dp, dp, vp], i32),
     "sq_ucc_energy_grad": ([vp, vp, vp, dp, dp, vp, vp, dp, vp], i32),

@@ -530,6 +530,48 @@ __global__ void k_tapply(SpaceView v, const Term* __restrict__ terms, int t0, in
   }
 }
 
+// Dense matrix of an operator: row I of M gets coef * sign at column J for
+// every term whose adjoint maps I to J, i.e. M[I][J] = <I|O|J> (the rows are
+// the gather of k_tapply).  With ref != 0 both indices are in the reference's
+// interleaved sign convention: M_ref[I][J] = s(I) s(J) M[I][J].  M must be
+// zero on entry.
+__device__ __forceinline__ int interleave_parity(uint32_t a, uint32_t b) {
+  int par = 0;
+  while (a) {
+    const int p = __ffs(a) - 1;
+    a &= a - 1;
+    par += __popc(b & ((1u << p) - 1u));
+  }
+  return par;
+}
+
+__global__ void k_operator_matrix(SpaceView v, const Term* __restrict__ terms, int t0, int t1, int ref,
+                                  double* __restrict__ M) {
+  const int ia = blockIdx.y;
+  const int ib = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib >= v.nsb) return;
+  const uint32_t a0 = v.astr[ia], b0 = v.bstr[ib];
+  const size_t dim = (size_t)v.nsa * v.nsb;
+  double* row = M + ((size_t)ia * v.nsb + ib) * dim;
+  const int pI = ref ? interleave_parity(a0, b0) : 0;
+  for (int t = t0; t < t1; ++t) {
+    const Term& T = terms[t];
+    uint32_t a = a0, b = b0;
+    int par = pI;
+    if (!apply_ops_adjoint(T.a, a, par)) continue;
+    if (!apply_ops_adjoint(T.b, b, par)) continue;
+    const int ra = v.arank[a], rb = v.brank[b];
+    if (ra < 0 || rb < 0) continue;
+    if (ref) par += interleave_parity(a, b);
+    row[(size_t)ra * v.nsb + rb] += (par & 1) ? -T.coef : T.coef;
+  }
+}
+
+__global__ void k_fill_zero(double* __restrict__ x, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
 // Partial dots sum_I lam[I] * (T psi)[I] per block.
 __global__ void k_tapply_dot(SpaceView v, const Term* __restrict__ terms, int t0, int t1,
                              const double* __restrict__ psi, const double* __restrict__ lam,
@@ -838,6 +880,23 @@ int sq_operator_apply(const sq_space* sp, const sq_operator* op, const double* i
     if (s.dim == 0) return;
     k_tapply<<<grid2d(s), kBlk, 0, as_stream(stream)>>>(view(s), op->d_terms, 0, (int)op->terms.size(), in, out,
                                                         nullptr, nullptr, 0, 0, 0, 0, 1.0);
+    SQ_HIP(hipGetLastError());
+  });
+}
+
+int sq_operator_matrix(const sq_space* sp, const sq_operator* op, int reference_convention, double* M,
+                       void* stream) {
+  return guarded([&] {
+    SQ_CHECK(sp && op && M, SQ_ERR_ARG, "NULL argument");
+    SQ_CHECK(op->space == &sp->s, SQ_ERR_ARG, "operator built for another space");
+    const Space& s = sp->s;
+    if (s.dim == 0) return;
+    hipStream_t st = as_stream(stream);
+    const size_t n = (size_t)s.dim * (size_t)s.dim;
+    k_fill_zero<<<(unsigned)std::min<size_t>((n + kBlk - 1) / kBlk, 65536), kBlk, 0, st>>>(M, n);
+    SQ_HIP(hipGetLastError());
+    k_operator_matrix<<<grid2d(s), kBlk, 0, st>>>(view(s), op->d_terms, 0, (int)op->terms.size(),
+                                                  reference_convention ? 1 : 0, M);
     SQ_HIP(hipGetLastError());
   });
 }

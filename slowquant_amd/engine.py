@@ -287,6 +287,13 @@ class Operator:
         N.check(N.lib().sq_operator_apply(self.space.handle, self._h, _ptr(psi), _ptr(out), _stream()))
         return out
 
+    def matrix(self, reference_convention: bool = True) -> torch.Tensor:
+        """Dense <I|O|J> on the device (dim x dim), reference sign convention by default."""
+        d = self.space.dim
+        M = torch.empty((d, d), dtype=torch.float64, device="cuda")
+        N.check(N.lib().sq_operator_matrix(self.space.handle, self._h, int(reference_convention), _ptr(M), _stream()))
+        return M
+
 
 _SPACES: dict = {}
 

@@ -176,32 +176,37 @@ def expectation_value(bra, operators: list, ket, ci_info: CI_Info, thetas=None, 
 
 
 def build_operator_matrix(op: FermionicOperator, ci_info: CI_Info, do_unsafe: bool = False) -> np.ndarray:
-    """Dense matrix of an active-space operator (operator_state_algebra.py:413-469), column by column."""
+    """Dense matrix of an active-space operator (operator_state_algebra.py:413-469),
+    built in one launch on the device (sq_operator_matrix)."""
     sp = _space(ci_info)
     O = E.Operator(sp, op.operators.items(), drop_outside=do_unsafe)
-    d = sp.dim
-    M = np.zeros((d, d))
-    eye = torch.zeros(d, dtype=torch.float64, device="cuda")
-    for j in range(d):
-        eye.zero_()
-        eye[j] = 1.0
-        ref = sp.from_reference(eye)
-        M[:, j] = _to_host(sp, O.apply(ref))
-    return M
+    return O.matrix().cpu().numpy()
 
 
 # ---- state-averaged (leading state axis) forms -------------------------------
 # operator_state_algebra.py:282-410, 633-867, 1415-1864, 2312-2976: the same
-# operations on every row of a (n_states, dim) array.  Each row is propagated
-# on the device with the single-state kernels; programs and operators are
-# built once per call.
+# operations on every row of a (n_states, dim) array.  The array crosses to
+# the device and back once per call; each row is propagated there with the
+# single-state kernels (no host synchronisation between rows); programs and
+# operators are built once per call.
 
 
-def _rows(sp: E.Space, state) -> np.ndarray:
+def _rows(sp: E.Space, state) -> list[torch.Tensor]:
+    """Rows of a (n_states, dim) host array on the device in the engine
+    convention: one host-to-device copy for the whole array."""
     a = np.asarray(state, dtype=np.float64)
     if a.ndim != 2 or a.shape[1] != sp.dim:
         raise ValueError(f"states have shape {a.shape}, expected (n_states, {sp.dim})")
-    return a
+    X = torch.as_tensor(np.ascontiguousarray(a)).to("cuda")
+    return [sp.from_reference(X[k]) for k in range(X.shape[0])]
+
+
+def _rows_to_host(sp: E.Space, rows: list[torch.Tensor]) -> np.ndarray:
+    """Device rows back to a host (n_states, dim) array in the reference
+    convention: one device-to-host copy."""
+    if not rows:
+        return np.zeros((0, sp.dim))
+    return torch.stack([sp.to_reference(r) for r in rows]).cpu().numpy()
 
 
 def propagate_state_SA(operators: list, state, ci_info: CI_Info, thetas: Sequence[float] | None = None,
@@ -212,15 +217,14 @@ def propagate_state_SA(operators: list, state, ci_info: CI_Info, thetas: Sequenc
     if len(operators) == 0:
         return np.array(state, dtype=np.float64, copy=True)
     sp = _space(ci_info)
-    st = _rows(sp, state)
+    rows = _rows(sp, state)
     steps = _compile_list(operators, ci_info, sp, thetas, wf_struct, do_folding, do_unsafe, ups_only=True)
-    out = np.empty_like(st)
-    for k, row in enumerate(st):
-        psi = sp.from_reference(row)
+    out = []
+    for psi in rows:
         for step in steps:
             psi = step(psi)
-        out[k] = _to_host(sp, psi)
-    return out
+        out.append(psi)
+    return _rows_to_host(sp, out)
 
 
 def expectation_value_SA(bra, operators: list, ket, ci_info: CI_Info, thetas: Sequence[float] | None = None,
@@ -235,32 +239,29 @@ def construct_ups_state_SA(state, ci_info: CI_Info, thetas: Sequence[float], ups
                            dagger: bool = False) -> np.ndarray:
     """U(theta) (or U^+) on every row (operator_state_algebra.py:1415-1864)."""
     sp = _space(ci_info)
-    st = _rows(sp, state)
     if len(thetas) != ups_struct.n_params:
         raise ValueError(f"Expected {ups_struct.n_params} thetas got {len(thetas)}")
+    rows = _rows(sp, state)
     prog = _program(sp, ups_struct)
-    th = np.asarray(thetas, dtype=np.float64)
-    return np.stack([_to_host(sp, prog.state_prep(th, sp.from_reference(r), dagger=dagger)) for r in st])
+    th = prog.theta_tensor(np.asarray(thetas, dtype=np.float64))
+    return _rows_to_host(sp, [prog.state_prep(th, r, dagger=dagger) for r in rows])
 
 
 def propagate_unitary_SA(state, idx: int, ci_info: CI_Info, thetas: Sequence[float],
                          ups_struct: UpsStructure) -> np.ndarray:
     """Gate idx on every row (operator_state_algebra.py:2312-2754)."""
     sp = _space(ci_info)
-    st = _rows(sp, state)
+    rows = _rows(sp, state)
     prog = _program(sp, ups_struct)
-    th = np.asarray(thetas, dtype=np.float64)
-    out = np.empty_like(st)
-    for k, r in enumerate(st):
-        psi = sp.from_reference(r)
-        prog.apply_gate(idx, th, psi)
-        out[k] = _to_host(sp, psi)
-    return out
+    th = prog.theta_tensor(np.asarray(thetas, dtype=np.float64))
+    for r in rows:
+        prog.apply_gate(idx, th, r)
+    return _rows_to_host(sp, rows)
 
 
 def get_grad_action_SA(state, idx: int, ci_info: CI_Info, ups_struct: UpsStructure) -> np.ndarray:
     """T_idx on every row (operator_state_algebra.py:2868-2976)."""
     sp = _space(ci_info)
-    st = _rows(sp, state)
+    rows = _rows(sp, state)
     prog = _program(sp, ups_struct)
-    return np.stack([_to_host(sp, prog.grad_action(idx, sp.from_reference(r))) for r in st])
+    return _rows_to_host(sp, [prog.grad_action(idx, r) for r in rows])

@@ -30,6 +30,12 @@ from .util import UpsStructure
 from .wavefunction_base import WaveFunctionBase
 
 
+
+# Largest CI dimension for which the K states' energies and gradients run as
+# one batched launch (one workgroup per state) rather than state by state on
+# the whole GPU.
+_BATCH_MAX_DIM = 20000
+
 class WaveFunctionSAUPS(WaveFunctionBase):
     def __init__(
         self,
@@ -210,7 +216,30 @@ class WaveFunctionSAUPS(WaveFunctionBase):
         self._rdm1 = r1 / self.num_states
         self._rdm2 = r2 / self.num_states
 
+    def _state_batch(self):
+        """The K states' energies and adjoint gradients as one persistent launch
+        (instances started from each reference state), where the batched
+        kernel applies and pays: rotation gates, at most _BATCH_MAX_DIM
+        determinants (one workgroup per state; larger spaces run each state
+        across the whole GPU instead).  None otherwise."""
+        if self._space().dim > _BATCH_MAX_DIM:
+            return None
+        return self._shifted_batch(1)
+
+    def _batch_run(self):
+        """(energies[K], gradients[K]) of the current theta from the state batch."""
+        th = np.asarray(self._thetas, dtype=np.float64)
+        H = self._hamiltonian()
+        hit = getattr(self, "_batch_result", None)
+        if hit is not None and hit[0] == th.tobytes() and hit[1] is H:
+            return hit[2]
+        res = self._state_batch().run([th] * self.num_states, graph=True)
+        self._batch_result = (th.tobytes(), H, res)  # H kept: compared by identity
+        return res
+
     def _state_energy_list(self) -> np.ndarray:
+        if self._state_batch() is not None:
+            return np.asarray(self._batch_run()[0], dtype=np.float64)
         H = self._hamiltonian()
         return np.array([H.sigma(p)[1] for p in self._psis()])
 
@@ -322,8 +351,12 @@ class WaveFunctionSAUPS(WaveFunctionBase):
             H = self._hamiltonian()
             th = np.asarray(self._thetas, dtype=np.float64)
             g = np.zeros(len(th))
-            for csf in self._csfs_dev():
-                g += self._prog().energy_grad(H, th, csf, self.gradient_method)[1]
+            if self.gradient_method == "adjoint" and self._state_batch() is not None:
+                for gk in self._batch_run()[1]:
+                    g += gk
+            else:
+                for csf in self._csfs_dev():
+                    g += self._prog().energy_grad(H, th, csf, self.gradient_method)[1]
             gradient[num_kappa:] += g / self.num_states
             self.num_energy_evals += 2 * np.sum(list(self.ups_layout.grad_param_R.values())) * self.num_states
         return gradient

@@ -161,3 +161,26 @@ def test_sa_h4_rotosolve_lowers_the_averaged_energy():
     assert wf.sa_energy < e0 - 1e-3
     # the subspace energies average to the state-averaged energy
     assert abs(np.mean(wf.energy_states) - wf.sa_energy) < 1e-10
+
+
+def test_sa_h4_batched_states_equal_state_loop(monkeypatch):
+    """The K states' energies and averaged adjoint gradient from one batched
+    launch (instances started from each reference state) equal the
+    state-by-state path to 1e-12."""
+    from slowquant_amd import sa_ups_wavefunction as S
+
+    wf = _h4_sa(2)
+    rng = np.random.default_rng(5)
+    theta = rng.normal(0.0, 0.2, wf.ups_layout.n_params)
+    wf.thetas = theta.tolist()
+    assert wf._state_batch() is not None
+    e_b = wf._state_energy_list()
+    g_b = wf._calc_gradient_optimization(theta, True, False)
+    monkeypatch.setattr(S, "_BATCH_MAX_DIM", 0)
+    wf2 = _h4_sa(2)
+    wf2.thetas = theta.tolist()
+    assert wf2._state_batch() is None
+    e_l = wf2._state_energy_list()
+    g_l = wf2._calc_gradient_optimization(theta, True, False)
+    assert np.abs(e_b - e_l).max() < 1e-12
+    assert np.abs(g_b - g_l).max() < 1e-12

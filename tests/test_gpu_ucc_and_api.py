@@ -241,3 +241,42 @@ def exp_single(op, psi, ci):
     from oracle.apply import propagate_state
 
     return propagate_state([op], psi, ci)
+
+
+@pytest.mark.parametrize("n,na,nb", [(4, 2, 2), (6, 3, 3), (5, 3, 2)])
+def test_operator_matrix_against_oracle(n, na, nb):
+    """build_operator_matrix (one launch, reference sign convention): M psi
+    equals the oracle's propagate_state of the same operator for random
+    states (1e-12), for E_pq products, a spin-adapted double generator and
+    the Hamiltonian; the generator's matrix is antisymmetric and H's
+    symmetric."""
+    from oracle import fermion as F
+    from oracle.apply import propagate_state as o_prop
+    from oracle.ci import get_indexing as o_idx
+
+    from slowquant_amd import operator_state_algebra as OSA
+    from slowquant_amd import operators as OP
+    from slowquant_amd.ci_spaces import get_indexing
+    from slowquant_amd.synthetic import synthetic_problem
+
+    ci = get_indexing(0, n, 0, na, nb)
+    oci = o_idx(0, n, 0, na, nb)
+    d = len(ci.idx2det)
+    rng = np.random.default_rng(10 + n)
+    h, g, _ = synthetic_problem(n, n, 1, 0.1, n)
+    cases = [
+        (OP.Epq(1, 2) * OP.Epq(2, 0), F.Epq(1, 2) * F.Epq(2, 0), None),
+        (OP.G2_sa(0, 1, 2, 3, 4, True), F.G2_sa(0, 1, 2, 3, 4, ah=True), "anti"),
+        (OP.hamiltonian_0i_0a(h, g, 0, n), F.hamiltonian_0i_0a(h, g, 0, n), "sym"),
+    ]
+    for op, oop, kind in cases:
+        M = OSA.build_operator_matrix(op, ci)
+        assert M.shape == (d, d)
+        for _ in range(3):
+            psi = rng.normal(size=d)
+            exp = o_prop([oop], psi, oci, do_folding=False)
+            assert np.abs(M @ psi - exp).max() < 1e-12 * max(1.0, np.abs(exp).max())
+        if kind == "anti":
+            assert np.abs(M + M.T).max() < 1e-13
+        if kind == "sym":
+            assert np.abs(M - M.T).max() < 1e-12 * np.abs(M).max()
