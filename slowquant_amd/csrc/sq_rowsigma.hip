@@ -257,11 +257,15 @@ __global__ __launch_bounds__(256) void k_alpha_gemm(const double* __restrict__ M
 // amplitude and beta table loads of later steps before its MFMAs.
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-template <int MT>
+template <int MT, int NSTEPS = 0>
 __device__ __forceinline__ void row_tile(double4_t (&acc)[MT], const double* __restrict__ sA,
                                          const double* __restrict__ ps, const int2* __restrict__ ra,
                                          const double* __restrict__ psi, const int2* __restrict__ rtb, int nsb,
-                                         int ib, int kq, int li, int nsteps) {
+                                         int ib, int kq, int li, int nsteps_rt) {
+  // NSTEPS > 0: the step count is known at compile time and the loop is
+  // unrolled, so the loads issued ahead stay in flight across steps (a loop
+  // header makes the compiler wait for every outstanding load)
+  const int nsteps = NSTEPS > 0 ? NSTEPS : nsteps_rt;
   constexpr int W = MT * 16;
   const uint32_t ibc = (uint32_t)(ib < nsb ? ib : nsb - 1);
 #pragma unroll
@@ -279,6 +283,7 @@ __device__ __forceinline__ void row_tile(double4_t (&acc)[MT], const double* __r
   double xa1, xb1;
   int ca1, cb1;
   const double* arow = sA + (size_t)kq * W + li;
+#pragma unroll
   for (int s = 0; s < nsteps; s += 2) {
     a = ra[(s + 1) * 4 + kq];
     xa1 = psi[(uint32_t)a.x + ibc];
@@ -465,7 +470,10 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT 
       const int r = f / tpr, c = f - r * tpr;
       const int ib = c * 16 + li;
       double4_t acc[MT];
-      row_tile<MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
+      if (K8 == 16 * MT)
+        row_tile<MT, 4 * MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
+      else
+        row_tile<MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
       // partners and coefficients of all 4 MT outputs first (rows m >= npair
       // and columns past nsb read row 0 / the last column and add zero), then
       // the LDS adds, with no branch between the loads
