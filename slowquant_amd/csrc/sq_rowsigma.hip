@@ -49,6 +49,11 @@ constexpr int kMaxSplit = 4;     // alpha GEMM partial products (workspace is si
 
 __host__ __device__ inline int rup(int x, int m) { return (x + m - 1) / m * m; }
 
+__global__ void k_zero(double* __restrict__ x, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = 0.0;
+}
+
 // ---------------------------------------------------------------------------
 // Space tables.
 // rta[k][s] (k < K8 + 16): {partner row * nsb, 2 * coefficient} of E+a_k on
@@ -416,20 +421,109 @@ __global__ __launch_bounds__(kRowThreads) void k_sigma_rows(
   }
 }
 
+// One 128 x 64 tile of Ma C over the contraction range [kb, ke) for the
+// fused sigma launch (512 threads: wave w owns rows 32 (w >> 1) and columns
+// 32 (w & 1) as 2 x 2 MFMA tiles); the tile is added to sigma by global
+// atomics and its <psi|.> partial returned (thread 0's value is the block's).
+constexpr int kGemmRows = 128, kGemmCols = 64;
+
+__device__ __forceinline__ double alpha_gemm_tile(int i0, int j0, int kb, int ke, const double* __restrict__ Ma,
+                                                  const double* __restrict__ C, double* __restrict__ sigma, int nsa,
+                                                  int nsb, double* smem, double* red) {
+  double(*sM)[kGemmRows + 1] = reinterpret_cast<double(*)[kGemmRows + 1]>(smem);          // [16][129]: [k][i]
+  double(*sC)[kGemmCols + 1] = reinterpret_cast<double(*)[kGemmCols + 1]>(smem + 16 * (kGemmRows + 1));
+  const int t = threadIdx.x, wave = t >> 6, lane = t & 63, kq = lane >> 4, li = lane & 15;
+  const int wi = (wave >> 1) * 32, wj = (wave & 1) * 32;
+  double4_t acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = double4_t{0.0, 0.0, 0.0, 0.0};
+  const int mr = t >> 2, mc = (t & 3) * 4;   // Ma slice 128 x 16
+  const int cr = t >> 5, cc = (t & 31) * 2;  // C slice 16 x 64
+  double rm[4], rc[2];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int I = i0 + mr, K = k0 + mc + j;
+      rm[j] = (I < nsa && K < ke) ? Ma[(size_t)I * nsa + K] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int Kc = k0 + cr, c = j0 + cc + j;
+      rc[j] = (Kc < ke && c < nsb) ? C[(size_t)Kc * nsb + c] : 0.0;
+    }
+  };
+  if (kb < ke) load(kb);
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) sM[mc + j][mr] = rm[j];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) sC[cr][cc + j] = rc[j];
+    __syncthreads();
+    if (k0 + 16 < ke) load(k0 + 16);
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const double a0 = sM[kk + kq][wi + li], a1 = sM[kk + kq][wi + 16 + li];
+      const double b0 = sC[kk + kq][wj + li], b1 = sC[kk + kq][wj + 16 + li];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
+  double e = 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int I = i0 + wi + a * 16 + kq + 4 * r, c = j0 + wj + b * 16 + li;
+        if (I < nsa && c < nsb) {
+          const size_t x = (size_t)I * nsb + c;
+          unsafeAtomicAdd(sigma + x, acc[a][b][r]);
+          e += acc[a][b][r] * C[x];
+        }
+      }
+  for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
+  if (lane == 0) red[wave] = e;
+  __syncthreads();
+  double tsum = 0.0;
+  if (t == 0)
+    for (int w = 0; w < kRowWaves; ++w) tsum += red[w];
+  return tsum;
+}
+
 // Row kernel, scatter form: LDS = A slice [K8][W] | sigma rows [R][nsb] |
 // psi rows [R][nsb] | alpha operand rows [R][K8 + 16] (small enough for two
 // blocks per CU).  Waves take tiles independently; each output G_k[I_a, J_b]
 // goes to the sigma row at P_k(J_b) (E+b_k is symmetric, so the partner and
 // coefficient are rtb[k][J_b]) by an LDS atomic add.  The order of those
 // adds varies between runs, so results agree to rounding, not bit for bit.
-template <int MT>
+// FUSED: the launch also holds the Ma C tiles as extra blocks after the row
+// blocks (they fill the CUs the last row blocks leave idle); every block adds
+// its part to sigma (zeroed beforehand) by global atomics.
+template <int MT, bool FUSED>
 __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT <= 5 ? 4 : 2, 8))) void k_sigma_rows_scatter(
     const double* __restrict__ At, int kpad, int K4, int K8, int mgroups, int npair, const double* __restrict__ psi,
     const double* __restrict__ sin, int nsplit, double* __restrict__ sigma, double* __restrict__ partial,
-    const int2* __restrict__ rta, const int2* __restrict__ rtb, int nsa, int nsb, int R, int xcd) {
+    const int2* __restrict__ rta, const int2* __restrict__ rtb, int nsa, int nsb, int R, int xcd, int nrowblk,
+    const double* __restrict__ Ma, int gkchunk) {
   constexpr int W = MT * 16;
   extern __shared__ double smem[];
   __shared__ double red[kRowWaves];
+  if (FUSED && (int)blockIdx.x >= nrowblk) {
+    const int g = (int)blockIdx.x - nrowblk;
+    const int tn = (nsb + kGemmCols - 1) / kGemmCols, tm = (nsa + kGemmRows - 1) / kGemmRows;
+    const int z = g / (tm * tn), rem = g - z * (tm * tn);
+    const int kb = z * gkchunk;
+    const double e = alpha_gemm_tile((rem / tn) * kGemmRows, (rem % tn) * kGemmCols, kb, min(nsa, kb + gkchunk), Ma,
+                                     psi, sigma, nsa, nsb, smem, red);
+    if (threadIdx.x == 0) partial[blockIdx.x] = e;
+    return;
+  }
   const int tpr = (nsb + 15) >> 4;
   const int RA = K8 + 16;
   double* sA = smem;
@@ -440,9 +534,10 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT 
   // with xcd, XCD x (workgroups go to the eight XCDs round robin) takes a
   // contiguous range of row groups, whose alpha partner rows overlap in its L2
   int blk = blockIdx.x;
-  if (xcd) {
-    const int q = (int)gridDim.x >> 3, rr = (int)gridDim.x & 7, x = blk & 7;
-    blk = x * q + min(x, rr) + (blk >> 3);
+  const int nrb = FUSED ? nrowblk : (int)gridDim.x;
+  if (xcd && blk < (nrb & ~7)) {  // the remainder past a multiple of 8 keeps its order
+    const int q = nrb >> 3, x = blk & 7;
+    blk = x * q + (blk >> 3);
   }
   const int row0 = blk * R;
   const int nr = min(R, nsa - row0);
@@ -503,8 +598,12 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT 
   double e = 0.0;
   for (int tau = tid; tau < ntg; tau += kRowThreads) {
     double v = sAcc[tau];
-    for (int z = 0; z < nsplit; ++z) v += sin[(size_t)z * dim + base + tau];
-    sigma[base + tau] = v;
+    if (FUSED) {
+      unsafeAtomicAdd(sigma + base + tau, v);
+    } else {
+      for (int z = 0; z < nsplit; ++z) v += sin[(size_t)z * dim + base + tau];
+      sigma[base + tau] = v;
+    }
     e += v * sPsi[tau];
   }
   for (int off = 32; off > 0; off >>= 1) e += __shfl_down(e, off, 64);
@@ -519,7 +618,8 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT 
 
 struct RowPlan {
   int mt, mgroups, R, ngroups, nsplit, kchunk;
-  bool scatter;
+  bool scatter, fused;
+  int ngemm;  // fused: Ma C tile blocks after the row blocks
   size_t lds;
 };
 
@@ -599,6 +699,23 @@ bool row_plan(const Space& s, RowPlan* out) {
   out->kchunk = rup((s.nsa + nsplit - 1) / nsplit, 16);
   out->nsplit = (s.nsa + out->kchunk - 1) / out->kchunk;
   out->lds = row_lds(s, mt, bestR, scatter);
+  // fused form: the Ma C tiles (128 x 64, contraction split nsplit ways) as
+  // extra blocks of the scatter launch, where two row blocks share a CU (the
+  // tile blocks then fill the slots the last row blocks leave: (12e,12o)
+  // 258 + 54 -> 296 us); with one row block per CU (14 orbitals) the tile
+  // blocks would run at that block's LDS footprint, so the GEMM stays separate
+  static const int fuse_env = env_int("SQ_SIGMA_FUSED_GEMM", 1);
+  out->fused = scatter && fuse_env != 0 && 2 * out->lds <= kLdsMax;
+  out->ngemm = 0;
+  if (out->fused) {
+    if (s.sig_nsplit <= 0) {
+      out->kchunk = rup((s.nsa + 3) / 4, 16);
+      out->nsplit = (s.nsa + out->kchunk - 1) / out->kchunk;
+    }
+    out->ngemm = ((s.nsa + kGemmRows - 1) / kGemmRows) * ((s.nsb + kGemmCols - 1) / kGemmCols) * out->nsplit;
+    const size_t glds = 8 * (size_t)16 * (kGemmRows + 1 + kGemmCols + 1);
+    out->lds = std::max(out->lds, glds);
+  }
   return true;
 }
 
@@ -681,10 +798,12 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
   RowPlan pl;
   SQ_CHECK(row_plan(s, &pl) && H.d_Ma, SQ_ERR_ARG, "row sigma does not apply to this space");
   SQ_CHECK(pl.nsplit <= kMaxSplit && pl.ngroups <= s.nsa, SQ_ERR_ARG, "row sigma shape exceeds its workspace");
+  SQ_CHECK(!pl.fused || pl.ngroups + pl.ngemm <= row_sigma_work_doubles(s) - 64, SQ_ERR_ARG,
+           "row sigma shape exceeds its workspace");
   double* sinp = work;
   double* partial = work + (size_t)pl.nsplit * s.dim;
   *partial_out = partial;
-  {
+  if (!pl.fused) {
     dim3 grid((s.nsb + 63) / 64, (s.nsa + 63) / 64, pl.nsplit);
     const double flops = 2.0 * (double)s.nsa * s.nsa * s.nsb;
     const double bytes = 8.0 * ((double)s.nsa * s.nsa + (double)s.dim * (1 + pl.nsplit));
@@ -696,16 +815,40 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
   const double flops = 2.0 * (double)s.npair * (s.npair + 1) * (double)s.dim;
   // algorithmic traffic: psi read (rows and alpha partners through the cache), the alpha partials read, sigma written
   const double bytes = 8.0 * (double)s.dim * (2 + pl.nsplit);
+  if (pl.scatter && pl.fused) {
+    static const int xcd = env_int("SQ_ROWS_XCD", 1) != 0 ? 1 : 0;
+    k_zero<<<(unsigned)std::min<int64_t>((s.dim + 255) / 256, 8192), 256, 0, st>>>(sigma, s.dim);
+    SQ_HIP(hipGetLastError());
+    double* fpart = work;
+    *partial_out = fpart;
+    const double fl = flops + 2.0 * (double)s.nsa * s.nsa * s.nsb;
+    const int nblk = pl.ngroups + pl.ngemm;
+    switch (pl.mt) {
+#define SQ_RSF(MT)                                                                                              \
+  case MT:                                                                                                      \
+    allow_lds(k_sigma_rows_scatter<MT, true>, pl.lds);                                                          \
+    SQ_PROF(PC_SIGMA_ROWS, st, bytes, fl,                                                                       \
+            k_sigma_rows_scatter<MT, true><<<nblk, kRowThreads, pl.lds, st>>>(                                  \
+                H.d_A, H.kpad, K4, K8, pl.mgroups, s.npair, psi, nullptr, 0, sigma, fpart, s.d_rta, s.d_rtb,    \
+                s.nsa, s.nsb, pl.R, xcd, pl.ngroups, H.d_Ma, pl.kchunk));                                       \
+    break;
+      SQ_RSF(1) SQ_RSF(2) SQ_RSF(3) SQ_RSF(4) SQ_RSF(5) SQ_RSF(6) SQ_RSF(7) SQ_RSF(8) SQ_RSF(9)
+#undef SQ_RSF
+      default:
+        throw Error{SQ_ERR_ARG, "unsupported orbital count for the row sigma"};
+    }
+    return nblk;
+  }
   if (pl.scatter) {
     static const int xcd = env_int("SQ_ROWS_XCD", 1) != 0 ? 1 : 0;
     switch (pl.mt) {
 #define SQ_RSS(MT)                                                                                              \
   case MT:                                                                                                      \
-    allow_lds(k_sigma_rows_scatter<MT>, pl.lds);                                                                \
+    allow_lds(k_sigma_rows_scatter<MT, false>, pl.lds);                                                         \
     SQ_PROF(PC_SIGMA_ROWS, st, bytes, flops,                                                                    \
-            k_sigma_rows_scatter<MT><<<pl.ngroups, kRowThreads, pl.lds, st>>>(                                  \
+            k_sigma_rows_scatter<MT, false><<<pl.ngroups, kRowThreads, pl.lds, st>>>(                           \
                 H.d_A, H.kpad, K4, K8, pl.mgroups, s.npair, psi, sinp, pl.nsplit, sigma, partial, s.d_rta,      \
-                s.d_rtb, s.nsa, s.nsb, pl.R, xcd));                                                             \
+                s.d_rtb, s.nsa, s.nsb, pl.R, xcd, pl.ngroups, nullptr, 0));                                     \
     break;
       SQ_RSS(1) SQ_RSS(2) SQ_RSS(3) SQ_RSS(4) SQ_RSS(5) SQ_RSS(6) SQ_RSS(7) SQ_RSS(8) SQ_RSS(9)
 #undef SQ_RSS
@@ -736,6 +879,7 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
 int64_t row_sigma_used_doubles(const Space& s) {
   RowPlan pl;
   if (!row_plan(s, &pl)) return -1;
+  if (pl.fused) return (int64_t)pl.ngroups + pl.ngemm;
   return (int64_t)pl.nsplit * s.dim + pl.ngroups;
 }
 
