@@ -90,11 +90,12 @@ int sq_state_to_reference(const sq_space* s, const double* internal, double* ref
 int sq_program_create(const sq_space* s, const sq_gate_desc* gates, int n_gates, sq_program** out);
 int sq_program_destroy(sq_program* p);
 int sq_program_ngates(const sq_program* p);
-/* Elements per tile of the fused rotation stages (0: automatic, 256..4096 by
- * the space's dimension); no counterpart in the reference (its gates have no
+/* Tile shape of the fused rotation stages: elements per tile (0: automatic,
+ * 256..2048 by the space's dimension; at most 4096) and threads per tile block
+ * (256 or 512; 0: 512).  No counterpart in the reference (its gates have no
  * tiles).  Synchronises the device and drops the program's tile plan, which
  * the next call rebuilds. */
-int sq_program_set_tile_target(sq_program* p, int target);
+int sq_program_set_tile_shape(sq_program* p, int target, int threads);
 
 /* construct_ups_state (operator_state_algebra.py:963-1412): psi_out = U(theta) psi_in,
  * or U(theta)^dagger psi_in when dagger != 0.  theta is a DEVICE array of

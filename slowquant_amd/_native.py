@@ -73,7 +73,7 @@ _SIGNATURES = {
     "sq_program_create": ([vp, ctypes.POINTER(GateDesc), i32, ctypes.POINTER(vp)], i32),
     "sq_program_destroy": ([vp], i32),
     "sq_program_ngates": ([vp], i32),
-    "sq_program_set_tile_target": ([vp, i32], i32),
+    "sq_program_set_tile_shape": ([vp, i32, i32], i32),
     "sq_state_prep": ([vp, vp, dp, dp, dp, i32, dp, vp], i32),
     "sq_apply_gate": ([vp, vp, i32, dp, dp, dp, vp], i32),
     "sq_grad_action": ([vp, vp, i32, dp, dp, vp], i32),

@@ -5,7 +5,7 @@ The reference's only tuning knob is its Numba thread count
 chosen here by timing on first use and remembered per device and workload:
 
 * single instance (a space and a gate program): the fused rotation stages'
-  tile size (sq_program_set_tile_target) by timing whole energy + gradient
+  tile size (sq_program_set_tile_shape: elements and threads per tile) by timing whole energy + gradient
   evaluations, then the row sigma's rows per block and alpha-GEMM split
   (sq_sigma_configure) by timing sigma alone;
 * batched instances: the persistent launch's workgroup count and kernel shape
@@ -32,6 +32,7 @@ from . import _native as N
 from . import engine as E
 
 TILE_TARGETS = (0, 256, 512, 1024, 2048, 4096)
+TILE_THREADS = (512, 256)
 SIGMA_ROWS = (0, 1, 2, 3, 4)
 SIGMA_SPLITS = (1, 2, 4)
 
@@ -85,15 +86,15 @@ def _time(fn, reps: int) -> float:
 
 
 def apply(space: E.Space, prog: E.Program, choice: dict) -> None:
-    prog.set_tile_target(int(choice.get("tile_target", 0)))
+    prog.set_tile_shape(int(choice.get("tile_target", 0)), int(choice.get("tile_threads", 0)))
     space.configure_sigma(int(choice.get("rows_per_block", 0)), int(choice.get("nsplit", 0)))
 
 
 def tune(space: E.Space, prog: E.Program, H: E.Hamiltonian, theta, psi0: torch.Tensor, reps: int = 5,
          use_cache: bool = True) -> dict:
-    """Pick the tile size and the sigma shape for (space, prog) by timing,
-    apply them, and return {"tile_target", "rows_per_block", "nsplit",
-    "ms_per_eval", "cached"}."""
+    """Pick the tile shape and the sigma shape for (space, prog) by timing,
+    apply them, and return {"tile_target", "tile_threads", "rows_per_block",
+    "nsplit", "ms_per_eval", "cached"}."""
     key = program_key(space, prog)
     if use_cache:
         hit = _load().get(key)
@@ -108,16 +109,17 @@ def tune(space: E.Space, prog: E.Program, H: E.Hamiltonian, theta, psi0: torch.T
     def evaluate():
         prog.energy_grad(H, th, psi0, "adjoint")
 
-    best_tile, best_t = 0, float("inf")
-    for t in TILE_TARGETS:
-        try:
-            prog.set_tile_target(t)
-            dt = _time(evaluate, reps)
-        except (ValueError, RuntimeError):
-            continue  # a tile size the stages cannot use
-        if dt < best_t:
-            best_tile, best_t = t, dt
-    prog.set_tile_target(best_tile)
+    best_tile, best_t = (0, 0), float("inf")
+    for nt in TILE_THREADS:
+        for t in TILE_TARGETS:
+            try:
+                prog.set_tile_shape(t, nt)
+                dt = _time(evaluate, reps)
+            except (ValueError, RuntimeError):
+                continue  # a tile size the stages cannot use
+            if dt < best_t:
+                best_tile, best_t = (t, nt), dt
+    prog.set_tile_shape(*best_tile)
     psi = prog.state_prep(th, psi0)
     best_sig, best_st = (0, 0), float("inf")
     for r in SIGMA_ROWS:
@@ -131,7 +133,8 @@ def tune(space: E.Space, prog: E.Program, H: E.Hamiltonian, theta, psi0: torch.T
                 best_sig, best_st = (r, ns), dt
     space.configure_sigma(*best_sig)
     ms = 1e3 * _time(evaluate, reps)
-    choice = {"tile_target": best_tile, "rows_per_block": best_sig[0], "nsplit": best_sig[1], "ms_per_eval": ms}
+    choice = {"tile_target": best_tile[0], "tile_threads": best_tile[1], "rows_per_block": best_sig[0],
+              "nsplit": best_sig[1], "ms_per_eval": ms}
     if use_cache:
         _store(key, choice)
     return {**choice, "cached": False}

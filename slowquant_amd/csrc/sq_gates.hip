@@ -818,15 +818,17 @@ int sq_program_destroy(sq_program* pr) {
 
 int sq_program_ngates(const sq_program* pr) { return pr ? (int)pr->p.gates.size() : -1; }
 
-int sq_program_set_tile_target(sq_program* pr, int target) {
+int sq_program_set_tile_shape(sq_program* pr, int target, int threads) {
   return guarded([&] {
-    SQ_CHECK(pr && target >= 0, SQ_ERR_ARG, "invalid arguments");
-    if (pr->p.tile_target == target) return;
+    SQ_CHECK(pr && target >= 0 && (threads == 0 || threads == 256 || threads == 512), SQ_ERR_ARG,
+             "tile target must be >= 0 and threads 0, 256 or 512");
+    if (pr->p.tile_target == target && pr->p.tile_threads == threads) return;
     SQ_HIP(hipDeviceSynchronize());  // the old plan's buffers may be in use by queued kernels
     if (pr->p.tiles) destroy_tile_plan(pr->p.tiles);
     pr->p.tiles = nullptr;
     pr->p.tiles_built = false;
     pr->p.tile_target = target;
+    pr->p.tile_threads = threads;
   });
 }
 

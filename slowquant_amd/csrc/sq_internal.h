@@ -138,7 +138,8 @@ struct Program {
   const Space* space;
   mutable TilePlan* tiles = nullptr;  // built on first use (sq_tiles.hip)
   mutable bool tiles_built = false;
-  int tile_target = 0;  // elements per tile (0: automatic; sq_program_set_tile_target)
+  int tile_target = 0;   // elements per tile (0: automatic; sq_program_set_tile_shape)
+  int tile_threads = 0;  // threads per tile block, 256 or 512 (0: 512)
   std::vector<GateDev> gates;
   std::vector<Term> terms;
   GateDev* d_gates = nullptr;

@@ -164,6 +164,7 @@ struct Stage {
 struct TilePlan {
   std::vector<Stage> stages;
   int ntiled = 0;
+  int nt = kNT;  // threads per block of the stage kernels (256 or 512)
   double* d_partial = nullptr;  // gradient partials of all tiled stages
   int64_t partial_len = 0;
   double* d_mats = nullptr;     // [ntiled][kMaxPairs][kMatVals]
@@ -596,6 +597,7 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
       off += (int64_t)S.ntiles * S.npairs * kQVals;
     }
     T->ntiled = (int)info.size();
+    T->nt = P.tile_threads == 256 ? 256 : kNT;
     SQ_HIP(hipMalloc(&T->d_partial, sizeof(double) * std::max<int64_t>(off, 1)));
     T->partial_len = off;
     SQ_HIP(hipMalloc(&T->d_mats, sizeof(double) * std::max<int64_t>((int64_t)T->ntiled * kMaxPairs * kMatVals, 1)));
@@ -1290,12 +1292,22 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
           k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
 }
 
-static void launch_fwd(const Stage& S, bool dagger, const double* src, double* dst, hipStream_t st) {
+static void launch_fwd(const Stage& S, int nt, bool dagger, const double* src, double* dst, hipStream_t st) {
+  if (nt == 256) {
+    allow_lds(k_stage_fwd<256>, S.lds_fwd);
+    k_stage_fwd<256><<<tile_grid(S.dev.nb, 16), 256, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
+    return;
+  }
   allow_lds(k_stage_fwd<kNT>, S.lds_fwd);
   k_stage_fwd<kNT><<<tile_grid(S.dev.nb, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
 }
 
-static void launch_rev(const Stage& S, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
+static void launch_rev(const Stage& S, int nt, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
+  if (nt == 256) {
+    allow_lds(k_stage_rev<256>, S.lds_rev);
+    k_stage_rev<256><<<tile_grid(S.dev.nb, 16), 256, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
+    return;
+  }
   allow_lds(k_stage_rev<kNT>, S.lds_rev);
   k_stage_rev<kNT><<<tile_grid(S.dev.nb, 8), kNT, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
 }
@@ -1325,7 +1337,7 @@ void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const do
       }
       continue;
     }
-    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, dagger, src, psi, st));
+    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, T.nt, dagger, src, psi, st));
     src = psi;
   }
   if (src != psi) copy_vec(src, psi, s.dim, st);
@@ -1345,7 +1357,7 @@ void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const do
       continue;
     }
     const int wb = i > 0 ? 1 : 0;
-    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(S, psi, lam, T.d_partial, wb, st));
+    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(S, T.nt, psi, lam, T.d_partial, wb, st));
   }
   if (T.ntiled > 0)
     SQ_PROF(PC_REDUCE, st, 8.0 * (double)T.partial_len, 0.0,

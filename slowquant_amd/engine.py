@@ -129,9 +129,10 @@ class Program:
         except (AttributeError, TypeError):  # interpreter shutdown: module globals already cleared
             pass
 
-    def set_tile_target(self, target: int) -> None:
-        """Elements per tile of the fused rotation stages (0: automatic)."""
-        N.check(N.lib().sq_program_set_tile_target(self._h, int(target)))
+    def set_tile_shape(self, target: int = 0, threads: int = 0) -> None:
+        """Elements per tile (0: automatic) and threads per tile block (256 or
+        512; 0: 512) of the fused rotation stages."""
+        N.check(N.lib().sq_program_set_tile_shape(self._h, int(target), int(threads)))
 
     def theta_tensor(self, theta) -> torch.Tensor:
         if isinstance(theta, torch.Tensor):

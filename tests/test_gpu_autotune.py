@@ -36,11 +36,12 @@ def test_every_shape_gives_the_default_results(n):
     psi = prog.state_prep(theta, psi0)
     s0, _ = H.sigma(psi)
     scale = float(s0.abs().max())
-    for t in autotune.TILE_TARGETS:
-        prog.set_tile_target(t)
-        e, g = prog.energy_grad(H, theta, psi0, "adjoint")
-        assert abs(e - e0) < 1e-12 and np.abs(g - g0).max() < 1e-12, t
-    prog.set_tile_target(0)
+    for nt in autotune.TILE_THREADS:
+        for t in autotune.TILE_TARGETS:
+            prog.set_tile_shape(t, nt)
+            e, g = prog.energy_grad(H, theta, psi0, "adjoint")
+            assert abs(e - e0) < 1e-12 and np.abs(g - g0).max() < 1e-12, (t, nt)
+    prog.set_tile_shape(0, 0)
     for det in (0, 1):
         for r in autotune.SIGMA_ROWS:
             for ns in autotune.SIGMA_SPLITS:
