@@ -90,13 +90,14 @@ constexpr int kMatVals = 36;
 // descriptor: kind (2 bits: 0 pair empty, 1 pair doubly occupied, 2 singly
 // occupied with hypercube bit 0, 3 singly occupied with hypercube bit 1) and
 // the hypercube bit index (3 bits).
-// One tile: alpha rows arows[r0, r0+R), beta stage positions [c0, c1), the
-// alpha group's pair descriptor and the tile's column run.
+// One dispatch slot: alpha rows arows[r0, r0+R), beta stage positions
+// [c0, c1), the alpha group's pair descriptor, the tile's column run and that
+// run's table offset, and the tile's index among the stage's tiles (its
+// gradient partials).  R = 0 marks an empty slot.
 struct TileRec {
   int32_t r0, R, c0, c1;
   uint32_t adesc;
-  int32_t run;
-  int32_t pad[2];
+  int32_t run, toff, tile;
 };
 
 // Orbit column lists of one run of beta stage positions [c0, c1), shared by
@@ -105,26 +106,28 @@ struct TileRec {
 // singly occupied with hypercube bit 0 (cnt[0][j] of them: 4-member orbits,
 // or 2-member beta orbits when the alpha pair is not single), then the
 // columns with beta pair j empty (cnt[1][j]), then doubly occupied
-// (cnt[2][j]).  Entry = column within the run (10 bits) | hypercube bit of
+// (cnt[2][j]).  inv[t][j] = 1 / max(cnt[t][j], 1) and inv[3][j] =
+// 1 / max(cnt[1][j] + cnt[2][j], 1) (float) split work items into (row,
+// list entry) without a division in the kernels.  Entry = column within the run (10 bits) | hypercube bit of
 // the pair << 10.  Columns holding the upper member of a single beta pair are
 // reached from their lower member and listed nowhere.  The counts are 32-bit
 // so that the tile kernels read them with uniform (scalar) loads.
 struct ColRun {
   int32_t c0, c1, toff, pad;
   int32_t cnt[3][8];
+  float inv[4][8];
 };
 
 struct StageDev {
   int npairs;
   int nsb;
-  int nt;                    // tiles (largest first)
-  int nb;                    // dispatch slots: block slot b runs tile order[b] (-1: none)
-  const int32_t* order;      // XCD-aware dispatch order of the tiles
+  int nt;                    // tiles
+  int nb;                    // dispatch slots (the tiles in XCD-aware order, and empty slots)
   int64_t poff;              // this stage's block of the gradient partials
   const double* mats;        // this stage's pair matrices [npairs][kMatVals]
   const int32_t* arows;      // concatenated alpha rows (canonical alpha index), groups sorted by hypercube index
   const int32_t* bcol;       // per beta stage position: canonical beta column
-  const struct TileRec* tiles;
+  const struct TileRec* tiles;  // [nb]: slot b's tile
   const ColRun* runs;        // column runs of the stage's tiles
   const uint16_t* ctab;      // their orbit column tables (runs start 16-byte aligned)
 };
@@ -458,8 +461,10 @@ static void finalize_stage(const Space& sp, Stage& S, int program_target) {
       }
       for (int t = 0; t < 3; ++t) {
         r.cnt[t][j] = (int32_t)lists[t].size();
+        r.inv[t][j] = 1.0f / (float)std::max<int32_t>(r.cnt[t][j], 1);
         ctab.insert(ctab.end(), lists[t].begin(), lists[t].end());
       }
+      r.inv[3][j] = 1.0f / (float)std::max<int32_t>(r.cnt[1][j] + r.cnt[2][j], 1);
       ctab.resize((size_t)r.toff + (size_t)(j + 1) * W, 0);
     }
     ctab.resize((ctab.size() + 7) / 8 * 8, 0);  // next run 16-byte aligned
@@ -470,15 +475,15 @@ static void finalize_stage(const Space& sp, Stage& S, int program_target) {
   std::vector<TileRec> recs(tiles.size());
   for (size_t t = 0; t < tiles.size(); ++t) {
     const int ga = tiles[t].x;
-    recs[t] = TileRec{A.goff[ga], A.goff[ga + 1] - A.goff[ga], tiles[t].y, tiles[t].z, adesc[ga],
-                      run_index(tiles[t].y, tiles[t].z), {0, 0}};
+    const int run = run_index(tiles[t].y, tiles[t].z);
+    recs[t] = TileRec{A.goff[ga], A.goff[ga + 1] - A.goff[ga], tiles[t].y, tiles[t].z, adesc[ga], run,
+                      runs[run].toff, (int32_t)t};
   }
   StageDev d{};
   d.npairs = S.npairs;
   d.nsb = sp.nsb;
   d.arows = upload(S.allocs, A.order);
   d.bcol = upload(S.allocs, B.order);
-  d.tiles = upload(S.allocs, recs);
   d.runs = upload(S.allocs, runs);
   d.ctab = upload(S.allocs, ctab);
   S.d_entries = upload(S.allocs, S.entries);
@@ -519,8 +524,12 @@ static void finalize_stage(const Space& sp, Stage& S, int program_target) {
       for (int x = 0; x < kXcds; ++x)
         for (size_t k = 0; k < bins[x].size(); ++k) order[k * kXcds + x] = bins[x][k];
     }
-    d.nb = (int)order.size();
-    d.order = upload(S.allocs, order);
+    // the slots hold the tile records themselves (no indirection in the kernels)
+    std::vector<TileRec> slots(order.size(), TileRec{0, 0, 0, 0, 0u, 0, 0, -1});
+    for (size_t b = 0; b < order.size(); ++b)
+      if (order[b] >= 0) slots[b] = recs[order[b]];
+    d.nb = (int)slots.size();
+    d.tiles = upload(S.allocs, slots);
   }
   S.dev = d;
   S.lds_fwd = S.lds_rev = 0;
@@ -864,6 +873,7 @@ struct TileView {
   int R, W, E, c0;
   float invW;
   uint32_t adesc;
+  int tile;  // index among the stage's tiles
   const ColRun* run;
 };
 
@@ -893,8 +903,8 @@ __device__ __forceinline__ TileLds carve_lds(double* smem, int E, int W, int npa
 // Tile descriptors into LDS: alpha rows, the columns' canonical indices and
 // the run's orbit column table (16-byte copies).
 template <int NT>
-__device__ __forceinline__ TileView tile_setup(const StageDev& S, double* smem, bool rev, int tile, TileLds& m) {
-  const TileRec T = S.tiles[tile];
+__device__ __forceinline__ TileView tile_setup(const StageDev& S, double* smem, bool rev, const TileRec& T,
+                                               TileLds& m) {
   TileView t;
   t.R = T.R;
   t.W = T.c1 - T.c0;
@@ -902,11 +912,12 @@ __device__ __forceinline__ TileView tile_setup(const StageDev& S, double* smem, 
   t.c0 = T.c0;
   t.invW = 1.0f / (float)t.W;
   t.adesc = T.adesc;
+  t.tile = T.tile;
   t.run = S.runs + T.run;
   m = carve_lds(smem, t.E, t.W, S.npairs, rev);
   for (int r = threadIdx.x; r < t.R; r += NT) m.rows[r] = S.arows[T.r0 + r];
   for (int c = threadIdx.x; c < t.W; c += NT) m.ccol[c] = (uint32_t)S.bcol[T.c0 + c];
-  const uint4* src = reinterpret_cast<const uint4*>(S.ctab + t.run->toff);
+  const uint4* src = reinterpret_cast<const uint4*>(S.ctab + T.toff);
   uint4* dst = reinterpret_cast<uint4*>(m.tab);
   const int n16 = (int)(tab_entries(S.npairs, t.W) / 8);
   for (int i = threadIdx.x; i < n16; i += NT) dst[i] = src[i];
@@ -981,9 +992,17 @@ __device__ __forceinline__ void mat4(const double (&M)[16], const double (&v)[4]
 // run's column lists, so every work item is one orbit and a wave runs one
 // orbit type except where two lists meet.  Orbit members: 0 = (alpha lower,
 // beta lower) at m0, 1 = alpha upper (+ dr), 2 = beta upper (+ dc), 3 = both.
-// The pair's matrix values are read into registers before the orbit loops: a
-// per-lane choice between two matrices inside the loop would otherwise turn
-// uniform loads into per-item vector loads.
+// The pair's matrix values are read into registers before the orbit loops,
+// and the 2-member orbits of beta-empty and beta-double columns run as
+// separate item ranges, so every matrix value stays a wave-uniform (scalar)
+// operand.
+template <bool TR>
+__device__ __forceinline__ void orbit2(double* X, int m0, int d, const double (&B)[4]) {
+  const double v0 = X[m0], v1 = X[m0 + d];
+  X[m0] = B[0] * v0 + (TR ? B[2] : B[1]) * v1;
+  X[m0 + d] = (TR ? B[1] : B[2]) * v0 + B[3] * v1;
+}
+
 template <int NT, bool TR>
 __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, const double* __restrict__ Mp, int j) {
   const uint32_t fa = (t.adesc >> (5 * j)) & 31u;
@@ -993,9 +1012,9 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
   const int W = t.W;
   if ((fa & 3u) >= 2u) {  // alpha pair single: 4-member orbits, then 2-member alpha orbits
     const int abit = (int)(fa >> 2);
-    const int Rh = t.R >> 1, dr = W << abit, n2 = n2e + n2d;
-    const int nA = Rh * n4, nT = nA + Rh * n2;
-    const float iA = 1.0f / (float)max(n4, 1), iB = 1.0f / (float)max(n2, 1);
+    const int Rh = t.R >> 1, dr = W << abit;
+    const int nA = Rh * n4, nE = nA + Rh * n2e, nT = nE + Rh * n2d;
+    const float iA = t.run->inv[0][j], iE = t.run->inv[1][j], iD = t.run->inv[2][j];
     double M[16], be[4], bd[4];
 #pragma unroll
     for (int q = 0; q < 16; ++q) M[q] = Mp[q];
@@ -1005,8 +1024,8 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
       bd[q] = Mp[20 + q];
     }
     for (int i = threadIdx.x; i < nT; i += NT) {
+      int rr, k;
       if (i < nA) {
-        int rr, k;
         split_item(i, n4, iA, rr, k);
         const uint32_t e = tab[k];
         const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
@@ -1018,17 +1037,12 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
         X[m0 + dr] = o[1];
         X[m2] = o[2];
         X[m2 + dr] = o[3];
-      } else {
-        int rr, k;
-        split_item(i - nA, n2, iB, rr, k);
-        const uint32_t e = tab[n4 + k];
-        const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
-        const bool dbl = k >= n2e;  // beta pair doubly occupied
-        const double b0 = dbl ? bd[0] : be[0], b1 = dbl ? bd[1] : be[1];
-        const double b2 = dbl ? bd[2] : be[2], b3 = dbl ? bd[3] : be[3];
-        const double v0 = X[m0], v1 = X[m0 + dr];
-        X[m0] = b0 * v0 + (TR ? b2 : b1) * v1;
-        X[m0 + dr] = (TR ? b1 : b2) * v0 + b3 * v1;
+      } else if (i < nE) {  // beta pair empty
+        split_item(i - nA, n2e, iE, rr, k);
+        orbit2<TR>(X, ins0(rr, abit) * W + (int)(tab[n4 + k] & 1023u), dr, be);
+      } else {  // beta pair doubly occupied
+        split_item(i - nE, n2d, iD, rr, k);
+        orbit2<TR>(X, ins0(rr, abit) * W + (int)(tab[n4 + n2e + k] & 1023u), dr, bd);
       }
     }
   } else {  // alpha pair empty or double: 2-member beta orbits; other amplitudes stay
@@ -1036,20 +1050,16 @@ __device__ __forceinline__ void fwd_pair(const TileView& t, const TileLds& m, co
     const double B[4] = {Bp[0], Bp[1], Bp[2], Bp[3]};
     const bool deriv = Mp[kPairVals] == 0.0;
     const int nA = t.R * n4;
-    const float iA = 1.0f / (float)max(n4, 1);
+    const float iA = t.run->inv[0][j];
     for (int i = threadIdx.x; i < nA; i += NT) {
       int rr, k;
       split_item(i, n4, iA, rr, k);
       const uint32_t e = tab[k];
-      const int m0 = rr * W + (int)(e & 1023u);
-      const int m2 = m0 + (1 << (e >> 10));
-      const double v0 = X[m0], v1 = X[m2];
-      X[m0] = B[0] * v0 + (TR ? B[2] : B[1]) * v1;
-      X[m2] = (TR ? B[1] : B[2]) * v0 + B[3] * v1;
+      orbit2<TR>(X, rr * W + (int)(e & 1023u), 1 << (e >> 10), B);
     }
     if (deriv) {  // derivative pair: T_g annihilates the amplitudes no gate moves
       const int n2 = n2e + n2d, nB = t.R * n2;
-      const float iB = 1.0f / (float)max(n2, 1);
+      const float iB = t.run->inv[3][j];
       for (int i = threadIdx.x; i < nB; i += NT) {
         int rr, k;
         split_item(i, n2, iB, rr, k);
@@ -1108,12 +1118,12 @@ template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ mats, int dagger,
                                                   const double* src, double* dst) {
   extern __shared__ double smem[];
-  // block b walks tiles b, b + grid, ... (tiles are sorted largest first)
+  // block b walks slots b, b + grid, ... (tiles are sorted largest first)
   for (int slot = blockIdx.x; slot < S.nb; slot += gridDim.x) {
-    const int tile = S.order[slot];
-    if (tile < 0) continue;
+    const TileRec T = S.tiles[slot];
+    if (T.R == 0) continue;
     TileLds m;
-    const TileView t = tile_setup<NT>(S, smem, false, tile, m);
+    const TileView t = tile_setup<NT>(S, smem, false, T, m);
     tile_load<NT, 1>(S, t, m, src, nullptr);
     __syncthreads();
     for (int j = 0; j < S.npairs; ++j) {
@@ -1124,6 +1134,20 @@ __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __re
     tile_store<NT, 1>(S, t, m, dst, nullptr);
     __syncthreads();  // the next tile overwrites the LDS tables
   }
+}
+
+// One 2-member orbit of the reverse sweep at (m0, m0 + d): returns its
+// product lam_0 psi_1 - lam_1 psi_0 and, with wb, un-applies the matrix (B^T)
+// on both vectors.
+__device__ __forceinline__ double rev_orbit2(double* X, double* L, int m0, int d, const double (&B)[4], bool wb) {
+  const double v0 = X[m0], v1 = X[m0 + d], w0 = L[m0], w1 = L[m0 + d];
+  if (wb) {
+    X[m0] = B[0] * v0 + B[2] * v1;
+    X[m0 + d] = B[1] * v0 + B[3] * v1;
+    L[m0] = B[0] * w0 + B[2] * w1;
+    L[m0 + d] = B[1] * w0 + B[3] * w1;
+  }
+  return w0 * v1 - w1 * v0;
 }
 
 // Reverse stage on (psi, lam), in place: per pair, the antisymmetric orbit
@@ -1141,10 +1165,10 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool wb = write_back != 0;
   for (int slot = blockIdx.x; slot < S.nb; slot += gridDim.x) {
-    const int tile = S.order[slot];
-    if (tile < 0) continue;
+    const TileRec T = S.tiles[slot];
+    if (T.R == 0) continue;
     TileLds m;
-    const TileView t = tile_setup<NT>(S, smem, true, tile, m);
+    const TileView t = tile_setup<NT>(S, smem, true, T, m);
     tile_load<NT, 2>(S, t, m, psi, lam);
     __syncthreads();
     const int W = t.W;
@@ -1160,9 +1184,9 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
       for (int q = 0; q < 8; ++q) Q[q] = 0.0;
       if ((fa & 3u) >= 2u) {
         const int abit = (int)(fa >> 2);
-        const int Rh = t.R >> 1, dr = W << abit, n2 = n2e + n2d;
-        const int nA = Rh * n4, nT = nA + Rh * n2;
-        const float iA = 1.0f / (float)max(n4, 1), iB = 1.0f / (float)max(n2, 1);
+        const int Rh = t.R >> 1, dr = W << abit;
+        const int nA = Rh * n4, nE = nA + Rh * n2e, nT = nE + Rh * n2d;
+        const float iA = t.run->inv[0][j], iE = t.run->inv[1][j], iD = t.run->inv[2][j];
         double M[16], be[4], bd[4];  // pair matrix in registers (see fwd_pair)
 #pragma unroll
         for (int q = 0; q < 16; ++q) M[q] = Mp[q];
@@ -1199,24 +1223,14 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
               L[m2] = ow[2];
               L[m2 + dr] = ow[3];
             }
-          } else {
+          } else if (i < nE) {  // beta pair empty
             int rr, k;
-            split_item(i - nA, n2, iB, rr, k);
-            const uint32_t e = tab[n4 + k];
-            const int m0 = ins0(rr, abit) * W + (int)(e & 1023u);
-            const bool dbl = k >= n2e;
-            const double v0 = X[m0], v1 = X[m0 + dr], w0 = L[m0], w1 = L[m0 + dr];
-            const double x = w0 * v1 - w1 * v0;
-            Q[6] += dbl ? 0.0 : x;
-            Q[7] += dbl ? x : 0.0;
-            if (wb) {
-              const double b0 = dbl ? bd[0] : be[0], b1 = dbl ? bd[1] : be[1];
-              const double b2 = dbl ? bd[2] : be[2], b3 = dbl ? bd[3] : be[3];
-              X[m0] = b0 * v0 + b2 * v1;
-              X[m0 + dr] = b1 * v0 + b3 * v1;
-              L[m0] = b0 * w0 + b2 * w1;
-              L[m0 + dr] = b1 * w0 + b3 * w1;
-            }
+            split_item(i - nA, n2e, iE, rr, k);
+            Q[6] += rev_orbit2(X, L, ins0(rr, abit) * W + (int)(tab[n4 + k] & 1023u), dr, be, wb);
+          } else {  // beta pair doubly occupied
+            int rr, k;
+            split_item(i - nE, n2d, iD, rr, k);
+            Q[7] += rev_orbit2(X, L, ins0(rr, abit) * W + (int)(tab[n4 + n2e + k] & 1023u), dr, bd, wb);
           }
         }
         double w2[2];
@@ -1235,21 +1249,12 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
         const double* Bp = Mp + ((fa & 3u) == 0u ? 24 : 28);
         const double B[4] = {Bp[0], Bp[1], Bp[2], Bp[3]};
         const int nA = t.R * n4;
-        const float iA = 1.0f / (float)max(n4, 1);
+        const float iA = t.run->inv[0][j];
         for (int i = threadIdx.x; i < nA; i += NT) {
           int rr, k;
           split_item(i, n4, iA, rr, k);
           const uint32_t e = tab[k];
-          const int m0 = rr * W + (int)(e & 1023u);
-          const int m2 = m0 + (1 << (e >> 10));
-          const double v0 = X[m0], v1 = X[m2], w0 = L[m0], w1 = L[m2];
-          Q[0] += w0 * v1 - w1 * v0;
-          if (wb) {
-            X[m0] = B[0] * v0 + B[2] * v1;
-            X[m2] = B[1] * v0 + B[3] * v1;
-            L[m0] = B[0] * w0 + B[2] * w1;
-            L[m2] = B[1] * w0 + B[3] * w1;
-          }
+          Q[0] += rev_orbit2(X, L, rr * W + (int)(e & 1023u), 1 << (e >> 10), B, wb);
         }
         double s = row_sum(Q[0]);
         s += __shfl_xor(s, 16, 64);
@@ -1267,7 +1272,7 @@ __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __re
       double v = 0.0;
       if (src >= 0)
         for (int w = 0; w < kWaves; ++w) v += red[j][src][w];
-      partial[S.poff + (size_t)x * S.nt + tile] = v;
+      partial[S.poff + (size_t)x * S.nt + t.tile] = v;
     }
     if (wb)  // not for the first stage of the program: the un-applied states are not needed
       tile_store<NT, 2>(S, t, m, psi, lam);
