@@ -46,6 +46,7 @@ namespace sq {
 constexpr int kMaxPairs = 6;        // hypercubes of at most 64 strings per spin
 constexpr int kTileMax = 4096;      // doubles per vector per tile
 constexpr int kTileMaxCols = 1024;  // columns per tile
+constexpr int kDenseCols = 256;     // columns per tile of a split stage's main launch
 constexpr int kTB = 256;            // threads per block of the reductions
 constexpr int kNT = 512;            // threads per block of the tile kernels
 constexpr int kMaxStageGates = 48;  // stage entries (sa_single counts twice)
@@ -160,6 +161,8 @@ struct Stage {
   int index = -1;              // position among the tiled stages
   StageDev dev{};              // launch descriptor
   size_t lds_fwd = 0, lds_rev = 0;  // dynamic LDS of the forward / reverse launches (largest tile)
+  StageDev dev_big{};          // tiles above the target when they run as a side launch (nb = 0: none)
+  size_t lds_fwd_big = 0, lds_rev_big = 0;
   const StageEntry* d_entries = nullptr;
   std::vector<void*> allocs;
 };
@@ -172,7 +175,12 @@ struct TilePlan {
   int64_t partial_len = 0;
   double* d_mats = nullptr;     // [ntiled][kMaxPairs][kMatVals]
   StageInfo* d_info = nullptr;  // [ntiled]
+  hipStream_t side = nullptr;   // side launches of large tiles (default priority, non-blocking)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   ~TilePlan() {
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     for (auto& s : stages)
       for (void* p : s.allocs) (void)hipFree(p);
     if (d_partial) (void)hipFree(d_partial);
@@ -399,27 +407,51 @@ static void finalize_stage(const Space& sp, Stage& S, int program_target) {
   GroupInfo A = group_strings(sp.h_astr, S.npairs, S.pp);
   GroupInfo B = group_strings(sp.h_bstr, S.npairs, S.pp);
   // tiles: each alpha group x runs of whole beta groups, up to the target
-  // size; a beta group that alone exceeds the target gets its own tile
+  // size and maxcols columns; a beta group that alone exceeds the target gets
+  // its own tile
   const int target = tile_target(sp.dim, program_target);
-  std::vector<int4> tiles;
   const int nag = (int)A.goff.size() - 1;
   const int nbg = (int)B.goff.size() - 1;
-  for (int ga = 0; ga < nag; ++ga) {
-    const int R = A.goff[ga + 1] - A.goff[ga];
-    SQ_CHECK(R <= 64, SQ_ERR_ARG, "tile does not fit: alpha hypercube too large");
-    int gb = 0;
-    while (gb < nbg) {
-      const int c0 = B.goff[gb];
-      int c1 = B.goff[gb + 1];
-      ++gb;
-      while (gb < nbg && (int64_t)R * (B.goff[gb + 1] - c0) <= target && B.goff[gb + 1] - c0 <= kTileMaxCols) {
-        c1 = B.goff[gb + 1];
+  auto build_tiles = [&](int maxcols) {
+    std::vector<int4> out;
+    for (int ga = 0; ga < nag; ++ga) {
+      const int R = A.goff[ga + 1] - A.goff[ga];
+      SQ_CHECK(R <= 64, SQ_ERR_ARG, "tile does not fit: alpha hypercube too large");
+      int gb = 0;
+      while (gb < nbg) {
+        const int c0 = B.goff[gb];
+        int c1 = B.goff[gb + 1];
         ++gb;
+        while (gb < nbg && (int64_t)R * (B.goff[gb + 1] - c0) <= target && B.goff[gb + 1] - c0 <= maxcols) {
+          c1 = B.goff[gb + 1];
+          ++gb;
+        }
+        const int64_t E = (int64_t)R * (c1 - c0);
+        SQ_CHECK(E <= kTileMax && c1 - c0 <= kTileMaxCols, SQ_ERR_ARG, "tile does not fit: hypercube groups too large");
+        out.push_back(int4{ga, c0, c1, (int)E});
       }
-      const int64_t E = (int64_t)R * (c1 - c0);
-      SQ_CHECK(E <= kTileMax && c1 - c0 <= kTileMaxCols, SQ_ERR_ARG, "tile does not fit: hypercube groups too large");
-      tiles.push_back(int4{ga, c0, c1, (int)E});
     }
+    return out;
+  };
+  std::vector<int4> tiles = build_tiles(kTileMaxCols);
+  // Occupancy.  Every block of a launch reserves the LDS of the launch's
+  // largest tile.  When the stage has more tiles than the CUs hold at once at
+  // that footprint, the stage runs as two launches side by side: the tiles
+  // above the target (alpha and beta groups that are both large hypercubes,
+  // up to twice the target) in a side launch, the others in the main launch,
+  // rebuilt with at most kDenseCols columns so that their column tables stay
+  // small too (SQ_TILE_SPLIT=0 keeps one launch of the tiles above).
+  bool split = false;
+  {
+    size_t lmax = 0;
+    for (const int4& t : tiles)
+      lmax = std::max(lmax, tile_lds(t.w, t.z - t.y, A.goff[t.x + 1] - A.goff[t.x], S.npairs, 2));
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const int64_t resident = (int64_t)std::max(1, cus) * std::max<int64_t>(1, (int64_t)(160 * 1024) / (int64_t)lmax);
+    const char* senv = getenv("SQ_TILE_SPLIT");
+    split = !(senv && senv[0] == '0') && (int64_t)tiles.size() > resident;
+    if (split) tiles = build_tiles(kDenseCols);
   }
   // largest tiles first: they start in the first dispatch round
   std::stable_sort(tiles.begin(), tiles.end(), [](const int4& a, const int4& b) { return a.w > b.w; });
@@ -488,23 +520,45 @@ static void finalize_stage(const Space& sp, Stage& S, int program_target) {
   d.ctab = upload(S.allocs, ctab);
   S.d_entries = upload(S.allocs, S.entries);
   d.nt = S.ntiles;
-  // Dispatch order.  Workgroups go to the XCDs round robin (slot b on XCD
-  // b % 8), each XCD with its own L2.  Tiles of one alpha group read the same
-  // rows of the state, so whole groups are given to one XCD (groups largest
-  // first, each to the XCD with the fewest elements so far) and run there
-  // back to back; SQ_TILE_XCD=0 keeps the plain largest-first order.
+  S.dev = d;
+  auto lds_of = [&](const TileRec& r, int nvec) {
+    const int W = r.c1 - r.c0;
+    return tile_lds(r.R * W, W, r.R, S.npairs, nvec);
+  };
+  // the tiles of the side launch (split stages only)
+  std::vector<char> big(tiles.size(), 0);
   {
+    int nbig = 0;
+    if (split)
+      for (size_t t = 0; t < tiles.size(); ++t) nbig += (big[t] = tiles[t].w > target) ? 1 : 0;
+    if (nbig == (int)tiles.size()) std::fill(big.begin(), big.end(), 0);
+  }
+  // Dispatch order of one launch.  Workgroups go to the XCDs round robin
+  // (slot b on XCD b % 8), each XCD with its own L2.  Tiles of one alpha
+  // group read the same rows of the state, so whole groups are given to one
+  // XCD (groups largest first, each to the XCD with the fewest elements so
+  // far) and run there back to back; SQ_TILE_XCD=0 keeps the plain
+  // largest-first order.  The slots hold the tile records themselves (empty
+  // slots: R = 0).
+  auto make_launch = [&](bool want_big, StageDev& out, size_t& lds_fwd, size_t& lds_rev) {
     constexpr int kXcds = 8;
+    std::vector<int32_t> mine;
+    for (size_t t = 0; t < tiles.size(); ++t)
+      if ((big[t] != 0) == want_big) mine.push_back((int32_t)t);
+    out = d;
+    out.nb = 0;
+    out.tiles = nullptr;
+    lds_fwd = lds_rev = 0;
+    if (mine.empty()) return;
     std::vector<int32_t> order;
     const char* xenv = getenv("SQ_TILE_XCD");
     if (xenv && xenv[0] == '0') {
-      order.resize(tiles.size());
-      std::iota(order.begin(), order.end(), 0);
+      order = mine;
     } else {
       std::vector<std::vector<int32_t>> of_group(nag);
       std::vector<int64_t> gsize(nag, 0);
-      for (size_t t = 0; t < tiles.size(); ++t) {
-        of_group[tiles[t].x].push_back((int32_t)t);
+      for (int32_t t : mine) {
+        of_group[tiles[t].x].push_back(t);
         gsize[tiles[t].x] += tiles[t].w;
       }
       std::vector<int> gorder(nag);
@@ -524,20 +578,18 @@ static void finalize_stage(const Space& sp, Stage& S, int program_target) {
       for (int x = 0; x < kXcds; ++x)
         for (size_t k = 0; k < bins[x].size(); ++k) order[k * kXcds + x] = bins[x][k];
     }
-    // the slots hold the tile records themselves (no indirection in the kernels)
     std::vector<TileRec> slots(order.size(), TileRec{0, 0, 0, 0, 0u, 0, 0, -1});
     for (size_t b = 0; b < order.size(); ++b)
       if (order[b] >= 0) slots[b] = recs[order[b]];
-    d.nb = (int)slots.size();
-    d.tiles = upload(S.allocs, slots);
-  }
-  S.dev = d;
-  S.lds_fwd = S.lds_rev = 0;
-  for (const TileRec& r : recs) {
-    const int W = r.c1 - r.c0;
-    S.lds_fwd = std::max(S.lds_fwd, tile_lds(r.R * W, W, r.R, S.npairs, 1));
-    S.lds_rev = std::max(S.lds_rev, tile_lds(r.R * W, W, r.R, S.npairs, 2));
-  }
+    out.nb = (int)slots.size();
+    out.tiles = upload(S.allocs, slots);
+    for (int32_t t : mine) {
+      lds_fwd = std::max(lds_fwd, lds_of(recs[t], 1));
+      lds_rev = std::max(lds_rev, lds_of(recs[t], 2));
+    }
+  };
+  make_launch(false, S.dev, S.lds_fwd, S.lds_rev);
+  make_launch(true, S.dev_big, S.lds_fwd_big, S.lds_rev_big);
 }
 
 TilePlan* build_tile_plan(const Space& sp, const Program& P) {
@@ -603,6 +655,7 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
       I.entries = S.d_entries;
       info.push_back(I);
       S.dev.poff = off;
+      S.dev_big.poff = off;
       off += (int64_t)S.ntiles * S.npairs * kQVals;
     }
     T->ntiled = (int)info.size();
@@ -613,9 +666,17 @@ TilePlan* build_tile_plan(const Space& sp, const Program& P) {
     for (auto& S : T->stages) {
       if (!S.tiled) continue;
       S.dev.mats = T->d_mats + (size_t)S.index * kMaxPairs * kMatVals;
+      S.dev_big.mats = S.dev.mats;
     }
     std::vector<void*> tmp;
     T->d_info = upload(tmp, info);
+    bool any_big = false;
+    for (auto& S : T->stages) any_big = any_big || (S.tiled && S.dev_big.nb > 0);
+    if (any_big) {
+      SQ_HIP(hipStreamCreateWithFlags(&T->side, hipStreamNonBlocking));
+      SQ_HIP(hipEventCreateWithFlags(&T->ev_fork, hipEventDisableTiming));
+      SQ_HIP(hipEventCreateWithFlags(&T->ev_join, hipEventDisableTiming));
+    }
   } catch (...) {
     delete T;
     throw;
@@ -1297,24 +1358,56 @@ static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, 
           k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
 }
 
-static void launch_fwd(const Stage& S, int nt, bool dagger, const double* src, double* dst, hipStream_t st) {
+static void launch_fwd1(const StageDev& D, size_t lds, int nt, bool dagger, const double* src, double* dst,
+                        hipStream_t st) {
   if (nt == 256) {
-    allow_lds(k_stage_fwd<256>, S.lds_fwd);
-    k_stage_fwd<256><<<tile_grid(S.dev.nb, 16), 256, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
+    allow_lds(k_stage_fwd<256>, lds);
+    k_stage_fwd<256><<<tile_grid(D.nb, 16), 256, lds, st>>>(D, D.mats, dagger ? 1 : 0, src, dst);
     return;
   }
-  allow_lds(k_stage_fwd<kNT>, S.lds_fwd);
-  k_stage_fwd<kNT><<<tile_grid(S.dev.nb, 8), kNT, S.lds_fwd, st>>>(S.dev, S.dev.mats, dagger ? 1 : 0, src, dst);
+  allow_lds(k_stage_fwd<kNT>, lds);
+  k_stage_fwd<kNT><<<tile_grid(D.nb, 8), kNT, lds, st>>>(D, D.mats, dagger ? 1 : 0, src, dst);
 }
 
-static void launch_rev(const Stage& S, int nt, double* psi, double* lam, double* partial, int wb, hipStream_t st) {
+static void launch_rev1(const StageDev& D, size_t lds, int nt, double* psi, double* lam, double* partial, int wb,
+                        hipStream_t st) {
   if (nt == 256) {
-    allow_lds(k_stage_rev<256>, S.lds_rev);
-    k_stage_rev<256><<<tile_grid(S.dev.nb, 16), 256, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
+    allow_lds(k_stage_rev<256>, lds);
+    k_stage_rev<256><<<tile_grid(D.nb, 16), 256, lds, st>>>(D, D.mats, psi, lam, partial, wb);
     return;
   }
-  allow_lds(k_stage_rev<kNT>, S.lds_rev);
-  k_stage_rev<kNT><<<tile_grid(S.dev.nb, 8), kNT, S.lds_rev, st>>>(S.dev, S.dev.mats, psi, lam, partial, wb);
+  allow_lds(k_stage_rev<kNT>, lds);
+  k_stage_rev<kNT><<<tile_grid(D.nb, 8), kNT, lds, st>>>(D, D.mats, psi, lam, partial, wb);
+}
+
+// A stage's launches: with a large-tile side launch, it runs on the plan's
+// side stream between a fork and a join event, concurrently with the main
+// launch on st (the two write disjoint amplitudes).
+template <class F>
+static void stage_launches(const TilePlan& T, const Stage& S, hipStream_t st, F&& launch) {
+  if (S.dev_big.nb == 0) {
+    launch(S.dev, false, st);
+    return;
+  }
+  SQ_HIP(hipEventRecord(T.ev_fork, st));
+  SQ_HIP(hipStreamWaitEvent(T.side, T.ev_fork, 0));
+  launch(S.dev_big, true, T.side);
+  launch(S.dev, false, st);
+  SQ_HIP(hipEventRecord(T.ev_join, T.side));
+  SQ_HIP(hipStreamWaitEvent(st, T.ev_join, 0));
+}
+
+static void launch_fwd(const TilePlan& T, const Stage& S, bool dagger, const double* src, double* dst, hipStream_t st) {
+  stage_launches(T, S, st, [&](const StageDev& D, bool big, hipStream_t s) {
+    launch_fwd1(D, big ? S.lds_fwd_big : S.lds_fwd, T.nt, dagger, src, dst, s);
+  });
+}
+
+static void launch_rev(const TilePlan& T, const Stage& S, double* psi, double* lam, double* partial, int wb,
+                       hipStream_t st) {
+  stage_launches(T, S, st, [&](const StageDev& D, bool big, hipStream_t s) {
+    launch_rev1(D, big ? S.lds_rev_big : S.lds_rev, T.nt, psi, lam, partial, wb, s);
+  });
 }
 
 // Forward sweep (dagger: U^+, gates reversed with -theta) over the whole
@@ -1342,7 +1435,7 @@ void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const do
       }
       continue;
     }
-    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(S, T.nt, dagger, src, psi, st));
+    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1), 0.0, launch_fwd(T, S, dagger, src, psi, st));
     src = psi;
   }
   if (src != psi) copy_vec(src, psi, s.dim, st);
@@ -1362,7 +1455,7 @@ void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const do
       continue;
     }
     const int wb = i > 0 ? 1 : 0;
-    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(S, T.nt, psi, lam, T.d_partial, wb, st));
+    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1), 0.0, launch_rev(T, S, psi, lam, T.d_partial, wb, st));
   }
   if (T.ntiled > 0)
     SQ_PROF(PC_REDUCE, st, 8.0 * (double)T.partial_len, 0.0,

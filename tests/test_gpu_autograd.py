@@ -115,3 +115,22 @@ def test_14o_tups6_energy_gradient_rdms():
     assert np.abs(np.einsum("pqrr->pq", r2) - 13.0 * r1).max() < 1e-9
     e_rdm = np.einsum("pq,pq->", h, r1) + 0.5 * np.einsum("pqrs,pqrs->", g, r2)
     assert abs(e_rdm - e_a) < 1e-10
+
+
+def test_14o_large_tile_side_launch_matches_single_launch(monkeypatch):
+    """At (14e,14o) the few tiles above the tile target run as a side launch
+    beside the main stage launch; with SQ_TILE_SPLIT=0 every tile runs in one
+    launch.  Both give the same state, energy and gradient (the tiles and
+    their arithmetic are the same; 1e-13)."""
+    from slowquant_amd import engine as E
+
+    lay, h, g, theta, sp, prog, H, psi0 = _problem(14, 6, 28, 0.03, 0)
+    a = prog.state_prep(theta, psi0)  # builds the stage plan with the side launches
+    monkeypatch.setenv("SQ_TILE_SPLIT", "0")
+    one = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    b = one.state_prep(theta, psi0)  # plan built with one launch per stage
+    assert (a - b).abs().max().item() < 1e-13
+    e_s, g_s = prog.energy_grad(H, theta, psi0, "adjoint")
+    e_o, g_o = one.energy_grad(H, theta, psi0, "adjoint")
+    assert abs(e_s - e_o) < 1e-13
+    assert np.abs(g_s - g_o).max() < 1e-13
