@@ -5,7 +5,7 @@
 //
 // T(theta) = sum_k theta_k T_k with every gate generator T_k taken from the
 // program's raw terms; exp(t T) v is applied matrix-free by a scaled Taylor
-// series: s steps of exp(t T / s) with |t| nu / s <= 1/2 and K terms per step
+// series: s steps of exp(t T / s) with |t| nu / s <= 2 and K terms per step
 // until (|t| nu / s)^K / K! < 2^-60, where nu bounds ||T||_2 (a power-iteration
 // estimate with margin, capped by the triangle-inequality bound).
 // The analytic gradient integrates <e^{-sT} lam | T_i | e^{-sT} psi> over
@@ -211,6 +211,63 @@ __global__ __launch_bounds__(64 * kCsrRows) void k_tapply_csr(int64_t dim, const
   }
 }
 
+// Node outputs of one multi-node Taylor series: per vector, kNodes states
+// (the 8 Gauss-Legendre nodes of a subinterval and its end) at out + q * dim,
+// and the weight of the current Taylor term for each.
+constexpr int kNodes = 9;
+struct NodeW {
+  double r[kNodes];
+};
+
+// Two vectors through the same T in one pass over the pattern:
+// nxt_v = f T cur_v (v = 0, 1) and out_v[q][I] += r_q nxt_v[I] for q < kNodes.
+// One wave per row as k_tapply_csr (same summation order); after the wave
+// sum, lanes q and 32 + q (q < kNodes) update node q of vector 0 and 1.
+__global__ __launch_bounds__(64 * kCsrRows) void k_tapply_csr2(
+    int64_t dim, const int64_t* __restrict__ rowp, const uint2* __restrict__ ent, const double* __restrict__ coef,
+    const double* __restrict__ w, const double* __restrict__ cur0, const double* __restrict__ cur1, double f,
+    double* __restrict__ nxt0, double* __restrict__ nxt1, double* __restrict__ out0, double* __restrict__ out1,
+    NodeW nw) {
+  const int lane = threadIdx.x & 63;
+  const int64_t I = (int64_t)blockIdx.x * kCsrRows + (threadIdx.x >> 6);
+  if (I >= dim) return;  // uniform per wave; no block barrier below
+  const int64_t b = rowp[I], e = rowp[I + 1];
+  double v0 = 0.0, v1 = 0.0;
+  for (int64_t k = b + lane; k < e; k += 64) {
+    const uint2 x = ent[k];
+    const uint32_t t = x.y & 0x7fffffffu;
+    const double wc = w[t] * coef[t];
+    const double c0 = wc * cur0[x.x], c1 = wc * cur1[x.x];
+    v0 += (x.y >> 31) ? -c0 : c0;
+    v1 += (x.y >> 31) ? -c1 : c1;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    v0 += __shfl_down(v0, off, 64);
+    v1 += __shfl_down(v1, off, 64);
+  }
+  v0 = f * __shfl(v0, 0, 64);
+  v1 = f * __shfl(v1, 0, 64);
+  if (lane == 0) nxt0[I] = v0;
+  if (lane == 32) nxt1[I] = v1;
+  const int q = lane & 31;
+  if (q < kNodes) {
+    double* o = (lane < 32 ? out0 : out1) + (size_t)q * dim + I;
+    *o += nw.r[q] * (lane < 32 ? v0 : v1);
+  }
+}
+
+// out_v[q][I] = base_v[I] for q < kNodes (the k = 0 term of every node).
+__global__ void k_nodes_init(int64_t dim, const double* __restrict__ b0, const double* __restrict__ b1,
+                             double* __restrict__ out0, double* __restrict__ out1) {
+  const int64_t I = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (I >= dim) return;
+  const double x0 = b0[I], x1 = b1[I];
+  for (int q = 0; q < kNodes; ++q) {
+    out0[(size_t)q * dim + I] = x0;
+    out1[(size_t)q * dim + I] = x1;
+  }
+}
+
 // nxt[I] = f * sum_g part[g][I]; acc[I] += nxt[I] when acc != nullptr.
 __global__ void k_tapply_sum(const double* __restrict__ part, int G, int64_t dim, double f, double* __restrict__ nxt,
                              double* __restrict__ acc) {
@@ -291,6 +348,40 @@ __global__ __launch_bounds__(kUB) void k_gate_dots_csr(const int64_t* __restrict
     double tot = 0.0;
     for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
     gsum[i] += f * tot;
+  }
+}
+
+// Gate dots over the 8 nodes of one subinterval: block (s, i) takes split s
+// of gate i's pattern entries (I, J, t, sign) and writes
+// dpart[s * ng + i] = sum sign coef[t] sum_q wq[q] v_q[I] u_q[J]
+// (v_q = v + q * dim, u_q likewise), the entries strided over the threads and
+// summed in a fixed order; k_gate_accum adds the splits in order.
+__global__ __launch_bounds__(kUB) void k_gate_dots_nodes(const int64_t* __restrict__ gptr,
+                                                         const uint4* __restrict__ gent,
+                                                         const double* __restrict__ coef,
+                                                         const double* __restrict__ u, const double* __restrict__ v,
+                                                         int64_t dim, NodeW wq, int ng, double* __restrict__ dpart) {
+  __shared__ double red[kUB / 64];
+  const int i = blockIdx.x, sp = blockIdx.y, S = gridDim.y;
+  const int64_t g0 = gptr[i], g1 = gptr[i + 1];
+  const int64_t chunk = (g1 - g0 + S - 1) / S;
+  const int64_t b = g0 + sp * chunk, e = min(g1, b + chunk);
+  double val = 0.0;
+  for (int64_t k = b + threadIdx.x; k < e; k += blockDim.x) {
+    const uint4 x = gent[k];
+    double d = 0.0;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) d += wq.r[q] * v[(size_t)q * dim + x.x] * u[(size_t)q * dim + x.y];
+    const double c = coef[x.z & 0x7fffffffu] * d;
+    val += (x.z >> 31) ? -c : c;
+  }
+  for (int off = 32; off > 0; off >>= 1) val += __shfl_down(val, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = val;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) tot += red[w];
+    dpart[(size_t)sp * ng + i] = tot;
   }
 }
 
@@ -500,9 +591,11 @@ static int term_groups(const Space& s) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(64, G));
 }
 
-// Work of the UCC routines (doubles), carved in this order.
+// Work of the UCC routines (doubles), carved in this order; A2, B2 and the
+// node states NC, ND (kNodes vectors each) serve the gradient's
+// multi-node series.
 struct UccWork {
-  double *A, *B, *C, *D, *w, *part, *dpart, *partial, *scal;
+  double *A, *B, *C, *D, *w, *part, *dpart, *partial, *scal, *A2, *B2, *NC, *ND;
 };
 
 static int64_t dot_blocks(const Space& s) { return (s.dim + kUB - 1) / kUB; }
@@ -515,7 +608,7 @@ static int gate_groups(const Space& s, int ng) {
 int64_t ucc_work_doubles(const Space& s, const Program& p) {
   const int64_t ng = (int64_t)std::max<size_t>(p.gates.size(), 1);
   return 4 * s.dim + (int64_t)p.terms.size() + 64 + (int64_t)term_groups(s) * s.dim + dot_blocks(s) * ng + 1024 +
-         64 + 64;
+         64 + 64 + (2 + 2 * kNodes) * s.dim;
 }
 
 static UccWork carve_ucc(const Space& s, const Program& p, double* work) {
@@ -530,6 +623,10 @@ static UccWork carve_ucc(const Space& s, const Program& p, double* work) {
   u.dpart = u.part + (size_t)term_groups(s) * s.dim;
   u.partial = u.dpart + (size_t)dot_blocks(s) * ng;
   u.scal = u.partial + 1024;
+  u.A2 = u.scal + 64;
+  u.B2 = u.A2 + s.dim;
+  u.NC = u.B2 + s.dim;
+  u.ND = u.NC + (size_t)kNodes * s.dim;
   return u;
 }
 
@@ -568,8 +665,21 @@ static double read_slot(const double* scal, int slot, hipStream_t st) {
   return h;
 }
 
+// Terms of a Taylor series of exp(t T) with |t| ||T||_2 <= r: the smallest K
+// with r^K / K! < 2^-60 (at most 60).
+static int taylor_terms(double r) {
+  int K = 1;
+  double term = r;
+  while (term > std::ldexp(1.0, -60) && K < 60) {
+    ++K;
+    term *= r / K;
+  }
+  return K;
+}
+
 // Taylor plan for exp(t T) with ||T||_2 <= nu: s steps of exp(t T / s) with
-// |t| nu / s <= 1/2, K terms per step until (|t| nu / s)^K / K! < 2^-60.
+// |t| nu / s <= 2 and taylor_terms(|t| nu / s) terms per step (a radius of 2
+// costs one bit to cancellation: the largest term is 2).
 struct TaylorPlan {
   int steps, K;
   double a;  // t / steps
@@ -578,14 +688,8 @@ struct TaylorPlan {
 static TaylorPlan taylor_plan(double nu, double t) {
   const double b = std::fabs(t) * nu;
   TaylorPlan P;
-  P.steps = std::max(1, (int)std::ceil(b / 0.5));
-  const double r = b / P.steps;
-  P.K = 1;
-  double term = r;
-  while (term > std::ldexp(1.0, -60) && P.K < 60) {
-    ++P.K;
-    term *= r / P.K;
-  }
+  P.steps = std::max(1, (int)std::ceil(b / 2.0));
+  P.K = taylor_terms(b / P.steps);
   P.a = t / P.steps;
   return P;
 }
@@ -674,8 +778,44 @@ static const double kGLw[8] = {0.05061426814518813, 0.11119051722668724, 0.15685
 // 2 ||T||_2 = omega; the 8-point rule's error on one subinterval is below
 // h (8!)^4 / (17 (16!)^3) (h omega)^16 max|f|, so over [0, 1] it is below
 // 1.7e-23 * 4^16 max|f| < 1e-13 max|f|).
-// psi and lam are propagated from node to node (C, D); every node adds all
-// gates' dots in one pass.
+// Without the gate-major pattern, psi and lam are propagated from node to
+// node (C, D) and every node adds all gates' dots in one pass.
+//
+// With the gate-major pattern, each subinterval [a, a + h] runs one Taylor
+// series of exp(-h T) on psi and lam together (k_tapply_csr2: the pattern read
+// once for both), whose terms z_k = (-h)^k / k! T^k C also give every node:
+// e^{-h x_q T} C = sum_k x_q^k z_k, accumulated into the node states as the
+// terms are made (and the end of the subinterval, x = 1, starts the next);
+// h nu <= 2 bounds the series' radius.  The 8 nodes' gate dots then run in
+// one launch, split over blocks per gate.
+static void nodes_apply(const Space& s, const UccCsr* c, const UccWork& u, double nu, double h, hipStream_t st) {
+  const int K = taylor_terms(h * nu);
+  const unsigned nb = (unsigned)((s.dim + kUB - 1) / kUB);
+  k_nodes_init<<<nb, kUB, 0, st>>>(s.dim, u.C, u.D, u.NC, u.ND);
+  SQ_HIP(hipGetLastError());
+  double x[kNodes], pw[kNodes];
+  for (int q = 0; q < 8; ++q) x[q] = kGLx[q];
+  x[8] = 1.0;
+  for (int q = 0; q < kNodes; ++q) pw[q] = 1.0;
+  const double* cur0 = u.C;
+  const double* cur1 = u.D;
+  double* b0[2] = {u.A, u.B};
+  double* b1[2] = {u.A2, u.B2};
+  const unsigned rows = (unsigned)((s.dim + kCsrRows - 1) / kCsrRows);
+  for (int k = 1; k <= K; ++k) {
+    NodeW r;
+    for (int q = 0; q < kNodes; ++q) r.r[q] = pw[q] *= x[q];
+    double* n0 = b0[k & 1];
+    double* n1 = b1[k & 1];
+    SQ_PROF(PC_TAPPLY, st, 32.0 * (double)s.dim + 8.0 * (double)c->nnz + 16.0 * kNodes * (double)s.dim, 0.0,
+            k_tapply_csr2<<<rows, 64 * kCsrRows, 0, st>>>(s.dim, c->rowp, c->ent, c->coef, u.w, cur0, cur1, -h / k,
+                                                          n0, n1, u.NC, u.ND, r));
+    SQ_HIP(hipGetLastError());
+    cur0 = n0;
+    cur1 = n1;
+  }
+}
+
 static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, double nu, const double* psi,
                           const double* lam, double* grad_dev, hipStream_t st) {
   const int ng = (int)p.gates.size();
@@ -687,6 +827,28 @@ static void ucc_grad_impl(const Space& s, const Program& p, const UccWork& u, do
   const int nb = (int)dot_blocks(s);
   const dim3 dgrid(nb, gate_groups(s, ng));
   const UccCsr* csr = ucc_csr(s, p, st);
+  static const int nodes_env = [] {
+    const char* e = getenv("SQ_UCC_NODES");
+    return (e && e[0] == '0') ? 0 : 1;
+  }();
+  if (csr && csr->gent && nodes_env) {
+    const double h = 1.0 / m;
+    const int S = (int)std::max<int64_t>(1, std::min<int64_t>(dot_blocks(s), std::max(1, 1024 / ng)));
+    NodeW wq{};
+    for (int q = 0; q < 8; ++q) wq.r[q] = 2.0 * kGLw[q] / m;
+    for (int j = 0; j < m; ++j) {
+      nodes_apply(s, csr, u, nu, h, st);
+      k_gate_dots_nodes<<<dim3((unsigned)ng, (unsigned)S), kUB, 0, st>>>(csr->gptr, csr->gent, csr->coef, u.NC,
+                                                                        u.ND, s.dim, wq, ng, u.dpart);
+      k_gate_accum<<<(unsigned)((ng + kUB - 1) / kUB), kUB, 0, st>>>(u.dpart, S, ng, 1.0, grad_dev);
+      SQ_HIP(hipGetLastError());
+      if (j + 1 < m) {
+        copy_vec(u.NC + (size_t)8 * s.dim, u.C, s.dim, st);
+        copy_vec(u.ND + (size_t)8 * s.dim, u.D, s.dim, st);
+      }
+    }
+    return;
+  }
   double prev = 0.0;
   for (int j = 0; j < m; ++j) {
     for (int q = 0; q < 8; ++q) {
