@@ -870,6 +870,7 @@ int sq_operator_destroy(sq_operator* op) {
   return guarded([&] {
     if (!op) return;
     if (op->d_terms) (void)hipFree(op->d_terms);
+    operator_pattern_free(*op);
     delete op;
   });
 }
@@ -880,6 +881,10 @@ int sq_operator_apply(const sq_space* sp, const sq_operator* op, const double* i
     SQ_CHECK(op->space == &sp->s, SQ_ERR_ARG, "operator built for another space");
     const Space& s = sp->s;
     if (s.dim == 0) return;
+    // repeated applications (linear response applies the same operators many
+    // times) go through the operator's sparsity pattern
+    sq_operator& o = const_cast<sq_operator&>(*op);
+    if (++o.applies >= 2 && operator_apply_pattern(s, o, in, out, as_stream(stream))) return;
     k_tapply<<<grid2d(s), kBlk, 0, as_stream(stream)>>>(view(s), op->d_terms, 0, (int)op->terms.size(), in, out,
                                                         nullptr, nullptr, 0, 0, 0, 0, 1.0);
     SQ_HIP(hipGetLastError());

@@ -283,4 +283,18 @@ struct sq_operator {
   const sq::Space* space = nullptr;
   std::vector<sq::Term> terms;
   sq::Term* d_terms = nullptr;
+  // sparsity pattern of the operator on its space, built at the second
+  // application (an operator applied once keeps the term scan)
+  int applies = 0;
+  bool csr_tried = false;
+  struct sq::UccCsr* csr = nullptr;
+  sq::TermFast* d_fast = nullptr;
 };
+
+namespace sq {
+// out = O in through the operator's sparsity pattern; false (nothing
+// launched) when the pattern is not built (first application, too large, or
+// SQ_UCC_CSR=0).
+bool operator_apply_pattern(const Space& s, sq_operator& op, const double* in, double* out, hipStream_t st);
+void operator_pattern_free(sq_operator& op);
+}  // namespace sq

@@ -451,23 +451,30 @@ static bool part_fast(const SpinOps& o, uint32_t& set, uint32_t& clear, uint32_t
   return true;
 }
 
-static const TermFast* term_fast(const Program& p) {
-  if (!p.d_term_fast) {
-    std::vector<TermFast> f(std::max<size_t>(p.terms.size(), 1), TermFast{0, 0, 0, 0, 0, 0, 0u, 0u, 0.0});
-    for (size_t t = 0; t < p.terms.size(); ++t) {
-      TermFast& F = f[t];
-      F.coef = p.terms[t].coef;
-      int ca0 = 0, cb0 = 0;
-      if (part_fast(p.terms[t].a, F.sa, F.ca, F.xa, ca0) && part_fast(p.terms[t].b, F.sb, F.cb, F.xb, cb0))
-        F.flags = 1u | ((uint32_t)((ca0 + cb0) & 1) << 1);
-      else
-        F = TermFast{0, 0, 0, 0, 0, 0, 0u, 0u, p.terms[t].coef};
-    }
-    TermFast* d = nullptr;
-    SQ_HIP(hipMalloc(&d, sizeof(TermFast) * f.size()));
-    SQ_HIP(hipMemcpy(d, f.data(), sizeof(TermFast) * f.size(), hipMemcpyHostToDevice));
-    p.d_term_fast = d;
+// Terms in mask form where their spin parts allow it (uploaded).
+static TermFast* upload_term_fast(const std::vector<Term>& terms) {
+  std::vector<TermFast> f(std::max<size_t>(terms.size(), 1), TermFast{0, 0, 0, 0, 0, 0, 0u, 0u, 0.0});
+  for (size_t t = 0; t < terms.size(); ++t) {
+    TermFast& F = f[t];
+    F.coef = terms[t].coef;
+    int ca0 = 0, cb0 = 0;
+    if (part_fast(terms[t].a, F.sa, F.ca, F.xa, ca0) && part_fast(terms[t].b, F.sb, F.cb, F.xb, cb0))
+      F.flags = 1u | ((uint32_t)((ca0 + cb0) & 1) << 1);
+    else
+      F = TermFast{0, 0, 0, 0, 0, 0, 0u, 0u, terms[t].coef};
   }
+  TermFast* d = nullptr;
+  SQ_HIP(hipMalloc(&d, sizeof(TermFast) * f.size()));
+  const hipError_t e = hipMemcpy(d, f.data(), sizeof(TermFast) * f.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    SQ_HIP(e);
+  }
+  return d;
+}
+
+static const TermFast* term_fast(const Program& p) {
+  if (!p.d_term_fast) p.d_term_fast = upload_term_fast(p.terms);
   return p.d_term_fast;
 }
 
@@ -481,6 +488,7 @@ struct UccCsr {
   // above kGateMaxEntries.
   int64_t* gptr = nullptr;  // [gates + 1]
   uint4* gent = nullptr;    // [nnz]
+  std::vector<int64_t> host_rowp;  // row offsets on the host while the pattern is built
 };
 
 void ucc_csr_free(UccCsr* c) {
@@ -529,24 +537,26 @@ static void build_gate_major(const Space& s, const Program& p, const std::vector
 // Largest pattern kept (entries of 8 bytes); above it, and with SQ_UCC_CSR=0,
 // T is applied by the term-scan kernels below.
 constexpr int64_t kCsrMaxEntries = (int64_t)1 << 29;
+// Largest pattern kept for a compiled operator (256 MB); larger operators
+// (a Hamiltonian over a big space) keep the term scan.
+constexpr int64_t kOpMaxEntries = (int64_t)1 << 25;
 
-// The program's sparsity pattern of T on space s, built on first use
-// (synchronous); nullptr where it is not used.
-static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
-  if (p.ucc_csr_tried) return p.ucc_csr;
-  p.ucc_csr_tried = true;
-  const char* env = getenv("SQ_UCC_CSR");
-  if (env && env[0] == '0') return nullptr;
-  const int nt = (int)p.terms.size();
+// Sparsity pattern of sum_t coef_t O_t (terms on the device, with their mask
+// forms) on space s: per determinant I, the entries (J, t | parity << 31) of
+// the terms whose adjoint maps I into the space (synchronous); nullptr when
+// it is empty, above max_nnz entries, or the one-wave-per-row product would
+// not fit one grid dimension.
+static UccCsr* build_pattern(const Space& s, const std::vector<Term>& terms, const Term* d_terms,
+                             const TermFast* fast, int64_t max_nnz, hipStream_t st) {
+  const int nt = (int)terms.size();
   // k_tapply_csr runs one 64-lane wave per determinant: 64 * dim work-items
   // must stay below the 2^32 limit of one grid dimension
   if (s.dim == 0 || nt == 0 || 64 * s.dim >= ((int64_t)1 << 32)) return nullptr;
-  const TermFast* fast = term_fast(p);
   const unsigned nb = (unsigned)((s.dim + kUB - 1) / kUB);
   int32_t* cnt = nullptr;
   SQ_HIP(hipMalloc(&cnt, sizeof(int32_t) * (size_t)s.dim));
   std::vector<int32_t> hc((size_t)s.dim);
-  k_csr_count<<<nb, kUB, 0, st>>>(sv(s), p.d_terms, fast, nt, cnt);
+  k_csr_count<<<nb, kUB, 0, st>>>(sv(s), d_terms, fast, nt, cnt);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipMemcpyAsync(hc.data(), cnt, sizeof(int32_t) * hc.size(), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -555,18 +565,18 @@ static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
   std::vector<int64_t> rowp((size_t)s.dim + 1, 0);
   for (size_t i = 0; i < hc.size(); ++i) rowp[i + 1] = rowp[i] + hc[i];
   const int64_t nnz = rowp.back();
-  if (nnz == 0 || nnz > kCsrMaxEntries) return nullptr;
+  if (nnz == 0 || nnz > max_nnz) return nullptr;
   UccCsr* c = new UccCsr;
   c->nnz = nnz;
   std::vector<double> coef((size_t)nt);
-  for (int t = 0; t < nt; ++t) coef[t] = p.terms[t].coef;
+  for (int t = 0; t < nt; ++t) coef[t] = terms[t].coef;
   e = hipMalloc(&c->rowp, sizeof(int64_t) * rowp.size());
   if (e == hipSuccess) e = hipMalloc(&c->ent, sizeof(uint2) * (size_t)nnz);
   if (e == hipSuccess) e = hipMalloc(&c->coef, sizeof(double) * coef.size());
   if (e == hipSuccess) e = hipMemcpy(c->rowp, rowp.data(), sizeof(int64_t) * rowp.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(c->coef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
-    k_csr_fill<<<nb, kUB, 0, st>>>(sv(s), p.d_terms, fast, nt, c->rowp, c->ent);
+    k_csr_fill<<<nb, kUB, 0, st>>>(sv(s), d_terms, fast, nt, c->rowp, c->ent);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(st);
@@ -574,14 +584,59 @@ static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
     ucc_csr_free(c);
     SQ_HIP(e);
   }
+  c->host_rowp = std::move(rowp);
+  return c;
+}
+
+static bool csr_enabled() {
+  const char* env = getenv("SQ_UCC_CSR");
+  return !(env && env[0] == '0');
+}
+
+// The program's sparsity pattern of T on space s, built on first use
+// (synchronous); nullptr where it is not used.
+static const UccCsr* ucc_csr(const Space& s, const Program& p, hipStream_t st) {
+  if (p.ucc_csr_tried) return p.ucc_csr;
+  p.ucc_csr_tried = true;
+  if (!csr_enabled()) return nullptr;
+  UccCsr* c = build_pattern(s, p.terms, p.d_terms, term_fast(p), kCsrMaxEntries, st);
+  if (!c) return nullptr;
   try {
-    build_gate_major(s, p, rowp, c);
+    build_gate_major(s, p, c->host_rowp, c);
   } catch (...) {
     ucc_csr_free(c);
     throw;
   }
+  c->host_rowp.clear();
+  c->host_rowp.shrink_to_fit();
   p.ucc_csr = c;
   return c;
+}
+
+bool operator_apply_pattern(const Space& s, sq_operator& op, const double* in, double* out, hipStream_t st) {
+  if (!op.csr_tried) {
+    op.csr_tried = true;
+    if (!csr_enabled()) return false;
+    op.d_fast = upload_term_fast(op.terms);
+    op.csr = build_pattern(s, op.terms, op.d_terms, op.d_fast, kOpMaxEntries, st);
+    if (op.csr) {
+      op.csr->host_rowp.clear();
+      op.csr->host_rowp.shrink_to_fit();
+    }
+  }
+  const UccCsr* c = op.csr;
+  if (!c) return false;
+  SQ_PROF(PC_TAPPLY, st, 16.0 * (double)s.dim + 8.0 * (double)c->nnz, 0.0,
+          k_tapply_csr<<<(unsigned)((s.dim + kCsrRows - 1) / kCsrRows), 64 * kCsrRows, 0, st>>>(
+              s.dim, c->rowp, c->ent, c->coef, nullptr, in, 1.0, out, nullptr));
+  return true;
+}
+
+void operator_pattern_free(sq_operator& op) {
+  if (op.csr) ucc_csr_free(op.csr);
+  op.csr = nullptr;
+  if (op.d_fast) (void)hipFree(op.d_fast);
+  op.d_fast = nullptr;
 }
 
 // Term groups of the split T application: enough (determinant, group)

@@ -40,6 +40,25 @@ def _program(sp: E.Space, layout) -> E.Program:
     return prog
 
 
+_OPERATORS: dict = {}
+
+
+def _operator(sp: E.Space, op: FermionicOperator, do_unsafe: bool) -> E.Operator:
+    """The compiled form of an (already folded) operator, reused across calls:
+    its second application on the device goes through the operator's
+    sparsity pattern instead of the term scan (linear response applies the
+    same operators many times).  Keyed by the operator's strings and
+    coefficients; at most 64 are kept."""
+    key = (id(sp), bool(do_unsafe), tuple(op.operators.items()))
+    O = _OPERATORS.get(key)
+    if O is None:
+        O = E.Operator(sp, op.operators.items(), drop_outside=do_unsafe)
+        if len(_OPERATORS) >= 64:
+            _OPERATORS.pop(next(iter(_OPERATORS)))
+        _OPERATORS[key] = O
+    return O
+
+
 def _ucc_program(sp: E.Space, layout: UccStructure) -> E.Program:
     types = ["sa_single_ucc" if t == "sa_single" else t for t in layout.excitation_operator_type]
 
@@ -146,8 +165,7 @@ def _compile_list(operators, ci: CI_Info, sp: E.Space, thetas, wf_struct, do_fol
                 raise TypeError(f"Got unknown wave function structure type, {type(wf_struct)}")
         else:
             folded = _fold(op, ci, do_folding)
-            O = E.Operator(sp, folded.operators.items(), drop_outside=do_unsafe)
-            steps.append(O.apply)
+            steps.append(_operator(sp, folded, do_unsafe).apply)
     return steps
 
 
@@ -179,8 +197,7 @@ def build_operator_matrix(op: FermionicOperator, ci_info: CI_Info, do_unsafe: bo
     """Dense matrix of an active-space operator (operator_state_algebra.py:413-469),
     built in one launch on the device (sq_operator_matrix)."""
     sp = _space(ci_info)
-    O = E.Operator(sp, op.operators.items(), drop_outside=do_unsafe)
-    return O.matrix().cpu().numpy()
+    return _operator(sp, op, do_unsafe).matrix().cpu().numpy()
 
 
 # ---- state-averaged (leading state axis) forms -------------------------------
