@@ -23,6 +23,11 @@ The JSON line also carries:
   batch512        the 512-instance batched workload (one hipGraph-replayed
                   persistent launch) in instances/s, against a loop of
                   per-instance engine calls
+  ucc8            unfactorised UCCSD (8e,8o): exp(T) psi0, energy and the
+                  quadrature gradient, evals/s
+  operators       the generic operator path at (8e,8o): H as a
+                  FermionicOperator through the term scan and through its
+                  sparsity pattern, beside the sigma kernel (ms each)
   cpu_baseline    the reference algorithm (C port of its Numba scan kernel,
                   Python driver as in the reference) on a bounded sample,
                   extrapolated to one evaluation, on all host cores (value)
@@ -454,6 +459,69 @@ def bench_ucc(args) -> dict:
     }
 
 
+def bench_operators(n: int = 8, reps: int = 20) -> dict:
+    """The generic operator path (propagate_state with FermionicOperators,
+    as linear response calls it) at (ne,no): H = hamiltonian_0i_0a of a
+    synthetic rank-2n h/g as a FermionicOperator, applied to a device state
+    through the term scan (every application re-tests every string per
+    determinant; SQ_UCC_CSR=0) and through the operator's sparsity pattern
+    (built at its second application), beside the engine's sigma kernel on
+    the same state.  Milliseconds per application (host clock, synchronised),
+    and the largest differences between the three."""
+    import torch
+
+    from slowquant_amd import engine as E
+    from slowquant_amd import operators as OP
+    from slowquant_amd.synthetic import synthetic_problem
+
+    h, g, _ = synthetic_problem(n, 2 * n, 1, 0.1, SEED)
+    sp = E.Space(n, n // 2, n // 2)
+    items = list(OP.hamiltonian_0i_0a(h, g, 0, n).operators.items())
+    gen = torch.Generator(device="cpu").manual_seed(SEED)
+    psi = torch.randn(sp.dim, generator=gen, dtype=torch.float64)
+    psi = (psi / psi.norm()).cuda()
+
+    def per_call(fn):
+        fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return 1e3 * (time.perf_counter() - t0) / reps, out
+
+    prev = os.environ.get("SQ_UCC_CSR")
+    os.environ["SQ_UCC_CSR"] = "0"
+    try:
+        O_scan = E.Operator(sp, items)
+        t_scan, v_scan = per_call(lambda: O_scan.apply(psi))
+    finally:
+        if prev is None:
+            os.environ.pop("SQ_UCC_CSR", None)
+        else:
+            os.environ["SQ_UCC_CSR"] = prev
+    O = E.Operator(sp, items)
+    O.apply(psi)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    O.apply(psi)  # second application: builds the pattern
+    torch.cuda.synchronize()
+    t_build = 1e3 * (time.perf_counter() - t0)
+    t_pat, v_pat = per_call(lambda: O.apply(psi))
+    Hd = E.Hamiltonian(sp, h, g)
+    t_sig, v_sig = per_call(lambda: Hd.sigma(psi, False)[0])
+    return {
+        "workload": f"H of a synthetic ({n}e,{n}o) h/g as a FermionicOperator ({len(items)} strings), CI dim {sp.dim}",
+        "unit": "ms per application",
+        "term_scan_ms": t_scan,
+        "pattern_ms": t_pat,
+        "pattern_build_ms": t_build,
+        "sigma_kernel_ms": t_sig,
+        "max_diff_pattern_vs_scan": float((v_pat - v_scan).abs().max()),
+        "max_diff_pattern_vs_sigma": float((v_pat - v_sig).abs().max()),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -461,7 +529,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-steps", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-batch", action="store_true", help="skip the 512-instance batch key of the default line")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the extra workload keys of the default line (batch512, ucc8, operators)")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--workload", choices=("tups12", "batch512", "tups14", "ucc"), default="tups12",
                     help="tups12: the headline (12e,12o) tUPS metric; batch512: 512 batched instances; "
@@ -558,6 +627,11 @@ def main() -> int:
         bb = bench_batch(bargs)
         out["batch512"] = {k: bb[k] for k in ("metric", "value", "unit", "ms_per_step", "steps", "loop",
                                               "batch_over_loop", "max_diff_batch_vs_loop", "config")}
+        # the unfactorised UCC path ((8e,8o) UCCSD) and the generic operator
+        # path, timed in the same run
+        uu = bench_ucc(argparse.Namespace(ucc_orbitals=8, steps=10, warmup=2))
+        out["ucc8"] = {k: uu[k] for k in ("metric", "value", "unit", "ms_per_step", "checks", "roofline", "config")}
+        out["operators"] = bench_operators()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(h, g, th, lay, args.cpu_budget)
     print(json.dumps(out), flush=True)

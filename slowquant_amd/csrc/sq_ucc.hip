@@ -751,9 +751,14 @@ static TaylorPlan taylor_plan(double nu, double t) {
 
 // ||T||_2 for the weights w: T is real antisymmetric (normal), so its
 // 2-norm is its spectral radius.  Power iteration (16 steps from a fixed
-// pseudo-random vector, in A and B) estimates it from below; the returned
-// value is 1.5 x the estimate + 0.05, capped by the triangle-inequality bound
-// sum_t |w_t coef_t| (each raw string has at most one unit entry per column).
+// pseudo-random vector, in A and B) estimates it from below as
+// ||T x_15|| / ||x_15||; the returned value is 1.5 x the estimate + 0.05,
+// capped by the triangle-inequality bound b = sum_t |w_t coef_t| (each raw
+// string has at most one unit entry per column).  The iterates are taken of
+// T / b (norm <= 1), so they need no renormalisation between steps and the
+// 16 products run back to back; two dots and one read-back finish it.  If
+// they underflow (||T|| below 2^-60 b), the steps are rerun with a
+// renormalisation each.
 static double t_norm(const Space& s, const Program& p, const std::vector<double>& th, const UccWork& u,
                      hipStream_t st) {
   double bound = 0.0;
@@ -765,15 +770,32 @@ static double t_norm(const Space& s, const Program& p, const std::vector<double>
   double* y = u.B;
   k_fill_pattern<<<nb, kUB, 0, st>>>(x, s.dim);
   SQ_HIP(hipGetLastError());
-  for (int it = 0; it < 16; ++it) {
-    dot_dev(x, x, s.dim, u.scal, 0, 1.0, u.partial, st);
-    k_inv_sqrt_slot<<<1, 64, 0, st>>>(u.scal, 0, 1);
-    k_scale<<<nb, kUB, 0, st>>>(x, s.dim, u.scal, 1);  // |x| = 1
-    tapply(s, p, u.w, x, 1.0, y, nullptr, u.part, st);  // y = T x
+  for (int it = 0; it < 15; ++it) {
+    tapply(s, p, u.w, x, 1.0 / bound, y, nullptr, u.part, st);  // y = (T / b) x
     std::swap(x, y);
   }
+  tapply(s, p, u.w, x, 1.0 / bound, y, nullptr, u.part, st);
   dot_dev(x, x, s.dim, u.scal, 0, 1.0, u.partial, st);
-  const double est = std::sqrt(std::max(0.0, read_slot(u.scal, 0, st)));
+  dot_dev(y, y, s.dim, u.scal, 1, 1.0, u.partial, st);
+  double d[2];
+  SQ_HIP(hipMemcpyAsync(d, u.scal, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  SQ_HIP(hipStreamSynchronize(st));
+  double est;
+  if (d[0] > 1e-200 && std::isfinite(d[1])) {
+    est = bound * std::sqrt(std::max(0.0, d[1] / d[0]));
+  } else {
+    k_fill_pattern<<<nb, kUB, 0, st>>>(x, s.dim);
+    SQ_HIP(hipGetLastError());
+    for (int it = 0; it < 16; ++it) {
+      dot_dev(x, x, s.dim, u.scal, 0, 1.0, u.partial, st);
+      k_inv_sqrt_slot<<<1, 64, 0, st>>>(u.scal, 0, 1);
+      k_scale<<<nb, kUB, 0, st>>>(x, s.dim, u.scal, 1);  // |x| = 1
+      tapply(s, p, u.w, x, 1.0, y, nullptr, u.part, st);  // y = T x
+      std::swap(x, y);
+    }
+    dot_dev(x, x, s.dim, u.scal, 0, 1.0, u.partial, st);
+    est = std::sqrt(std::max(0.0, read_slot(u.scal, 0, st)));
+  }
   return std::min(bound, 1.5 * est + 0.05);
 }
 
