@@ -253,7 +253,10 @@ typedef struct sq_operator sq_operator;
 int sq_operator_create(const sq_space* s, const sq_op_string* strings, int n_strings, int drop_outside,
                        sq_operator** out);
 int sq_operator_destroy(sq_operator* op);
-/* out = O in (device vectors, internal convention, in != out). */
+/* out = O in (device vectors, internal convention, in != out).  The first
+ * application scans the terms per determinant; the second builds the
+ * operator's sparsity pattern (kept with the operator, up to 2^25 entries),
+ * which serves that and every later application. */
 int sq_operator_apply(const sq_space* s, const sq_operator* op, const double* in, double* out, void* stream);
 /* build_operator_matrix (operator_state_algebra.py:413-469): the dense
  * dim x dim matrix M[I][J] = <I|O|J> (device, row-major) in one launch, in the
