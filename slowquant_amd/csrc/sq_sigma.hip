@@ -492,27 +492,40 @@ void dot_dev(const double* x, const double* y, int64_t n, double* out_dev, int s
 }
 
 // Partial M = D D^T over one chunk of determinants (split over the
-// determinant axis), D generated while it is staged: D[p*n+q][I] =
-// (E_pq psi)[I] = sign * psi[J] summed over the alpha and beta images J of I
-// (full excitation tables, entry x = q*n+p as k_exc writes them), D[n*n][I]
-// = psi[I], rows above it zero, so D never touches memory.  WAVES waves per
-// block; wave w owns the upper-triangle 16x16 output tiles (ti <= tj) w,
-// w + WAVES, ... of the RT x RT grid.  Determinants are staged KS at a time
-// in LDS, transposed (sD[k][row], row stride LD + 1).  Each thread owns
-// NLOAD fixed (row, determinant offset) elements of a slice; their table
-// entries are loaded two slices ahead and their amplitudes one slice ahead,
-// so both overlap the MFMAs of the current slice.  dim < 2^31.
-template <int RT, int WAVES>
+// determinant axis), D generated while it is staged, so D never touches
+// memory.  Rows ("slots") of D, in three blocks of whole 16-row tiles:
+//   L  slots [0, 16 MTL):          E_ab psi for the pairs a < b (lexicographic)
+//   U  slots [16 MTL, 32 MTL):     E_ba psi for the same pairs, in the same order
+//   Dg slots [32 MTL, 32 MTL + 16): E_kk psi (k < n), then psi, then zero rows
+// (padding rows zero), with (E_pq psi)[I] = sign * psi[J] summed over the
+// alpha and beta images J of I (full excitation tables, entry x = q*n+p as
+// k_exc writes them).  Besides M = M^T, M is invariant under transposing
+// both pairs up to 1-RDM terms (real psi, [E_pq, E_rs] = d_qr E_ps -
+// d_ps E_rq):
+//   M[(a,b)][(c,d)] = M[(b,a)][(d,c)] - d_bd g_ca + d_ac g_bd,
+// so the U x U and U x Dg blocks and the lower tile triangle of L x U are
+// skipped: only L x L (upper tiles), L x U (tiles i <= j), L x Dg and Dg x Dg
+// are accumulated, NT = MTL (MTL + 1) + MTL + 1 tiles instead of the
+// RT (RT + 1) / 2 of the plain upper triangle (RT = 2 MTL + 1); the host
+// rebuilds the rest.  WAVES waves per block; wave w owns tiles w, w + WAVES,
+// ....  Determinants are staged KS at a time in LDS, transposed (sD[k][row],
+// row stride LD + 1).  Each thread owns NLOAD fixed (row, determinant
+// offset) elements of a slice; their table entries are loaded two slices
+// ahead and their amplitudes one slice ahead, so both overlap the MFMAs of
+// the current slice.  dim < 2^31.
+template <int MTL, int WAVES>
 __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int nsb, const int32_t* __restrict__ afull,
                                                           const int32_t* __restrict__ bfull,
                                                           const double* __restrict__ psi, int64_t chunk,
                                                           int nchunks, int xcd, double* __restrict__ partial) {
   typedef double double4_t __attribute__((ext_vector_type(4)));
+  constexpr int RT = 2 * MTL + 1;
   constexpr int LD = RT * 16, LDP = LD + 1;
   constexpr int KS = 16;
   constexpr int NTH = WAVES * 64;
   constexpr int NLOAD = (LD * KS + NTH - 1) / NTH;
-  constexpr int NT = RT * (RT + 1) / 2;
+  constexpr int NA = MTL * (MTL + 1) / 2;
+  constexpr int NT = 2 * NA + MTL + 1;
   constexpr int PER = (NT + WAVES - 1) / WAVES;
   __shared__ double sD[KS * LDP];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -520,18 +533,27 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int ns
   int tile_i[PER], tile_j[PER];
 #pragma unroll
   for (int t = 0; t < PER; ++t) {
-    int tile = wave + WAVES * t, ti = 0;
-    while (tile >= RT - ti && ti < RT) {
-      tile -= RT - ti;
-      ++ti;
+    int tile = wave + WAVES * t, ti = 0, tj = 0;
+    if (tile < 2 * NA) {  // L x L, then L x U: upper triangles of MTL x MTL tiles
+      const int blk = tile >= NA ? 1 : 0;
+      int u = tile - blk * NA;
+      while (u >= MTL - ti && ti < MTL) {
+        u -= MTL - ti;
+        ++ti;
+      }
+      tj = ti + u + blk * MTL;
+    } else if (tile < 2 * NA + MTL) {  // L x Dg
+      ti = tile - 2 * NA;
+      tj = 2 * MTL;
+    } else {  // Dg x Dg
+      ti = tj = 2 * MTL;
     }
     tile_i[t] = ti;
-    tile_j[t] = ti + tile;
+    tile_j[t] = tj;
   }
   double4_t acc[PER];
 #pragma unroll
   for (int t = 0; t < PER; ++t) acc[t] = double4_t{0.0, 0.0, 0.0, 0.0};
-  const int nn = n * n;
   const uint32_t dim = (uint32_t)nsa * (uint32_t)nsb;
   // chunk of this block: with xcd = 1, XCD b % 8 (workgroups go to the XCDs
   // round robin) takes a contiguous range of chunks, so the blocks running
@@ -543,6 +565,7 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int ns
   const uint32_t c1 = (uint32_t)min((int64_t)c0 + chunk, (int64_t)dim);
   const double inv_nsb = 1.0 / (double)nsb;
   // owned elements: kind 0 zero row, 1 diagonal E_pp, 2 E_pq (p != q), 3 psi row
+  const int npair = n * (n - 1) / 2;
   int kind[NLOAD], ex[NLOAD];
 #pragma unroll
   for (int u = 0; u < NLOAD; ++u) {
@@ -550,12 +573,29 @@ __global__ __launch_bounds__(WAVES * 64) void k_gram_mfma(int n, int nsa, int ns
     const int r = i / KS;
     kind[u] = 0;
     ex[u] = 0;
-    if (i < LD * KS && r < nn) {
-      const int p = r / n, q = r - p * n;
-      kind[u] = p == q ? 1 : 2;
-      ex[u] = q * n + p;
-    } else if (i < LD * KS && r == nn) {
-      kind[u] = 3;
+    if (i >= LD * KS) continue;
+    if (r < 32 * MTL) {
+      const bool up = r >= 16 * MTL;
+      int pi = up ? r - 16 * MTL : r;
+      if (pi < npair) {
+        int a = 0;
+        while (pi >= n - 1 - a) {  // pair pi = (a, b), a < b, lexicographic
+          pi -= n - 1 - a;
+          ++a;
+        }
+        const int b = a + 1 + pi;
+        const int p = up ? b : a, q = up ? a : b;
+        kind[u] = 2;
+        ex[u] = q * n + p;
+      }
+    } else {
+      const int k = r - 32 * MTL;
+      if (k < n) {
+        kind[u] = 1;
+        ex[u] = k * n + k;
+      } else if (k == n) {
+        kind[u] = 3;
+      }
     }
   }
   int ta[NLOAD], tb[NLOAD], tia[NLOAD], tib[NLOAD];
@@ -662,15 +702,19 @@ __global__ void k_sum_partials(const double* __restrict__ partial, int nchunks, 
   out[i] = v;
 }
 
-// RDMs.  Rows of D: x = p*n + q holds (E_pq psi), row n*n holds psi, the
-// remaining padding rows are zero.  M = D D^T is accumulated by k_gram_mfma
-// (D generated on the fly) over determinant chunks (one partial per chunk,
-// summed in a fixed order), then
-//   rdm1_pq   = M[p*n+q][n*n]                  (= <E_pq psi|psi>)
-//   rdm2_pqrs = M[q*n+p][r*n+s] - delta_qr rdm1_ps   (<psi|E_pq E_rs|psi> - ...)
+// RDMs.  With D laid out in the L | U | Dg slot blocks of k_gram_mfma, the
+// kernel accumulates the needed tiles of M = D D^T over determinant chunks
+// (one partial per chunk, summed in a fixed order); the host reads, for pair
+// indices x = (p, q) (E_pq psi) and the psi row,
+//   G(x, y) = <E_x psi | E_y psi> from a computed tile, directly or through
+//             M = M^T, or else through the pair transposition
+//             G((a,b),(c,d)) = G((b,a),(d,c)) - d_bd g_ca + d_ac g_bd,
+//   rdm1_pq   = g_pq = <psi|E_pq psi>,
+//   rdm2_pqrs = G((q,p),(r,s)) - d_qr rdm1_ps  (<psi|E_pq E_rs|psi> - ...).
+static int rdm_mtl(int n) { return std::max(1, (n * (n - 1) / 2 + 15) / 16); }
+
 int64_t rdm_work_bytes(const Space& s, int* rt_out, int* nchunks_out, int64_t* chunk_out) {
-  const int rows = s.n * s.n + 1;
-  const int rt = (rows + 15) / 16;
+  const int rt = 2 * rdm_mtl(s.n) + 1;
   int64_t chunk = 4096;
   int64_t nchunks = (s.dim + chunk - 1) / chunk;
   while (nchunks > 1024) {
@@ -690,29 +734,32 @@ void rdm_impl(const Space& s, const double* psi, double* rdm1, double* rdm2, dou
   int64_t chunk;
   rdm_work_bytes(s, &rt, &nchunks, &chunk);
   SQ_CHECK(s.dim < ((int64_t)1 << 31), SQ_ERR_ARG, "RDMs need fewer than 2^31 determinants");
+  SQ_CHECK(s.n <= 15, SQ_ERR_ARG, "RDMs support at most 15 orbitals");
   const int n = s.n;
+  const int mtl = rdm_mtl(n);
   const int64_t ld = (int64_t)rt * 16;
   double* partial = work;
   double* M = partial + (size_t)nchunks * ld * ld;
-  // algorithmic traffic: psi and the two excitation-table rows per E_pq; flops: upper-triangle tiles
+  const int nt = mtl * (mtl + 1) + mtl + 1;
+  // algorithmic traffic: psi and the two excitation-table rows per E_pq; flops: the accumulated tiles
   const double gram_bytes = 8.0 * (double)s.dim + 4.0 * (double)(n * n) * (s.nsa + s.nsb);
-  const double gram_flops = 2.0 * 256.0 * (rt * (rt + 1) / 2) * (double)s.dim;
+  const double gram_flops = 2.0 * 256.0 * nt * (double)s.dim;
   const char* xenv = getenv("SQ_GRAM_XCD");
   const int xcd = (xenv && xenv[0] == '0') ? 0 : 1;
   const unsigned gblocks = xcd ? (unsigned)((nchunks + 7) / 8 * 8) : (unsigned)nchunks;
-  switch (rt) {
-#define SQ_R(RT)                                                                                          \
-  case RT:                                                                                                \
-    if (RT >= 6)                                                                                          \
-      SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                    \
-              k_gram_mfma<RT, 16><<<gblocks, 1024, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi, \
-                                                            chunk, nchunks, xcd, partial));               \
-    else                                                                                                  \
-      SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                    \
-              k_gram_mfma<RT, 4><<<gblocks, 256, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, psi,   \
-                                                          chunk, nchunks, xcd, partial));                 \
+  switch (mtl) {
+#define SQ_R(MTL)                                                                                          \
+  case MTL:                                                                                                \
+    if (MTL >= 3)                                                                                          \
+      SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                     \
+              k_gram_mfma<MTL, 16><<<gblocks, 1024, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full, \
+                                                             psi, chunk, nchunks, xcd, partial));          \
+    else                                                                                                   \
+      SQ_PROF(PC_RDM_GRAM, st, gram_bytes, gram_flops,                                                     \
+              k_gram_mfma<MTL, 4><<<gblocks, 256, 0, st>>>(n, s.nsa, s.nsb, s.d_aexc_full, s.d_bexc_full,   \
+                                                           psi, chunk, nchunks, xcd, partial));            \
     break;
-    SQ_R(1) SQ_R(2) SQ_R(3) SQ_R(4) SQ_R(5) SQ_R(6) SQ_R(7) SQ_R(8) SQ_R(9) SQ_R(10) SQ_R(11) SQ_R(12) SQ_R(13)
+    SQ_R(1) SQ_R(2) SQ_R(3) SQ_R(4) SQ_R(5) SQ_R(6) SQ_R(7)
 #undef SQ_R
     default:
       throw Error{SQ_ERR_ARG, "unsupported orbital count for the RDM kernel"};
@@ -724,20 +771,54 @@ void rdm_impl(const Space& s, const double* psi, double* rdm1, double* rdm2, dou
   std::vector<double> hM(len);
   SQ_HIP(hipMemcpyAsync(hM.data(), M, sizeof(double) * len, hipMemcpyDeviceToHost, st));
   SQ_HIP(hipStreamSynchronize(st));
-  const int nn = n * n;
+  // slot of pair (p, q) and of the psi row
+  std::vector<int> pidx((size_t)n * n, -1);
+  for (int a = 0, i = 0; a < n; ++a)
+    for (int b = a + 1; b < n; ++b, ++i) pidx[(size_t)a * n + b] = i;
+  const int psi_slot = 32 * mtl + n;
+  auto slot = [&](int p, int q) {
+    if (p == q) return 32 * mtl + p;
+    return p < q ? pidx[(size_t)p * n + q] : 16 * mtl + pidx[(size_t)q * n + p];
+  };
+  // accumulated tile pairs (t1 <= t2)
+  auto computed = [&](int t1, int t2) {
+    if (t2 < mtl) return true;                                    // L x L
+    if (t1 < mtl && t2 < 2 * mtl) return t1 <= t2 - mtl;         // L x U, upper tiles
+    if (t1 < mtl && t2 == 2 * mtl) return true;                   // L x Dg
+    return t1 == 2 * mtl && t2 == 2 * mtl;                        // Dg x Dg
+  };
+  // M at slots (s1, s2) when its tile pair (in either order) was accumulated
+  auto get = [&](int s1, int s2, double& v) {
+    if (s1 / 16 > s2 / 16) std::swap(s1, s2);
+    if (!computed(s1 / 16, s2 / 16)) return false;
+    v = hM[(size_t)s1 * ld + s2];
+    return true;
+  };
+  // 1-RDM g_pq = <psi|E_pq psi>: psi x L and psi x Dg are accumulated; g is symmetric
+  std::vector<double> g1((size_t)n * n);
   for (int p = 0; p < n; ++p)
-    for (int q = 0; q < n; ++q) {
-      const int x = p * n + q;
-      rdm1[p * n + q] = (x / 16 <= nn / 16) ? hM[(size_t)x * ld + nn] : hM[(size_t)nn * ld + x];
+    for (int q = p; q < n; ++q) {
+      double v = 0.0;
+      SQ_CHECK(get(psi_slot, slot(p, q), v), SQ_ERR_ARG, "RDM tile layout is incomplete");
+      g1[(size_t)p * n + q] = g1[(size_t)q * n + p] = v;
     }
+  for (size_t i = 0; i < g1.size(); ++i) rdm1[i] = g1[i];
+  // G((a,b),(c,d)) = <E_ab psi|E_cd psi>
+  auto G = [&](int a, int b, int c, int d) {
+    double v = 0.0;
+    if (get(slot(a, b), slot(c, d), v)) return v;
+    const bool ok = get(slot(b, a), slot(d, c), v);
+    SQ_CHECK(ok, SQ_ERR_ARG, "RDM tile layout is incomplete");
+    if (b == d) v -= g1[(size_t)c * n + a];
+    if (a == c) v += g1[(size_t)b * n + d];
+    return v;
+  };
   for (int p = 0; p < n; ++p)
     for (int q = 0; q < n; ++q)
       for (int r = 0; r < n; ++r)
         for (int t = 0; t < n; ++t) {
-          const int x = q * n + p, y = r * n + t;
-          // only tiles with row tile <= column tile were accumulated
-          double v = (x / 16 <= y / 16) ? hM[(size_t)x * ld + y] : hM[(size_t)y * ld + x];
-          if (q == r) v -= rdm1[p * n + t];
+          double v = G(q, p, r, t);
+          if (q == r) v -= g1[(size_t)p * n + t];
           rdm2[((size_t)(p * n + q) * n + r) * n + t] = v;
         }
 }
