@@ -54,6 +54,8 @@ def build_library(verbose: bool = False) -> str:
         "-o",
         tmp,
         *srcs,
+        "-L" + os.path.join(os.path.dirname(os.path.dirname(os.path.realpath(hipcc()))), "lib"),
+        "-lrocblas",  # the plain dense GEMM of the alpha-string Hamiltonian (sq_rowsigma.hip)
     ]
     if verbose:
         print(" ".join(cmd), file=sys.stderr)

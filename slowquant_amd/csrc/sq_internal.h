@@ -183,8 +183,10 @@ bool row_sigma_applies(const Space& s);
 int64_t row_sigma_work_doubles(const Space& s);
 int64_t row_sigma_used_doubles(const Space& s);
 void build_alpha_ham(const Space& s, Hamiltonian& H);
+// allow_library = false keeps every step in this library's own kernels (no
+// library call that may allocate).
 int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, double* sigma, double* work,
-                  double** partial_out, hipStream_t st);
+                  double** partial_out, hipStream_t st, bool allow_library = true);
 
 // Kernel timing classes (sq_profile.hip).
 enum ProfClass {

@@ -61,7 +61,7 @@ extern "C" int sq_sigma_noalloc(const sq_space* sp, const sq_hamiltonian* H, con
     double* w = scratch_for_current_device();
     hipStream_t st = as_stream(stream);
     double* partial = nullptr;
-    const int64_t nb = row_sigma(s, H->h, psi, sigma, w, &partial, st);
+    const int64_t nb = row_sigma(s, H->h, psi, sigma, w, &partial, st, false);
     if (energy) reduce_impl(partial, nb, energy, 0, false, 1.0, st);
   });
 }
