@@ -301,8 +301,16 @@ def test_fucc_sa_doubles_parity():
         assert np.abs(grad - g_ref).max() < 1e-10, method
 
 
-@pytest.mark.parametrize("n,na,nb", [(4, 2, 2), (6, 3, 3), (5, 2, 3), (8, 4, 4), (9, 4, 4), (10, 5, 4)])
+@pytest.mark.parametrize(
+    "n,na,nb",
+    [(4, 2, 2), (6, 3, 3), (5, 2, 3), (8, 4, 4), (9, 4, 4), (10, 5, 4), (11, 2, 3), (12, 2, 2), (13, 1, 2), (14, 2, 1)],
+)
 def test_rdm_parity(n, na, nb):
+    """RDMs against the oracle's element-by-element expectation values.  The
+    Gram kernel's tile layout depends on the orbital count only (pairs
+    p < q, their transposes, the diagonal), so few electrons keep the oracle
+    fast at 11-14 orbitals, where the layout has 4-6 tiles per block and the
+    transposition symmetry rebuilds most of the matrix."""
     from oracle.ci import get_indexing
     from oracle.rdm import rdm1, rdm2
 
