@@ -16,6 +16,7 @@
 #include <cmath>
 #include <map>
 #include <numeric>
+#include <string>
 
 #include "sq_internal.h"
 
@@ -318,6 +319,57 @@ __global__ __launch_bounds__(kBT) void k_batch(BatchDev B) {
   }
 }
 
+// Large instances of one (space, gate layout) run together through the
+// single-instance kernels with an instance grid dimension (tile stages, row
+// sigma, gradient): one persistent workgroup per instance would leave them
+// bound by one CU each.  Per-instance buffers are contiguous at fixed
+// strides.
+struct BatchGroup {
+  const Space* space = nullptr;
+  const Program* prog = nullptr;
+  int n = 0, ng = 0;
+  int64_t dim = 0, A_stride = 0, Ma_stride = 0, mats_stride = 0, tpart_stride = 0, spart_stride = 0;
+  double hf = 1.0;  // the HF determinant's amplitude in the engine's sign convention
+  int32_t *d_inst = nullptr, *d_theta0 = nullptr, *d_out0 = nullptr;  // per member
+  double *theta = nullptr, *A = nullptr, *Ma = nullptr, *psi = nullptr, *lam = nullptr, *mats = nullptr,
+         *tpart = nullptr, *spart = nullptr, *grad = nullptr;
+};
+
+// theta_g[m][i] = theta_all[theta0[m] + i] (member m = blockIdx.y)
+__global__ void k_group_theta(const double* __restrict__ theta_all, const int32_t* __restrict__ theta0, int ng,
+                              double* __restrict__ theta_g) {
+  const int m = blockIdx.y;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ng; i += gridDim.x * blockDim.x)
+    theta_g[(size_t)m * ng + i] = theta_all[theta0[m] + i];
+}
+
+// psi_g[m] = the member's start state, or the HF determinant (amplitude hf at 0)
+__global__ void k_group_init(const double* const* __restrict__ psi0, const int32_t* __restrict__ inst, int64_t dim,
+                             double hf, double* __restrict__ psi) {
+  const int m = blockIdx.y;
+  const double* src = psi0[inst[m]];
+  double* dst = psi + (size_t)m * dim;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < dim; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = src ? src[i] : (i == 0 ? hf : 0.0);
+}
+
+// energy[inst[m]] = sum of member m's sigma energy partials (fixed order);
+// grad[out0[m] + g] = grad_g[m][g]
+__global__ void k_group_out(const double* __restrict__ spart, int64_t nblk, const double* __restrict__ grad_g, int ng,
+                            const int32_t* __restrict__ inst, const int32_t* __restrict__ out0,
+                            double* __restrict__ energy, double* __restrict__ grad) {
+  __shared__ double red[4];
+  const int m = blockIdx.x;
+  const double* p = spart + (size_t)m * nblk;
+  double v = 0.0;
+  for (int64_t i = threadIdx.x; i < nblk; i += blockDim.x) v += p[i];
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) energy[inst[m]] = red[0] + red[1] + red[2] + red[3];
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) grad[out0[m] + g] = grad_g[(size_t)m * ng + g];
+}
+
 struct Batch {
   std::vector<BatchInstance> inst;
   int nwg = 0;
@@ -328,6 +380,8 @@ struct Batch {
   BatchDev dev{};
   double* d_theta = nullptr;
   const double** d_psi0 = nullptr;  // start-state pointer table [inst]
+  std::vector<BatchGroup> groups;   // instances run through the tile and row-sigma kernels
+  std::vector<std::pair<int, int>> group_members;  // (group, instance), in instance order
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int32_t> grad_off;
@@ -353,6 +407,124 @@ static T* bupload(Batch& B, const std::vector<T>& v) {
   if (!v.empty()) SQ_HIP(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
   B.allocs.push_back(d);
   return d;
+}
+
+
+// Instances this large go to groups (the persistent launch keeps the rest):
+// at (10e,10o) one workgroup per instance runs a 3-layer evaluation in
+// 14.5 ms against 0.25 ms through the single-instance kernels.
+constexpr int64_t kGroupMinDim = 16384;
+
+// Bytes of a program's gate layout (kinds, pairs, scales and generator
+// terms): equal keys on one space give identical tile plans.
+static std::string layout_key(const Program& p) {
+  std::string k;
+  auto put = [&](const void* x, size_t n) { k.append(reinterpret_cast<const char*>(x), n); };
+  for (size_t g = 0; g < p.gates.size(); ++g) {
+    const GateDev& gd = p.gates[g];
+    put(&gd.kind, sizeof gd.kind);
+    put(&gd.scale, sizeof gd.scale);
+    const int32_t pp = g < p.pair_p.size() ? p.pair_p[g] : -1;
+    put(&pp, sizeof pp);
+    const int32_t nt = gd.term_end - gd.term_begin;
+    put(&nt, sizeof nt);
+    for (int t = gd.term_begin; t < gd.term_end; ++t) {
+      const Term& T = p.terms[t];
+      put(&T.coef, sizeof T.coef);
+      for (const SpinOps* o : {&T.a, &T.b}) {
+        put(&o->n, sizeof o->n);
+        put(o->orb, (size_t)o->n);
+        put(o->cre, (size_t)o->n);
+      }
+    }
+  }
+  return k;
+}
+
+template <class T>
+static T* balloc(Batch& B, size_t n) {
+  T* d = nullptr;
+  SQ_HIP(hipMalloc(&d, sizeof(T) * std::max<size_t>(n, 1)));
+  B.allocs.push_back(d);
+  return d;
+}
+
+// Buffers of every group: members' theta offsets, output offsets, their
+// Hamiltonians copied into contiguous A and Ma arrays, and the work vectors.
+static void build_groups(Batch& B, const sq_hamiltonian* const* hams, const sq_space* const* spaces) {
+  (void)spaces;
+  std::vector<std::vector<int>> members(B.groups.size());
+  for (const auto& gm : B.group_members) members[gm.first].push_back(gm.second);
+  for (size_t gi = 0; gi < B.groups.size(); ++gi) {
+    BatchGroup& G = B.groups[gi];
+    const Space& s = *G.space;
+    const std::vector<int>& mem = members[gi];
+    G.n = (int)mem.size();
+    SQ_CHECK(G.n <= 65535, SQ_ERR_ARG, "too many instances in one batched group");
+    G.dim = s.dim;
+    const Hamiltonian& H0 = hams[mem[0]]->h;
+    const int K4 = (s.npair + 1 + 3) / 4 * 4;
+    G.A_stride = (int64_t)K4 * H0.kpad;
+    G.Ma_stride = (int64_t)s.nsa * s.nsa;
+    G.mats_stride = tiles_mats_doubles(s, *G.prog);
+    G.tpart_stride = tiles_partial_doubles(s, *G.prog);
+    G.spart_stride = row_sigma_group_partials(s);
+    SQ_CHECK(G.mats_stride > 0 && G.tpart_stride > 0 && G.spart_stride > 0, SQ_ERR_ARG, "group path unavailable");
+    {
+      const uint32_t a = s.h_astr[0], b = s.h_bstr[0];
+      int par = 0;
+      for (uint32_t aa = a; aa; aa &= aa - 1) par += __builtin_popcount(b & ((1u << __builtin_ctz(aa)) - 1u));
+      G.hf = (par & 1) ? -1.0 : 1.0;
+    }
+    std::vector<int32_t> inst(mem.begin(), mem.end()), th0, out0;
+    for (int i : mem) {
+      th0.push_back(B.inst[i].theta0);
+      out0.push_back(B.inst[i].out0);
+    }
+    G.d_inst = bupload(B, inst);
+    G.d_theta0 = bupload(B, th0);
+    G.d_out0 = bupload(B, out0);
+    const size_t n = (size_t)G.n;
+    G.theta = balloc<double>(B, n * G.ng);
+    G.A = balloc<double>(B, n * G.A_stride);
+    G.Ma = balloc<double>(B, n * G.Ma_stride);
+    G.psi = balloc<double>(B, n * G.dim);
+    G.lam = balloc<double>(B, n * G.dim);
+    G.mats = balloc<double>(B, n * G.mats_stride);
+    G.tpart = balloc<double>(B, n * G.tpart_stride);
+    G.spart = balloc<double>(B, n * G.spart_stride);
+    G.grad = balloc<double>(B, n * G.ng);
+    for (size_t m = 0; m < n; ++m) {
+      const Hamiltonian& H = hams[mem[m]]->h;
+      SQ_CHECK(H.kpad == H0.kpad && H.d_Ma, SQ_ERR_ARG, "group Hamiltonians differ in shape");
+      SQ_HIP(hipMemcpy(G.A + m * G.A_stride, H.d_A, sizeof(double) * G.A_stride, hipMemcpyDeviceToDevice));
+      SQ_HIP(hipMemcpy(G.Ma + m * G.Ma_stride, H.d_Ma, sizeof(double) * G.Ma_stride, hipMemcpyDeviceToDevice));
+    }
+  }
+}
+
+// One group's evaluation on st: thetas and start states in, forward sweep,
+// sigma and energies, reverse sweep with gradients, results out.
+static void group_launch(const Batch& B, const BatchGroup& G, hipStream_t st) {
+  const Space& s = *G.space;
+  Multi M;
+  M.n = G.n;
+  M.theta = G.ng;
+  M.mats = G.mats_stride;
+  M.vec = G.dim;
+  M.part = G.tpart_stride;
+  M.grad = G.ng;
+  k_group_theta<<<dim3((unsigned)((G.ng + 255) / 256), (unsigned)G.n), 256, 0, st>>>(B.dev.theta, G.d_theta0, G.ng,
+                                                                                     G.theta);
+  const unsigned ib = (unsigned)std::min<int64_t>((G.dim + 255) / 256, 64);
+  k_group_init<<<dim3(ib, (unsigned)G.n), 256, 0, st>>>(B.dev.psi0, G.d_inst, G.dim, G.hf, G.psi);
+  SQ_HIP(hipGetLastError());
+  tiles_forward_group(s, *G.prog, G.theta, G.psi, G.mats, M, st);
+  row_sigma_group(s, G.A, G.A_stride, G.Ma, G.Ma_stride, G.psi, G.lam, G.spart, G.n, st);
+  tiles_reverse_group(s, *G.prog, G.theta, G.psi, G.lam, G.mats, G.tpart, G.grad, M, st);
+  k_group_out<<<(unsigned)G.n, 256, 0, st>>>(G.spart, G.spart_stride, G.grad, G.ng, G.d_inst, G.d_out0,
+                                             B.dev.energy, B.dev.grad);
+  SQ_HIP(hipGetLastError());
 }
 
 }  // namespace sq
@@ -387,7 +559,11 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       std::vector<BatchSpace> bsp;
       std::vector<BatchRot> rots;
       std::vector<double> Astore;
-      std::vector<int64_t> cost(n_inst);
+      std::vector<int64_t> cost(n_inst, 0);
+      std::vector<char> grouped(n_inst, 0);
+      std::map<std::pair<const Space*, std::string>, int> group_of;  // (space, gate layout) -> group
+      const char* genv = getenv("SQ_BATCH_GROUPS");
+      const bool groups_on = !(genv && genv[0] == '0');
       int theta_off = 0, grad_off = 0;
       for (int i = 0; i < n_inst; ++i) {
         const Space& s = spaces[i]->s;
@@ -395,6 +571,33 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
         const Hamiltonian& H = hams[i]->h;
         SQ_CHECK(p.space == &s && H.space == &s, SQ_ERR_ARG, "program / Hamiltonian built for another space");
         SQ_CHECK(s.dim < (int64_t)1 << 30, SQ_ERR_ARG, "batched spaces must have fewer than 2^30 determinants");
+        if (groups_on && s.dim >= kGroupMinDim && H.d_Ma && row_sigma_group_ok(s) && tiles_group_ok(s, p)) {
+          // a large instance: grouped with the instances of the same space and gate layout
+          for (const GateDev& gd : p.gates)
+            SQ_CHECK(gd.kind == GC_ROT || gd.kind == GC_SA_SINGLE, SQ_ERR_ARG,
+                     "batched launches support rotation gates (single, double, sa_single, pair doubles) only");
+          const auto key = std::make_pair(&s, layout_key(p));
+          auto it = group_of.find(key);
+          if (it == group_of.end()) {
+            it = group_of.emplace(key, (int)B.groups.size()).first;
+            BatchGroup G;
+            G.space = &s;
+            G.prog = &p;
+            G.ng = (int)p.gates.size();
+            B.groups.push_back(G);
+          }
+          grouped[i] = 1;
+          BatchInstance I{};
+          I.theta0 = theta_off;
+          I.nparam = (int)p.gates.size();
+          theta_off += I.nparam;
+          I.out0 = grad_off;
+          B.grad_off.push_back(grad_off);
+          grad_off += I.nparam;
+          B.inst.push_back(I);
+          B.group_members.push_back({it->second, i});
+          continue;
+        }
         if (!sp_index.count(spaces[i])) {
           sp_index[spaces[i]] = (int)bsp.size();
           BatchSpace b{};
@@ -457,10 +660,14 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
         SQ_CHECK(bs.kpad <= 16 * kBatchMT && bs.npair < 91, SQ_ERR_ARG, "batched spaces limited to 12 orbitals");
       B.ngrad = grad_off;
       // longest processing time first, each to the least-loaded workgroup
+      // (the instances of the persistent launch: those not in a group)
+      int n_small = 0;
+      for (int i = 0; i < n_inst; ++i) n_small += grouped[i] ? 0 : 1;
       int nwg = n_workgroups > 0 ? n_workgroups : 256;
-      nwg = std::min(nwg, n_inst);
-      std::vector<int> order(n_inst);
-      std::iota(order.begin(), order.end(), 0);
+      nwg = std::min(nwg, n_small);
+      std::vector<int> order;
+      for (int i = 0; i < n_inst; ++i)
+        if (!grouped[i]) order.push_back(i);
       std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
       std::vector<std::vector<int32_t>> wg_inst(nwg);
       std::vector<int64_t> load(nwg, 0);
@@ -485,7 +692,7 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       B.dev.slot_doubles = B.slot;
       B.dev.dmax = B.dmax;
       double* scratch = nullptr;
-      SQ_HIP(hipMalloc(&scratch, sizeof(double) * (size_t)B.slot * nwg));
+      SQ_HIP(hipMalloc(&scratch, sizeof(double) * std::max<size_t>((size_t)B.slot * nwg, 1)));
       B.allocs.push_back(scratch);
       B.dev.scratch = scratch;
       double* en = nullptr;
@@ -501,6 +708,7 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       B.dev.theta = B.d_theta;
       B.d_psi0 = bupload(B, std::vector<const double*>(n_inst, nullptr));
       B.dev.psi0 = B.d_psi0;
+      build_groups(B, hams, spaces);
     } catch (...) {
       delete bo;
       throw;
@@ -550,7 +758,8 @@ static BatchKernel batch_kernel(const Batch& B) {
 }
 
 static void batch_launch(Batch& B, hipStream_t st) {
-  SQ_PROF(PC_BATCH, st, B.bytes, B.flops, batch_kernel(B)<<<B.nwg, kBT, B.lds, st>>>(B.dev));
+  if (B.nwg > 0) SQ_PROF(PC_BATCH, st, B.bytes, B.flops, batch_kernel(B)<<<B.nwg, kBT, B.lds, st>>>(B.dev));
+  for (const BatchGroup& G : B.groups) group_launch(B, G, st);
 }
 
 /* Kernel shape: rotation pairs per thread in flight (1, 2 or 4) and sigma
@@ -577,14 +786,33 @@ int sq_batch_run_graph(sq_batch* bo, void* stream) {
     Batch& B = bo->b;
     hipStream_t st = as_stream(stream);
     if (!B.exec) {
+      // the groups' kernels take their shared-memory attributes on an eager
+      // launch first (not a stream operation, so kept out of the capture)
+      if (!B.groups.empty()) {
+        batch_launch(B, st);
+        SQ_HIP(hipStreamSynchronize(st));
+      }
       hipStream_t cap;
       SQ_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      const bool prof_was = prof_pause();  // event records would become graph nodes
       SQ_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
-      batch_kernel(B)<<<B.nwg, kBT, B.lds, cap>>>(B.dev);
-      hipError_t e = hipGetLastError();
+      hipError_t e = hipSuccess;
+      try {
+        if (B.nwg > 0) batch_kernel(B)<<<B.nwg, kBT, B.lds, cap>>>(B.dev);
+        e = hipGetLastError();
+        for (const BatchGroup& G : B.groups) group_launch(B, G, cap);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(cap, &g);
+        if (g) (void)hipGraphDestroy(g);
+        (void)hipStreamDestroy(cap);
+        prof_resume(prof_was);
+        throw;
+      }
       hipGraph_t g = nullptr;
       hipError_t e2 = hipStreamEndCapture(cap, &g);
       (void)hipStreamDestroy(cap);
+      prof_resume(prof_was);
       SQ_CHECK(e == hipSuccess && e2 == hipSuccess, SQ_ERR_HIP, "graph capture failed");
       B.graph = g;
       SQ_HIP(hipGraphInstantiate(&B.exec, B.graph, nullptr, nullptr, 0));

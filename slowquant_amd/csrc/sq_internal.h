@@ -210,6 +210,8 @@ enum ProfClass {
   kProfClasses
 };
 bool prof_on();
+bool prof_pause();
+void prof_resume(bool was);
 void prof_begin(int cls, hipStream_t st);
 void prof_end(int cls, hipStream_t st, double bytes, double flops);
 
@@ -249,6 +251,32 @@ void reverse_step_impl(const Space& s, const Program& p, int g, const double* th
 
 // Implemented in sq_tiles.hip
 const TilePlan* tiles_for(const Space& s, const Program& p);  // nullptr when no tiled stage applies
+
+// Several instances of one (space, program) evaluated together (batched
+// groups, sq_batch.hip): the tile, gradient and row-sigma kernels run with
+// blockIdx.y = instance and add y * stride to each per-instance buffer;
+// n = 1 with zero strides is one instance.
+struct Multi {
+  int n = 1;
+  int64_t theta = 0, mats = 0, vec = 0, part = 0, grad = 0;
+};
+// Group path of a program: every stage tiled, no side launches.
+bool tiles_group_ok(const Space& s, const Program& p);
+int64_t tiles_mats_doubles(const Space& s, const Program& p);     // pair matrices per instance
+int64_t tiles_partial_doubles(const Space& s, const Program& p);  // gradient partials per instance
+// psi (in place, start states already there) = U(theta) psi for every instance
+void tiles_forward_group(const Space& s, const Program& p, const double* theta, double* psi, double* mats,
+                         const Multi& M, hipStream_t st);
+// reverse sweep on (psi, lam) with dE/dtheta into grad (n x gates)
+void tiles_reverse_group(const Space& s, const Program& p, const double* theta, double* psi, double* lam,
+                         const double* mats, double* partial, double* grad, const Multi& M, hipStream_t st);
+// sigma = H psi for n instances of one space (row form with its alpha GEMM
+// tiles in the launch); A and Ma per instance at the given strides; energy
+// partials per instance (row_sigma_group_partials of them) at partial.
+bool row_sigma_group_ok(const Space& s);
+int64_t row_sigma_group_partials(const Space& s);
+void row_sigma_group(const Space& s, const double* A, int64_t A_stride, const double* Ma, int64_t Ma_stride,
+                     const double* psi, double* sigma, double* partial, int n, hipStream_t st);
 void tiles_forward(const Space& s, const Program& p, const TilePlan& T, const double* theta, const double* psi_in,
                    double* psi, double* scratch, double* gwork, int deriv_gate, bool dagger, hipStream_t st);
 void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const double* theta, double* psi, double* lam,

@@ -47,6 +47,17 @@ const char* kClassNames[kProfClasses] = {"rotate", "rotate_adjoint", "t_apply", 
 
 bool prof_on() { return prof().on; }
 
+// Profiling off for a stretch of launches (e.g. while a graph is captured:
+// its event records would become graph nodes); returns the previous state.
+bool prof_pause() {
+  Prof& P = prof();
+  const bool was = P.on;
+  P.on = false;
+  return was;
+}
+
+void prof_resume(bool was) { prof().on = was; }
+
 void prof_begin(int cls, hipStream_t st) {
   Prof& P = prof();
   if (!P.on || !((P.mask >> cls) & 1u)) return;

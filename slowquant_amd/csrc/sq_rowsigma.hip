@@ -514,10 +514,17 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT 
     const double* __restrict__ At, int kpad, int K4, int K8, int mgroups, int npair, const double* __restrict__ psi,
     const double* __restrict__ sin, int nsplit, double* __restrict__ sigma, double* __restrict__ partial,
     const int2* __restrict__ rta, const int2* __restrict__ rtb, int nsa, int nsb, int R, int xcd, int nrowblk,
-    const double* __restrict__ Ma, int gkchunk) {
+    const double* __restrict__ Ma, int gkchunk, int64_t A_stride, int64_t Ma_stride, int64_t vec_stride,
+    int64_t part_stride) {
   constexpr int W = MT * 16;
   extern __shared__ double smem[];
   __shared__ double red[kRowWaves];
+  // instance blockIdx.y of a group (zero strides: one instance)
+  At += blockIdx.y * A_stride;
+  psi += blockIdx.y * vec_stride;
+  sigma += blockIdx.y * vec_stride;
+  partial += blockIdx.y * part_stride;
+  if (FUSED) Ma += blockIdx.y * Ma_stride;
   if (FUSED && (int)blockIdx.x >= nrowblk) {
     const int g = (int)blockIdx.x - nrowblk;
     const int tn = (nsb + kGemmCols - 1) / kGemmCols, tm = (nsa + kGemmRows - 1) / kGemmRows;
@@ -875,7 +882,7 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
     SQ_PROF(PC_SIGMA_ROWS, st, bytes, fl,                                                                       \
             k_sigma_rows_scatter<MT, true><<<nblk, kRowThreads, pl.lds, st>>>(                                  \
                 H.d_A, H.kpad, K4, K8, pl.mgroups, s.npair, psi, nullptr, 0, sigma, fpart, s.d_rta, s.d_rtb,    \
-                s.nsa, s.nsb, pl.R, xcd, pl.ngroups, H.d_Ma, pl.kchunk));                                       \
+                s.nsa, s.nsb, pl.R, xcd, pl.ngroups, H.d_Ma, pl.kchunk, 0, 0, 0, 0));                           \
     break;
       SQ_RSF(1) SQ_RSF(2) SQ_RSF(3) SQ_RSF(4) SQ_RSF(5) SQ_RSF(6) SQ_RSF(7) SQ_RSF(8) SQ_RSF(9)
 #undef SQ_RSF
@@ -893,7 +900,7 @@ int64_t row_sigma(const Space& s, const Hamiltonian& H, const double* psi, doubl
     SQ_PROF(PC_SIGMA_ROWS, st, bytes, flops,                                                                    \
             k_sigma_rows_scatter<MT, false><<<pl.ngroups, kRowThreads, pl.lds, st>>>(                           \
                 H.d_A, H.kpad, K4, K8, pl.mgroups, s.npair, psi, sinp, nsplit, sigma, partial, s.d_rta,         \
-                s.d_rtb, s.nsa, s.nsb, pl.R, xcd, pl.ngroups, nullptr, 0));                                     \
+                s.d_rtb, s.nsa, s.nsb, pl.R, xcd, pl.ngroups, nullptr, 0, 0, 0, 0, 0));                         \
     break;
       SQ_RSS(1) SQ_RSS(2) SQ_RSS(3) SQ_RSS(4) SQ_RSS(5) SQ_RSS(6) SQ_RSS(7) SQ_RSS(8) SQ_RSS(9)
 #undef SQ_RSS
@@ -926,6 +933,48 @@ int64_t row_sigma_used_doubles(const Space& s) {
   if (!row_plan(s, &pl)) return -1;
   if (pl.fused) return (int64_t)pl.ngroups + pl.ngemm;
   return (int64_t)pl.nsplit * s.dim + pl.ngroups;
+}
+
+bool row_sigma_group_ok(const Space& s) {
+  RowPlan pl;
+  return rows_enabled() && s.d_rtb != nullptr && row_plan(s, &pl) && pl.scatter && pl.fused;
+}
+
+int64_t row_sigma_group_partials(const Space& s) {
+  RowPlan pl;
+  if (!row_plan(s, &pl) || !pl.fused) return 0;
+  return (int64_t)pl.ngroups + pl.ngemm;
+}
+
+void row_sigma_group(const Space& s, const double* A, int64_t A_stride, const double* Ma, int64_t Ma_stride,
+                     const double* psi, double* sigma, double* partial, int n, hipStream_t st) {
+  RowPlan pl;
+  SQ_CHECK(row_sigma_group_ok(s) && row_plan(s, &pl) && n > 0 && n <= 65535, SQ_ERR_ARG,
+           "the grouped row sigma does not apply to this space");
+  static const int xcd = env_int("SQ_ROWS_XCD", 1) != 0 ? 1 : 0;
+  const int K8 = row_k8(s);
+  const int K4 = rup(s.npair + 1, 4);
+  const int nblk = pl.ngroups + pl.ngemm;
+  const int64_t total = (int64_t)n * s.dim;
+  k_zero<<<(unsigned)std::min<int64_t>((total + 255) / 256, 16384), 256, 0, st>>>(sigma, total);
+  SQ_HIP(hipGetLastError());
+  const double fl = (2.0 * (double)s.npair * (s.npair + 1) * (double)s.dim + 2.0 * (double)s.nsa * s.nsa * s.nsb) * n;
+  const double bytes = 8.0 * (double)s.dim * 3 * n;
+  const int kpad = rup(s.npair + 1, 16);
+  switch (pl.mt) {
+#define SQ_RSG(MT)                                                                                              \
+  case MT:                                                                                                      \
+    allow_lds(k_sigma_rows_scatter<MT, true>, pl.lds);                                                          \
+    SQ_PROF(PC_SIGMA_ROWS, st, bytes, fl,                                                                       \
+            (k_sigma_rows_scatter<MT, true><<<dim3(nblk, n), kRowThreads, pl.lds, st>>>(                        \
+                A, kpad, K4, K8, pl.mgroups, s.npair, psi, nullptr, 0, sigma, partial, s.d_rta, s.d_rtb, s.nsa, \
+                s.nsb, pl.R, xcd, pl.ngroups, Ma, pl.kchunk, A_stride, Ma_stride, s.dim, nblk)));               \
+    break;
+    SQ_RSG(1) SQ_RSG(2) SQ_RSG(3) SQ_RSG(4) SQ_RSG(5) SQ_RSG(6) SQ_RSG(7) SQ_RSG(8) SQ_RSG(9)
+#undef SQ_RSG
+    default:
+      throw Error{SQ_ERR_ARG, "unsupported orbital count for the row sigma"};
+  }
 }
 
 }  // namespace sq

@@ -811,8 +811,10 @@ __device__ __forceinline__ void stage_entries(const StageEntry* __restrict__ ent
 // One thread per (stage, pair, class): the pair's gate product in forward
 // order, with T_g inserted after gate g when deriv_gate is on this pair.
 __global__ __launch_bounds__(64) void k_stage_mats(const StageInfo* __restrict__ info, const double* __restrict__ theta, int deriv_gate,
-                             double* __restrict__ mats) {
+                             double* __restrict__ mats, int64_t th_stride, int64_t mats_stride) {
   __shared__ EntryLds E;
+  theta += blockIdx.y * th_stride;
+  mats += blockIdx.y * mats_stride;
   const StageInfo* __restrict__ I = info + blockIdx.x;
   stage_entries(I->entries, 0, I->nentries, theta, E);
   __syncthreads();
@@ -859,7 +861,11 @@ constexpr int kQa[6] = {0, 0, 0, 1, 1, 2};
 constexpr int kQb[6] = {1, 2, 3, 2, 3, 3};
 
 __global__ __launch_bounds__(kTB) void k_stage_grad(const StageInfo* __restrict__ info, const double* __restrict__ theta,
-                             const double* __restrict__ partial, double* __restrict__ grad) {
+                             const double* __restrict__ partial, double* __restrict__ grad, int64_t th_stride,
+                             int64_t part_stride, int64_t grad_stride) {
+  theta += blockIdx.y * th_stride;
+  partial += blockIdx.y * part_stride;
+  grad += blockIdx.y * grad_stride;
   __shared__ double Q[kQVals];
   __shared__ double dots[kMaxPairEntries * 5];
   __shared__ double red[kQVals * (kTB / 64)];
@@ -1177,8 +1183,12 @@ __device__ __forceinline__ double row_sum(double v) {
 // sweep's input vector (first stage).
 template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_fwd(StageDev S, const double* __restrict__ mats, int dagger,
-                                                  const double* src, double* dst) {
+                                                  const double* src, double* dst, int64_t mats_stride,
+                                                  int64_t vec_stride) {
   extern __shared__ double smem[];
+  mats += blockIdx.y * mats_stride;
+  src += blockIdx.y * vec_stride;
+  dst += blockIdx.y * vec_stride;
   // block b walks slots b, b + grid, ... (tiles are sorted largest first)
   for (int slot = blockIdx.x; slot < S.nb; slot += gridDim.x) {
     const TileRec T = S.tiles[slot];
@@ -1219,7 +1229,12 @@ __device__ __forceinline__ double rev_orbit2(double* X, double* L, int m0, int d
 template <int NT>
 __global__ __launch_bounds__(NT) void k_stage_rev(StageDev S, const double* __restrict__ mats,
                                                   double* __restrict__ psi, double* __restrict__ lam,
-                                                  double* __restrict__ partial, int write_back) {
+                                                  double* __restrict__ partial, int write_back, int64_t mats_stride,
+                                                  int64_t vec_stride, int64_t part_stride) {
+  mats += blockIdx.y * mats_stride;
+  psi += blockIdx.y * vec_stride;
+  lam += blockIdx.y * vec_stride;
+  partial += blockIdx.y * part_stride;
   constexpr int kWaves = NT / 64;
   extern __shared__ double smem[];
   __shared__ double red[kMaxPairs][8][kWaves];
@@ -1355,29 +1370,29 @@ static unsigned tile_grid(int nslots, int blocks_per_cu) {
 
 static void launch_mats(const TilePlan& T, const double* theta, int deriv_gate, hipStream_t st) {
   SQ_PROF(PC_OTHER, st, 0.0, 0.0,
-          k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats));
+          k_stage_mats<<<T.ntiled, 64, 0, st>>>(T.d_info, theta, deriv_gate, T.d_mats, 0, 0));
 }
 
 static void launch_fwd1(const StageDev& D, size_t lds, int nt, bool dagger, const double* src, double* dst,
                         hipStream_t st) {
   if (nt == 256) {
     allow_lds(k_stage_fwd<256>, lds);
-    k_stage_fwd<256><<<tile_grid(D.nb, 16), 256, lds, st>>>(D, D.mats, dagger ? 1 : 0, src, dst);
+    k_stage_fwd<256><<<tile_grid(D.nb, 16), 256, lds, st>>>(D, D.mats, dagger ? 1 : 0, src, dst, 0, 0);
     return;
   }
   allow_lds(k_stage_fwd<kNT>, lds);
-  k_stage_fwd<kNT><<<tile_grid(D.nb, 8), kNT, lds, st>>>(D, D.mats, dagger ? 1 : 0, src, dst);
+  k_stage_fwd<kNT><<<tile_grid(D.nb, 8), kNT, lds, st>>>(D, D.mats, dagger ? 1 : 0, src, dst, 0, 0);
 }
 
 static void launch_rev1(const StageDev& D, size_t lds, int nt, double* psi, double* lam, double* partial, int wb,
                         hipStream_t st) {
   if (nt == 256) {
     allow_lds(k_stage_rev<256>, lds);
-    k_stage_rev<256><<<tile_grid(D.nb, 16), 256, lds, st>>>(D, D.mats, psi, lam, partial, wb);
+    k_stage_rev<256><<<tile_grid(D.nb, 16), 256, lds, st>>>(D, D.mats, psi, lam, partial, wb, 0, 0, 0);
     return;
   }
   allow_lds(k_stage_rev<kNT>, lds);
-  k_stage_rev<kNT><<<tile_grid(D.nb, 8), kNT, lds, st>>>(D, D.mats, psi, lam, partial, wb);
+  k_stage_rev<kNT><<<tile_grid(D.nb, 8), kNT, lds, st>>>(D, D.mats, psi, lam, partial, wb, 0, 0, 0);
 }
 
 // A stage's launches: with a large-tile side launch, it runs on the plan's
@@ -1459,7 +1474,84 @@ void tiles_reverse(const Space& s, const Program& p, const TilePlan& T, const do
   }
   if (T.ntiled > 0)
     SQ_PROF(PC_REDUCE, st, 8.0 * (double)T.partial_len, 0.0,
-            k_stage_grad<<<T.ntiled * kMaxPairs, kTB, 0, st>>>(T.d_info, theta, T.d_partial, grad_dev));
+            k_stage_grad<<<T.ntiled * kMaxPairs, kTB, 0, st>>>(T.d_info, theta, T.d_partial, grad_dev, 0, 0, 0));
+}
+
+// ---- groups of instances of one program ------------------------------------
+
+bool tiles_group_ok(const Space& s, const Program& p) {
+  const TilePlan* T = tiles_for(s, p);
+  if (!T || T->ntiled == 0) return false;
+  for (const Stage& S : T->stages)
+    if (!S.tiled || S.dev_big.nb != 0) return false;
+  return true;
+}
+
+int64_t tiles_mats_doubles(const Space& s, const Program& p) {
+  const TilePlan* T = tiles_for(s, p);
+  return T ? (int64_t)T->ntiled * kMaxPairs * kMatVals : 0;
+}
+
+int64_t tiles_partial_doubles(const Space& s, const Program& p) {
+  const TilePlan* T = tiles_for(s, p);
+  return T ? std::max<int64_t>(T->partial_len, 1) : 0;
+}
+
+// The launches of one stage over M.n instances (grid y), each stage's pair
+// matrices at mats + stage index * kMaxPairs * kMatVals of every instance.
+static void launch_fwd_group(const TilePlan& T, const Stage& S, const double* mats, double* psi, const Multi& M,
+                             hipStream_t st) {
+  const StageDev& D = S.dev;
+  const double* sm = mats + (size_t)S.index * kMaxPairs * kMatVals;
+  if (T.nt == 256) {
+    allow_lds(k_stage_fwd<256>, S.lds_fwd);
+    k_stage_fwd<256><<<dim3(tile_grid(D.nb, 16), M.n), 256, S.lds_fwd, st>>>(D, sm, 0, psi, psi, M.mats, M.vec);
+  } else {
+    allow_lds(k_stage_fwd<kNT>, S.lds_fwd);
+    k_stage_fwd<kNT><<<dim3(tile_grid(D.nb, 8), M.n), kNT, S.lds_fwd, st>>>(D, sm, 0, psi, psi, M.mats, M.vec);
+  }
+  SQ_HIP(hipGetLastError());
+}
+
+static void launch_rev_group(const TilePlan& T, const Stage& S, const double* mats, double* psi, double* lam,
+                             double* partial, int wb, const Multi& M, hipStream_t st) {
+  const StageDev& D = S.dev;
+  const double* sm = mats + (size_t)S.index * kMaxPairs * kMatVals;
+  if (T.nt == 256) {
+    allow_lds(k_stage_rev<256>, S.lds_rev);
+    k_stage_rev<256><<<dim3(tile_grid(D.nb, 16), M.n), 256, S.lds_rev, st>>>(D, sm, psi, lam, partial, wb, M.mats,
+                                                                            M.vec, M.part);
+  } else {
+    allow_lds(k_stage_rev<kNT>, S.lds_rev);
+    k_stage_rev<kNT><<<dim3(tile_grid(D.nb, 8), M.n), kNT, S.lds_rev, st>>>(D, sm, psi, lam, partial, wb, M.mats,
+                                                                          M.vec, M.part);
+  }
+  SQ_HIP(hipGetLastError());
+}
+
+void tiles_forward_group(const Space& s, const Program& p, const double* theta, double* psi, double* mats,
+                         const Multi& M, hipStream_t st) {
+  const TilePlan* T = tiles_for(s, p);
+  SQ_CHECK(T && tiles_group_ok(s, p) && M.n > 0 && M.n <= 65535, SQ_ERR_ARG, "program has no group tile path");
+  SQ_PROF(PC_OTHER, st, 0.0, 0.0,
+          k_stage_mats<<<dim3(T->ntiled, M.n), 64, 0, st>>>(T->d_info, theta, -1, mats, M.theta, M.mats));
+  for (const Stage& S : T->stages)
+    SQ_PROF(PC_TILE_FWD, st, stage_bytes(s, 1) * M.n, 0.0, launch_fwd_group(*T, S, mats, psi, M, st));
+}
+
+void tiles_reverse_group(const Space& s, const Program& p, const double* theta, double* psi, double* lam,
+                         const double* mats, double* partial, double* grad, const Multi& M, hipStream_t st) {
+  const TilePlan* T = tiles_for(s, p);
+  SQ_CHECK(T && tiles_group_ok(s, p) && M.n > 0 && M.n <= 65535, SQ_ERR_ARG, "program has no group tile path");
+  for (int i = (int)T->stages.size() - 1; i >= 0; --i) {
+    const Stage& S = T->stages[i];
+    const int wb = i > 0 ? 1 : 0;
+    SQ_PROF(PC_TILE_REV, st, stage_bytes(s, wb ? 2 : 1) * M.n, 0.0,
+            launch_rev_group(*T, S, mats, psi, lam, partial, wb, M, st));
+  }
+  SQ_PROF(PC_REDUCE, st, 8.0 * (double)T->partial_len * M.n, 0.0,
+          k_stage_grad<<<dim3(T->ntiled * kMaxPairs, M.n), kTB, 0, st>>>(T->d_info, theta, partial, grad, M.theta,
+                                                                        M.part, M.grad));
 }
 
 }  // namespace sq

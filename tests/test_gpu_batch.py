@@ -207,8 +207,11 @@ def test_batch512_workload():
     inst, thetas = build_instances(specs)
     b = E.autotune_batch(inst, np.concatenate(thetas))
     e, g = b.run(thetas, graph=True)
-    e2, g2 = b.run(thetas, graph=True)  # a second replay gives the same numbers
-    _close(e2, g2, e, g, 0.0 + 1e-14)
+    # a second replay gives the same numbers to rounding: the grouped (10e,10o)
+    # instances run the row sigma, whose LDS-atomic sums may add in another
+    # order (a replay that reused stale or cleared memory would be off by far more)
+    e2, g2 = b.run(thetas, graph=True)
+    _close(e2, g2, e, g, 1e-12)
     e_loop, g_loop = loop_energy_grad(inst, thetas)
     _close(e, g, e_loop, g_loop)
     print(f"batch512: workgroups {b.n_workgroups}, shape {b.shape}", flush=True)
