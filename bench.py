@@ -298,7 +298,7 @@ def bench_batch(args) -> dict:
         "config": {"workload": f"{len(inst)} tUPS instances, spaces uniform over (4e,4o),(6e,6o),(8e,8o),(10e,10o), "
                                "1-3 layers, rank-2n synthetic h/g, theta~N(0,0.1^2), one hipGraph-replayed launch",
                    "mean_ci_dim": float(np.mean([s.dim for s, _, _ in inst])),
-                   "workgroups": b.n_workgroups, "rotation_pairs_in_flight": b.shape[0],
+                   "workgroups": b.n_workgroups, "group_tile_shape": b.tile_shape, "rotation_pairs_in_flight": b.shape[0],
                    "gemm_column_tiles_per_wave": b.shape[1]},
         "loop": {"value": len(inst) / t_loop, "unit": "instances/s", "ms_per_step": 1e3 * t_loop},
         "batch_over_loop": t_loop / t_batch,

@@ -207,7 +207,14 @@ int sq_ucc_energy_grad(const sq_space* s, const sq_program* p, const sq_hamilton
  * persistent launch computes every instance's energy and adjoint gradient
  * from the HF determinant.  Programs may hold rotation gates only (single,
  * double, sa_single, pair doubles: tUPS / QNP / fUCC-SD layouts); spaces up
- * to 12 orbitals.  n_workgroups <= 0 picks 256. */
+ * to 12 orbitals.  n_workgroups <= 0 picks 256.  Instances of 16384 or more
+ * determinants whose programs are all brick stages run instead in groups
+ * of equal (space, gate layout) through the tile-stage, row-sigma and
+ * gradient kernels with one grid row per instance (SQ_BATCH_GROUPS=0 keeps
+ * them in the persistent launch).  Such a batch uses the programs' tile
+ * plans at every launch: the programs must outlive it, and changing a
+ * grouped program's tile shape afterwards makes its launches fail with
+ * SQ_ERR_ARG (create the batch again). */
 typedef struct sq_batch sq_batch;
 int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program* const* programs,
                     const sq_hamiltonian* const* hams, int n_workgroups, sq_batch** out);

@@ -152,7 +152,12 @@ def tune_batch(instances, theta, use_cache: bool = True, **kw) -> E.Batch:
     if use_cache:
         hit = _load().get(key)
         if isinstance(hit, dict):
+            ts = hit.get("tile_shape")
+            if ts:
+                for p in E.grouped_programs(instances):
+                    p.set_tile_shape(*ts)
             b = E.Batch(instances, int(hit["workgroups"]))
+            b.tile_shape = tuple(ts) if ts else None
             b.set_theta(theta)
             b.configure(*hit["shape"])
             b.tuned_ms = hit.get("ms")
@@ -161,5 +166,6 @@ def tune_batch(instances, theta, use_cache: bool = True, **kw) -> E.Batch:
     b = E.autotune_batch(instances, theta, **kw)
     b.cached = False
     if use_cache:
-        _store(key, {"workgroups": b.n_workgroups, "shape": list(b.shape), "ms": b.tuned_ms})
+        _store(key, {"workgroups": b.n_workgroups, "shape": list(b.shape), "ms": b.tuned_ms,
+                     "tile_shape": list(b.tile_shape) if b.tile_shape else None})
     return b

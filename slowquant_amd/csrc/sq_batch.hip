@@ -330,6 +330,7 @@ struct BatchGroup {
   int n = 0, ng = 0;
   int64_t dim = 0, A_stride = 0, Ma_stride = 0, mats_stride = 0, tpart_stride = 0, spart_stride = 0;
   double hf = 1.0;  // the HF determinant's amplitude in the engine's sign convention
+  int plan_gen = 0;  // the program's tile-plan generation the buffers are sized for
   int32_t *d_inst = nullptr, *d_theta0 = nullptr, *d_out0 = nullptr;  // per member
   double *theta = nullptr, *A = nullptr, *Ma = nullptr, *psi = nullptr, *lam = nullptr, *mats = nullptr,
          *tpart = nullptr, *spart = nullptr, *grad = nullptr;
@@ -385,6 +386,8 @@ struct Batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<int32_t> grad_off;
+  hipStream_t side = nullptr;  // the persistent launch beside the groups (default priority)
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int rot_pairs = 2;    // rotation pairs per thread in flight (1, 2, 4)
   int gemm_tiles = 1;   // sigma GEMM column tiles per wave (1, 2)
   void drop_graph() {
@@ -396,6 +399,9 @@ struct Batch {
   ~Batch() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (side) (void)hipStreamDestroy(side);
     for (void* p : allocs) (void)hipFree(p);
   }
 };
@@ -466,6 +472,7 @@ static void build_groups(Batch& B, const sq_hamiltonian* const* hams, const sq_s
     const int K4 = (s.npair + 1 + 3) / 4 * 4;
     G.A_stride = (int64_t)K4 * H0.kpad;
     G.Ma_stride = (int64_t)s.nsa * s.nsa;
+    G.plan_gen = G.prog->plan_gen;
     G.mats_stride = tiles_mats_doubles(s, *G.prog);
     G.tpart_stride = tiles_partial_doubles(s, *G.prog);
     G.spart_stride = row_sigma_group_partials(s);
@@ -503,10 +510,21 @@ static void build_groups(Batch& B, const sq_hamiltonian* const* hams, const sq_s
   }
 }
 
+// A grouped program whose tile shape changed after the batch was created has
+// a new tile plan: the group's buffers (and a captured graph) are sized for
+// the old one.
+static void check_groups(const Batch& B) {
+  for (const BatchGroup& G : B.groups)
+    SQ_CHECK(G.prog->plan_gen == G.plan_gen, SQ_ERR_ARG,
+             "a batched program's tile shape changed after the batch was created; create the batch again");
+}
+
 // One group's evaluation on st: thetas and start states in, forward sweep,
 // sigma and energies, reverse sweep with gradients, results out.
 static void group_launch(const Batch& B, const BatchGroup& G, hipStream_t st) {
   const Space& s = *G.space;
+  SQ_CHECK(G.prog->plan_gen == G.plan_gen, SQ_ERR_ARG,
+           "a batched program's tile shape changed after the batch was created; create the batch again");
   Multi M;
   M.n = G.n;
   M.theta = G.ng;
@@ -709,6 +727,11 @@ int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program*
       B.d_psi0 = bupload(B, std::vector<const double*>(n_inst, nullptr));
       B.dev.psi0 = B.d_psi0;
       build_groups(B, hams, spaces);
+      if (!B.groups.empty() && n_small > 0) {
+        SQ_HIP(hipStreamCreateWithFlags(&B.side, hipStreamNonBlocking));
+        SQ_HIP(hipEventCreateWithFlags(&B.ev_fork, hipEventDisableTiming));
+        SQ_HIP(hipEventCreateWithFlags(&B.ev_join, hipEventDisableTiming));
+      }
     } catch (...) {
       delete bo;
       throw;
@@ -757,7 +780,19 @@ static BatchKernel batch_kernel(const Batch& B) {
   return table[u][B.gemm_tiles - 1];
 }
 
+// The persistent launch and the groups write disjoint outputs; with both,
+// the persistent launch runs on the batch's side stream between a fork and
+// a join event, beside the groups' kernels on st.
 static void batch_launch(Batch& B, hipStream_t st) {
+  if (B.nwg > 0 && !B.groups.empty()) {
+    SQ_HIP(hipEventRecord(B.ev_fork, st));
+    SQ_HIP(hipStreamWaitEvent(B.side, B.ev_fork, 0));
+    SQ_PROF(PC_BATCH, B.side, B.bytes, B.flops, batch_kernel(B)<<<B.nwg, kBT, B.lds, B.side>>>(B.dev));
+    for (const BatchGroup& G : B.groups) group_launch(B, G, st);
+    SQ_HIP(hipEventRecord(B.ev_join, B.side));
+    SQ_HIP(hipStreamWaitEvent(st, B.ev_join, 0));
+    return;
+  }
   if (B.nwg > 0) SQ_PROF(PC_BATCH, st, B.bytes, B.flops, batch_kernel(B)<<<B.nwg, kBT, B.lds, st>>>(B.dev));
   for (const BatchGroup& G : B.groups) group_launch(B, G, st);
 }
@@ -777,7 +812,10 @@ int sq_batch_configure(sq_batch* bo, int rot_pairs, int gemm_tiles) {
 }
 
 int sq_batch_run(sq_batch* bo, void* stream) {
-  return guarded([&] { batch_launch(bo->b, as_stream(stream)); });
+  return guarded([&] {
+    check_groups(bo->b);
+    batch_launch(bo->b, as_stream(stream));
+  });
 }
 
 /* Capture the launch in a hipGraph (once) and replay it. */
@@ -785,6 +823,7 @@ int sq_batch_run_graph(sq_batch* bo, void* stream) {
   return guarded([&] {
     Batch& B = bo->b;
     hipStream_t st = as_stream(stream);
+    check_groups(B);
     if (!B.exec) {
       // the groups' kernels take their shared-memory attributes on an eager
       // launch first (not a stream operation, so kept out of the capture)
@@ -798,9 +837,8 @@ int sq_batch_run_graph(sq_batch* bo, void* stream) {
       SQ_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
       hipError_t e = hipSuccess;
       try {
-        if (B.nwg > 0) batch_kernel(B)<<<B.nwg, kBT, B.lds, cap>>>(B.dev);
+        batch_launch(B, cap);
         e = hipGetLastError();
-        for (const BatchGroup& G : B.groups) group_launch(B, G, cap);
       } catch (...) {
         hipGraph_t g = nullptr;
         (void)hipStreamEndCapture(cap, &g);

@@ -827,6 +827,7 @@ int sq_program_set_tile_shape(sq_program* pr, int target, int threads) {
     if (pr->p.tiles) destroy_tile_plan(pr->p.tiles);
     pr->p.tiles = nullptr;
     pr->p.tiles_built = false;
+    ++pr->p.plan_gen;
     pr->p.tile_target = target;
     pr->p.tile_threads = threads;
   });

@@ -140,6 +140,7 @@ struct Program {
   mutable bool tiles_built = false;
   int tile_target = 0;   // elements per tile (0: automatic; sq_program_set_tile_shape)
   int tile_threads = 0;  // threads per tile block, 256 or 512 (0: 512)
+  int plan_gen = 0;      // bumped whenever the tile plan is dropped (batched groups check it)
   std::vector<GateDev> gates;
   std::vector<Term> terms;
   GateDev* d_gates = nullptr;

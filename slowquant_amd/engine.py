@@ -351,6 +351,7 @@ class Batch:
         self._h = b
         self.n_workgroups = int(n_workgroups)
         self.shape = (2, 1)  # the library's default kernel shape
+        self.tile_shape = None  # the grouped programs' tile shape when an autotuner chose it
         self.n_params = int(N.lib().sq_batch_num_params(b))
         self.offsets = np.cumsum([0] + [p.n_gates for _, p, _ in self.instances])
 
@@ -419,11 +420,62 @@ class Batch:
 SHAPES = tuple((u, ct) for u in (1, 2, 4) for ct in (1, 2))
 
 
-def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), shapes=SHAPES, reps: int = 3) -> Batch:
-    """Build one Batch per workgroup count in ``candidates`` (capped at the
-    number of instances); for each, time ``reps`` graph replays of every
-    kernel shape in ``shapes`` ((rotation pairs in flight, GEMM column tiles
-    per wave)) and return the fastest batch, configured with its best shape."""
+# Instances of this many determinants or more run in the library's batched
+# groups (sq_batch.hip, kGroupMinDim): through the tile-stage kernels, whose
+# tile shape (elements per tile, threads per block) is the program's.
+BATCH_GROUP_MIN_DIM = 16384
+GROUP_TILE_SHAPES = ((0, 0), (1024, 256), (2048, 256), (4096, 512))
+
+
+def _graph_ms(b: "Batch", reps: int) -> float:
+    b.launch(graph=True)
+    torch.cuda.synchronize()
+    t0 = torch.cuda.Event(enable_timing=True)
+    t1 = torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        b.launch(graph=True)
+    t1.record()
+    t1.synchronize()
+    return t0.elapsed_time(t1)
+
+
+def grouped_programs(instances) -> list:
+    """The distinct programs of the instances the batch runs in groups."""
+    out, seen = [], set()
+    for s, p, _ in instances:
+        if s.dim >= BATCH_GROUP_MIN_DIM and id(p) not in seen:
+            seen.add(id(p))
+            out.append(p)
+    return out
+
+
+def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), shapes=SHAPES, reps: int = 3,
+                   tile_shapes=GROUP_TILE_SHAPES) -> Batch:
+    """First, when some instances run in groups, time graph replays of a
+    batch per tile shape in ``tile_shapes`` (set on the grouped programs)
+    and keep the fastest.  Then build one Batch per workgroup count in
+    ``candidates`` (capped at the number of instances); for each, time
+    ``reps`` graph replays of every kernel shape in ``shapes`` ((rotation
+    pairs in flight, GEMM column tiles per wave)) and return the fastest
+    batch, configured with its best shape (its ``tile_shape`` attribute holds
+    the grouped programs' tile shape, None without groups)."""
+    big = grouped_programs(instances)
+    best_ts = None
+    if big:
+        best_tt = float("inf")
+        for ts in tile_shapes:
+            for p in big:
+                p.set_tile_shape(*ts)
+            b = Batch(instances, 256)
+            b.set_theta(theta)
+            b.configure(4, 1)
+            t = _graph_ms(b, reps)
+            del b
+            if t < best_tt:
+                best_tt, best_ts = t, tuple(ts)
+        for p in big:
+            p.set_tile_shape(*best_ts)
     best, best_t, best_shape = None, float("inf"), None
     seen = set()
     for c in candidates:
@@ -436,20 +488,12 @@ def autotune_batch(instances, theta, candidates=(64, 128, 256, 512, 1024), shape
         b_best_t, b_shape = float("inf"), None
         for shape in shapes:
             b.configure(*shape)
-            b.launch(graph=True)
-            torch.cuda.synchronize()
-            t0 = torch.cuda.Event(enable_timing=True)
-            t1 = torch.cuda.Event(enable_timing=True)
-            t0.record()
-            for _ in range(reps):
-                b.launch(graph=True)
-            t1.record()
-            t1.synchronize()
-            t = t0.elapsed_time(t1)
+            t = _graph_ms(b, reps)
             if t < b_best_t:
                 b_best_t, b_shape = t, shape
         if b_best_t < best_t:
             best, best_t, best_shape = b, b_best_t, b_shape
     best.configure(*best_shape)
     best.tuned_ms = best_t / reps
+    best.tile_shape = best_ts
     return best

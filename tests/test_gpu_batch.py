@@ -226,3 +226,39 @@ def test_batch512_workload():
         psi = construct_ups_state(csf, ci, theta, layout)
         assert abs(e[k] - expectation_value(psi, [Hf], psi, ci, do_folding=False)) < 1e-10
         assert np.abs(g[k] - gradient_reference(Hf, csf, ci, theta, layout)).max() < 1e-10
+
+
+def test_batch_groups_match_persistent_launch(monkeypatch):
+    """(10e,10o) instances run in groups (tile stages, row sigma and gradient
+    kernels with one grid row per instance; three gate layouts, so three
+    groups) beside a (6e,6o) instance in the persistent launch: energies and
+    gradients equal the all-persistent batch (SQ_BATCH_GROUPS=0) and the
+    per-instance loop to 1e-10, across graph replays; after a grouped
+    program's tile shape changes, the batch's launches raise ValueError
+    instead of using buffers sized for the old tile plan."""
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import InstanceSpec, build_instances, loop_energy_grad
+
+    specs = [InstanceSpec(10, 5, 5, L, seed) for L, seed in ((1, 21), (2, 22), (3, 23), (2, 24))]
+    specs.append(InstanceSpec(6, 3, 3, 1, 25))
+    inst, thetas = build_instances(specs)
+    b = E.Batch(inst)
+    e, g = b.run(thetas, graph=True)
+    e2, g2 = b.run(thetas, graph=True)
+    _close(e2, g2, e, g, 1e-12)
+    e3, g3 = b.run(thetas, graph=False)
+    _close(e3, g3, e, g, 1e-12)
+    monkeypatch.setenv("SQ_BATCH_GROUPS", "0")
+    b0 = E.Batch(inst)
+    e0, g0 = b0.run(thetas, graph=True)
+    _close(e, g, e0, g0)
+    e_l, g_l = loop_energy_grad(inst, thetas)
+    _close(e, g, e_l, g_l)
+    inst[0][1].set_tile_shape(1024, 256)
+    with pytest.raises(ValueError):
+        b.run(thetas, graph=True)
+    with pytest.raises(ValueError):
+        b.run(thetas, graph=False)
+    e4, g4 = b0.run(thetas, graph=True)  # the all-persistent batch does not use tile plans
+    _close(e4, g4, e0, g0, 1e-12)
+
