@@ -513,18 +513,24 @@ static void build_groups(Batch& B, const sq_hamiltonian* const* hams, const sq_s
 // A grouped program whose tile shape changed after the batch was created has
 // a new tile plan: the group's buffers (and a captured graph) are sized for
 // the old one.
+// The group buffers are sized for the tile plan of each grouped program and
+// the row-sigma block count of each grouped space when the batch was made.
+static void check_group(const BatchGroup& G) {
+  SQ_CHECK(G.prog->plan_gen == G.plan_gen, SQ_ERR_ARG,
+           "a batched program's tile shape changed after the batch was created; create the batch again");
+  SQ_CHECK(row_sigma_group_partials(*G.space) == G.spart_stride, SQ_ERR_ARG,
+           "a batched space's sigma shape changed after the batch was created; create the batch again");
+}
+
 static void check_groups(const Batch& B) {
-  for (const BatchGroup& G : B.groups)
-    SQ_CHECK(G.prog->plan_gen == G.plan_gen, SQ_ERR_ARG,
-             "a batched program's tile shape changed after the batch was created; create the batch again");
+  for (const BatchGroup& G : B.groups) check_group(G);
 }
 
 // One group's evaluation on st: thetas and start states in, forward sweep,
 // sigma and energies, reverse sweep with gradients, results out.
 static void group_launch(const Batch& B, const BatchGroup& G, hipStream_t st) {
   const Space& s = *G.space;
-  SQ_CHECK(G.prog->plan_gen == G.plan_gen, SQ_ERR_ARG,
-           "a batched program's tile shape changed after the batch was created; create the batch again");
+  check_group(G);
   Multi M;
   M.n = G.n;
   M.theta = G.ng;

@@ -234,8 +234,9 @@ def test_batch_groups_match_persistent_launch(monkeypatch):
     groups) beside a (6e,6o) instance in the persistent launch: energies and
     gradients equal the all-persistent batch (SQ_BATCH_GROUPS=0) and the
     per-instance loop to 1e-10, across graph replays; after a grouped
-    program's tile shape changes, the batch's launches raise ValueError
-    instead of using buffers sized for the old tile plan."""
+    program's tile shape or a grouped space's sigma shape changes, the batch's
+    launches raise ValueError instead of using buffers sized for the old
+    plan."""
     from slowquant_amd import engine as E
     from slowquant_amd.batch import InstanceSpec, build_instances, loop_energy_grad
 
@@ -261,4 +262,20 @@ def test_batch_groups_match_persistent_launch(monkeypatch):
         b.run(thetas, graph=False)
     e4, g4 = b0.run(thetas, graph=True)  # the all-persistent batch does not use tile plans
     _close(e4, g4, e0, g0, 1e-12)
+    # the same for the grouped space's row-sigma shape (block count of the
+    # grouped sigma launch): refused after a change, accepted once restored
+    monkeypatch.delenv("SQ_BATCH_GROUPS")
+    sp10 = inst[0][0]
+    sp10.configure_sigma(0, 0, -1)  # spaces are shared per process: start from the automatic shape
+    b2 = E.Batch(inst)
+    _close(*b2.run(thetas, graph=True), e0, g0)
+    try:
+        sp10.configure_sigma(4, 1, -1)
+        with pytest.raises(ValueError):
+            b2.run(thetas, graph=True)
+        with pytest.raises(ValueError):
+            b2.run(thetas, graph=False)
+    finally:
+        sp10.configure_sigma(0, 0, -1)
+    _close(*b2.run(thetas, graph=True), e0, g0)
 
