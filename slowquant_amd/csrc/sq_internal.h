@@ -113,7 +113,7 @@ struct Space {
   int32_t* d_bexc_full = nullptr;
   int2* d_gta = nullptr;  // sigma GEMM operand tables [K4 + 16][nsa], [K4 + 16][nsb] (sq_sigma.hip)
   int2* d_gtb = nullptr;
-  // row sigma tables (sq_rowsigma.hip): operand rows [K8 + 8][nsa], [K8 + 16][nsb],
+  // row sigma tables (sq_rowsigma.hip): operand rows [K8 + 16][nsa], [K8 + 32][nsb],
   // beta neighbour lists by partner column with per-tile offsets
   int2* d_rta = nullptr;
   int2* d_rtb = nullptr;

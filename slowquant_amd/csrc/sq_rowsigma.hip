@@ -576,8 +576,13 @@ __global__ __launch_bounds__(kRowThreads) __attribute__((amdgpu_waves_per_eu(MT 
       const int r = f / tpr, c = f - r * tpr;
       const int ib = c * 16 + li;
       double4_t acc[MT];
+      // one m-group: the step count is 4 MT, or 4 MT - 2 where the contraction
+      // ends in the first half of the last 16-row block (10 and 11 orbitals)
       if (K8 == 16 * MT)
         row_tile<MT, 4 * MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
+      else if (K8 == 16 * MT - 8)
+        row_tile<MT, 4 * MT - 2>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li,
+                                 nsteps);
       else
         row_tile<MT>(acc, sA, sPsi + (size_t)r * nsb, sRA + (size_t)r * RA, psi, rtb, nsb, ib, kq, li, nsteps);
       // partners and coefficients of all 4 MT outputs first (rows m >= npair
@@ -634,7 +639,7 @@ struct RowPlan {
   size_t lds;
 };
 
-int row_k8(const Space& s) { return rup(s.npair + 1, 16); }  // contraction rows, 4 k-steps per iteration
+int row_k8(const Space& s) { return rup(s.npair + 1, 8); }  // contraction rows: two 4-deep k-steps per iteration
 
 // Dynamic LDS of one block (bytes): staged form A slice | G tiles | psi rows |
 // alpha operand rows; scatter form A slice | sigma rows | psi rows | alpha

@@ -279,3 +279,19 @@ def test_batch_groups_match_persistent_launch(monkeypatch):
         sp10.configure_sigma(0, 0, -1)
     _close(*b2.run(thetas, graph=True), e0, g0)
 
+
+
+def test_batch_all_grouped():
+    """A batch whose instances are all large enough to be grouped (no
+    persistent launch): (10e,10o) instances in two gate layouts, eagerly and
+    over graph replays, equal the per-instance loop to 1e-10."""
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import InstanceSpec, build_instances, loop_energy_grad
+
+    specs = [InstanceSpec(10, 5, 5, L, seed) for L, seed in ((1, 31), (2, 32), (1, 33))]
+    inst, thetas = build_instances(specs)
+    e_l, g_l = loop_energy_grad(inst, thetas)
+    b = E.Batch(inst, 4)
+    _close(*b.run(thetas), e_l, g_l)
+    for _ in range(2):
+        _close(*b.run(thetas, graph=True), e_l, g_l)
