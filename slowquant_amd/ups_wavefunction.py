@@ -254,9 +254,11 @@ class WaveFunctionUPS(WaveFunctionBase):
     def _calc_energy_rotosolve_optimization(self, parameters, theta_diffs, theta_idx) -> list[float]:
         """Energies with theta[theta_idx] replaced by each of theta_diffs
         (ups_wavefunction.py:1144-1194, which propagates the shifted states as
-        one batch).  All shifted instances run in one persistent batched
-        launch when the layout allows it (rotation gates, at most 12 active
-        orbitals), else one engine energy per shift."""
+        one batch).  All shifted instances run in one batched launch when the
+        layout allows it (rotation gates, at most 12 active orbitals: the
+        persistent kernel for small spaces, the tile-stage and row-sigma
+        kernels over the group of copies from 16384 determinants), else one
+        engine energy per shift."""
         th = np.asarray(parameters, dtype=np.float64)
         thetas = np.repeat(th[None, :], len(theta_diffs), axis=0)
         thetas[:, theta_idx] = np.asarray(theta_diffs, dtype=np.float64)

@@ -246,3 +246,39 @@ def test_rotosolve_optimisation():
     H, ci = _oracle_parts(wf2)
     e_fci = np.linalg.eigvalsh(operator_matrix(H, ci)).min()
     assert abs(wf2.energy_elec - e_fci) < 1e-9
+
+
+@pytest.mark.parametrize("do_pp", [False, True])
+def test_rotosolve_shifted_energies_10o_grouped(do_pp):
+    """Linear H10 (1.5 Angstrom, STO-3G, RHF), (10e,10o) tUPS with one layer:
+    63504 determinants, so the 2R+1 shifted instances of RotoSolve's batch run
+    as one group through the tile-stage and row-sigma kernels (started from
+    the HF determinant or the perfect-pairing CSF).  The batched energies
+    equal the per-shift engine energies to 1e-10, and the order-R rebuild from
+    them gives the engine energy at the unshifted angle to 1e-9."""
+    from slowquant_amd import engine as E
+    from slowquant_amd import optimizers as O
+    from slowquant_amd.hchain import linear_h_chain
+    from slowquant_amd.ups_wavefunction import WaveFunctionUPS
+
+    sq = linear_h_chain(10, 1.5)
+    opts = {"n_layers": 1, "do_pp": True} if do_pp else {"n_layers": 1}
+    wf = WaveFunctionUPS((10, 10), sq.hartree_fock.mo_coeff, sq, "tUPS", opts)
+    assert wf._pp == do_pp
+    assert wf._space().dim == 63504 >= E.BATCH_GROUP_MIN_DIM
+    theta = np.random.default_rng(12).normal(0.0, 0.1, wf.ups_layout.n_params)
+    Rs = [wf.ups_layout.grad_param_R[n] for n in wf.ups_layout.param_names]
+    for idx in sorted({Rs.index(4), Rs.index(2), len(theta) - 1}):
+        R = Rs[idx]
+        diffs = O.shift_points(R)
+        assert wf._shifted_batch(len(diffs)) is not None
+        e = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, idx)
+        batched = wf._shifted_batch
+        wf._shifted_batch = lambda n: None  # the per-shift engine path
+        try:
+            e_loop = wf._calc_energy_rotosolve_optimization(theta.tolist(), diffs, idx)
+            e0 = wf._calc_energy_rotosolve_optimization(theta.tolist(), [theta[idx]], idx)[0]
+        finally:
+            wf._shifted_batch = batched
+        assert np.abs(np.array(e) - np.array(e_loop)).max() < 1e-10, idx
+        assert abs(O.reconstructed_f([theta[idx]], e, R)[0] - e0) < 1e-9, idx
