@@ -1,7 +1,7 @@
 /*
  * CPU restatement of the reference's Numba determinant-scan kernel
  * apply_operator_threaded (operator_state_algebra.py:139-219) -- TEST AND
- * BASELINE INFRASTRUCTURE ONLY (oracle).
+ * CPU-BASELINE INFRASTRUCTURE ONLY (oracle).
  *
  * For one normal-ordered operator string: every target determinant that has
  * all creator bits set and no screened-annihilator bit set is mapped to its
