@@ -213,8 +213,9 @@ int sq_ucc_energy_grad(const sq_space* s, const sq_program* p, const sq_hamilton
  * gradient kernels with one grid row per instance (SQ_BATCH_GROUPS=0 keeps
  * them in the persistent launch).  Such a batch uses the programs' tile
  * plans at every launch: the programs must outlive it, and changing a
- * grouped program's tile shape afterwards makes its launches fail with
- * SQ_ERR_ARG (create the batch again). */
+ * grouped program's tile shape (sq_program_set_tile_shape) or a grouped
+ * space's row-sigma shape (sq_sigma_configure) afterwards makes its launches
+ * fail with SQ_ERR_ARG (create the batch again). */
 typedef struct sq_batch sq_batch;
 int sq_batch_create(int n_inst, const sq_space* const* spaces, const sq_program* const* programs,
                     const sq_hamiltonian* const* hams, int n_workgroups, sq_batch** out);
