@@ -295,3 +295,35 @@ def test_batch_all_grouped():
     _close(*b.run(thetas), e_l, g_l)
     for _ in range(2):
         _close(*b.run(thetas, graph=True), e_l, g_l)
+
+
+def test_batch_group_members_own_start_states():
+    """Three (10e,10o) instances with one gate layout form one group; each
+    member starts from its own random unit state (and one from HF), eagerly
+    and over graph replays, equal to Program.energy_grad from the same state
+    to 1e-10."""
+    import torch
+
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import InstanceSpec, build_instances
+
+    specs = [InstanceSpec(10, 5, 5, 1, seed) for seed in (41, 42, 43)]
+    inst, thetas = build_instances(specs)
+    b = E.Batch(inst, 1)
+    gen = torch.Generator().manual_seed(44)
+    starts = []
+    for k, (sp, _, _) in enumerate(inst):
+        if k == 1:
+            starts.append(None)
+            continue
+        x = torch.randn(sp.dim, generator=gen, dtype=torch.float64)
+        starts.append((x / x.norm()).cuda())
+    b.set_start_states(starts)
+    e_ref, g_ref = [], []
+    for (sp, prog, H), th, x in zip(inst, thetas, starts):
+        ei, gi = prog.energy_grad(H, th, E.hf_state(sp) if x is None else x)
+        e_ref.append(ei)
+        g_ref.append(gi)
+    _close(*b.run(thetas), e_ref, g_ref)
+    for _ in range(2):
+        _close(*b.run(thetas, graph=True), e_ref, g_ref)
