@@ -71,8 +71,7 @@ class WaveFunctionUCC(WaveFunctionBase):
 
     def _psi(self):
         if self._psi_dev is None:
-            sp = self._space()
-            self._psi_dev = self._expm(np.asarray(self._thetas, dtype=np.float64), sp.from_reference(self.csf_coeffs))
+            self._psi_dev = self._expm(np.asarray(self._thetas, dtype=np.float64), self._csf_device())
         return self._psi_dev
 
     @property
@@ -125,16 +124,15 @@ class WaveFunctionUCC(WaveFunctionBase):
         from . import engine as E
 
         th = self._thetas if thetas is None else thetas
-        return E.ucc_energy_grad(self._prog(), self._hamiltonian(), th, self._space().from_reference(self.csf_coeffs))
+        return E.ucc_energy_grad(self._prog(), self._hamiltonian(), th, self._csf_device())
 
     def _finite_difference_gradient(self) -> np.ndarray:
         """The reference's forward difference (ucc_wavefunction.py:1069-1097):
         g_i = 2 (<exp(T(theta) + h T_i) csf | H psi> - E) / h, h = sqrt(eps)."""
-        sp = self._space()
         psi = self._psi()
         lam, E = self._hamiltonian().sigma(psi)
         eps = np.finfo(np.float64).eps ** 0.5
-        csf = sp.from_reference(self.csf_coeffs)
+        csf = self._csf_device()
         th = np.asarray(self._thetas, dtype=np.float64)
         g = np.zeros(len(th))
         for i in range(len(th)):

@@ -126,7 +126,7 @@ class WaveFunctionUPS(WaveFunctionBase):
         return self._program
 
     def _csf_dev(self):
-        return self._space().from_reference(self.csf_coeffs)
+        return self._csf_device()
 
     def _psi(self):
         """Device state U(theta)|CSF> in the engine convention."""
@@ -158,6 +158,10 @@ class WaveFunctionUPS(WaveFunctionBase):
 
     @thetas.setter
     def thetas(self, theta_vals: list[float]) -> None:
+        """ups_wavefunction.py:345-365, which rebuilds the state at once; here
+        the state U(theta)|CSF> is prepared on first use (the adjoint gradient
+        prepares its own, so an optimiser's gradient call costs no extra
+        state preparation)."""
         if len(theta_vals) != len(self._thetas):
             raise ValueError(f"Expected {len(self._thetas)} theta1 values got {len(theta_vals)}")
         self._rdm1 = None
@@ -167,7 +171,7 @@ class WaveFunctionUPS(WaveFunctionBase):
         self._energy_elec = None
         self._thetas = list(theta_vals).copy()
         self._ci_host = None
-        self._psi_dev = self._prog().state_prep(np.asarray(self._thetas, dtype=np.float64), self._csf_dev())
+        self._psi_dev = None
 
     # ---- properties computed on the device -----------------------------------
     @property
@@ -212,6 +216,12 @@ class WaveFunctionUPS(WaveFunctionBase):
         if kappa_optimization:
             E = get_electronic_energy(self.h_mo, self.g_mo, self.num_inactive_orbs, self.num_active_orbs,
                                       self.rdm1, self.rdm2)
+        elif theta_optimization and self.gradient_method == "adjoint":
+            # the optimisers ask for the gradient at the same point next: one
+            # adjoint evaluation (state, H psi, reverse sweep) gives both
+            H, prog = self._hamiltonian(), self._prog()
+            E, g = prog.energy_grad(H, np.asarray(self._thetas, dtype=np.float64), self._csf_dev(), "adjoint")
+            self._grad_cache = (np.array(parameters, dtype=np.float64), H, prog, g)
         else:
             _, E = self._hamiltonian().sigma(self._psi())
         self._E_opt_old = E
@@ -230,8 +240,14 @@ class WaveFunctionUPS(WaveFunctionBase):
         if kappa_optimization:
             gradient[:num_kappa] = self._orbital_gradient(self.rdm1, self.rdm2)
         if theta_optimization:
-            _, g = self._prog().energy_grad(self._hamiltonian(), np.asarray(self._thetas, dtype=np.float64),
-                                            self._csf_dev(), self.gradient_method)
+            c = getattr(self, "_grad_cache", None)
+            H, prog = self._hamiltonian(), self._prog()
+            if (not kappa_optimization and self.gradient_method == "adjoint" and c is not None and c[1] is H
+                    and c[2] is prog and np.array_equal(c[0], np.asarray(parameters, dtype=np.float64))):
+                g = c[3]  # from the energy call at these parameters
+            else:
+                _, g = prog.energy_grad(H, np.asarray(self._thetas, dtype=np.float64), self._csf_dev(),
+                                        self.gradient_method)
             gradient[num_kappa:] += g
             self.num_energy_evals += 2 * np.sum(list(self.ups_layout.grad_param_R.values()))
         return gradient

@@ -192,6 +192,16 @@ class WaveFunctionBase:
             self._dev_space = engine.space_for(self.num_active_orbs, self.num_active_elec_alpha, self.num_active_elec_beta)
         return self._dev_space
 
+    def _csf_device(self):
+        """csf_coeffs on the device (engine convention), converted once per
+        array: the reference builds csf_coeffs in its constructor and never
+        changes it; assigning a new array converts again."""
+        c = self.__dict__.get("_csf_dev_cached")
+        if c is None or c[0] is not self.csf_coeffs:
+            c = (self.csf_coeffs, self._space().from_reference(self.csf_coeffs))
+            self.__dict__["_csf_dev_cached"] = c
+        return c[1]
+
     def _hamiltonian(self):
         """Folded active-space Hamiltonian on the device, rebuilt when h_mo/g_mo change."""
         from . import engine

@@ -282,3 +282,54 @@ def test_rotosolve_shifted_energies_10o_grouped(do_pp):
             wf._shifted_batch = batched
         assert np.abs(np.array(e) - np.array(e_loop)).max() < 1e-10, idx
         assert abs(O.reconstructed_f([theta[idx]], e, R)[0] - e0) < 1e-9, idx
+
+
+def test_optimizer_calls_pair_energy_and_gradient():
+    """The optimiser entry points (ups_wavefunction.py:1019-1142): an energy
+    call computes the adjoint gradient with the energy, and the gradient call
+    at the same parameters returns it; a gradient at other parameters, the
+    other gradient methods and a changed kappa are computed afresh; ci_coeffs
+    follow the last thetas set (the state is prepared on first use).  All
+    against fresh engine calls and the oracle to 1e-10."""
+    from oracle.ups import construct_ups_state
+
+    _sq, wf = _h4_wf(layers=2)
+    n = wf.ups_layout.n_params
+    rng = np.random.default_rng(21)
+    p1, p2 = rng.normal(0.0, 0.2, n), rng.normal(0.0, 0.2, n)
+    wf._old_opt_parameters = np.full(n, 1e20)
+    wf._E_opt_old = 0.0
+
+    def fresh(p):
+        return wf.energy_and_gradient(list(p))
+
+    e1 = wf._calc_energy_optimization(p1, True, False)
+    g1 = wf._calc_gradient_optimization(p1, True, False)
+    ef, gf = fresh(p1)
+    assert abs(e1 - ef) < 1e-12 and np.abs(g1 - gf).max() < 1e-12
+    g2 = wf._calc_gradient_optimization(p2, True, False)  # no energy call at p2
+    assert np.abs(g2 - fresh(p2)[1]).max() < 1e-12
+    _, ci = _oracle_parts(wf)
+    assert np.abs(wf.ci_coeffs - construct_ups_state(wf.csf_coeffs, ci, p2, wf.ups_layout)).max() < 1e-12
+    e3 = wf._calc_energy_optimization(p1, True, False)
+    wf.gradient_method = "reference"
+    try:
+        g3 = wf._calc_gradient_optimization(p1, True, False)
+    finally:
+        wf.gradient_method = "adjoint"
+    assert abs(e3 - ef) < 1e-12 and np.abs(g3 - gf).max() < 1e-10
+    # a kappa step changes the Hamiltonian: the gradient is not taken from the
+    # cache (H4 with a (2e,2o) active space, so kappa is not empty)
+    from slowquant_amd.ups_wavefunction import WaveFunctionUPS
+
+    wk = WaveFunctionUPS((2, 2), _sq.hartree_fock.mo_coeff, _sq, "tUPS", {"n_layers": 1})
+    assert len(wk.kappa) > 0
+    pk = rng.normal(0.0, 0.2, wk.ups_layout.n_params)
+    wk._old_opt_parameters = np.full(pk.size, 1e20)
+    wk._E_opt_old = 0.0
+    e4 = wk._calc_energy_optimization(pk, True, False)
+    wk.kappa = list(rng.normal(0.0, 0.05, len(wk.kappa)))
+    g4 = wk._calc_gradient_optimization(pk, True, False)
+    e5, g5 = wk.energy_and_gradient(list(pk))
+    assert np.abs(g4 - g5).max() < 1e-12
+    assert abs(e4 - e5) > 1e-8  # the orbitals did move
