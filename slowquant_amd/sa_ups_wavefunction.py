@@ -155,7 +155,10 @@ class WaveFunctionSAUPS(WaveFunctionBase):
 
     @kappa.setter
     def kappa(self, k: list[float]) -> None:
-        """Moves the expansion point (sa_ups_wavefunction.py:296-311)."""
+        """Moves the expansion point (sa_ups_wavefunction.py:296-311); the
+        kappa already set, with no expansion point pending, changes nothing."""
+        if list(k) == self._kappa and self._kappa_old == self._kappa:
+            return
         self._h_mo = None
         self._g_mo = None
         self.__dict__["_mo_dev_cache"] = {}

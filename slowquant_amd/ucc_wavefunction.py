@@ -86,8 +86,12 @@ class WaveFunctionUCC(WaveFunctionBase):
 
     @thetas.setter
     def thetas(self, theta: list[float]) -> None:
+        """ucc_wavefunction.py:278-293; the thetas already set change nothing
+        (the state and the quantities built from it stay valid)."""
         if len(theta) != len(self._thetas):
             raise ValueError(f"Expected {len(self._thetas)} theta1 values got {len(theta)}")
+        if list(theta) == self._thetas:
+            return
         self._rdm1 = None
         self._rdm2 = None
         self._rdm3 = None

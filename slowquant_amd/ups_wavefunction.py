@@ -147,6 +147,7 @@ class WaveFunctionUPS(WaveFunctionBase):
     @ci_coeffs.setter
     def ci_coeffs(self, value) -> None:
         self._ci_host = np.array(value, dtype=np.float64, copy=True)
+        self._state_from_thetas = False
         self._psi_dev = None
         self._rdm1 = None
         self._rdm2 = None
@@ -161,9 +162,13 @@ class WaveFunctionUPS(WaveFunctionBase):
         """ups_wavefunction.py:345-365, which rebuilds the state at once; here
         the state U(theta)|CSF> is prepared on first use (the adjoint gradient
         prepares its own, so an optimiser's gradient call costs no extra
-        state preparation)."""
+        state preparation).  The thetas already set, with the state still the
+        one they give (no ci_coeffs assigned since), change nothing."""
         if len(theta_vals) != len(self._thetas):
             raise ValueError(f"Expected {len(self._thetas)} theta1 values got {len(theta_vals)}")
+        if self.__dict__.get("_state_from_thetas", True) and list(theta_vals) == self._thetas:
+            return
+        self._state_from_thetas = True
         self._rdm1 = None
         self._rdm2 = None
         self._rdm3 = None

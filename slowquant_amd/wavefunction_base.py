@@ -121,6 +121,13 @@ class WaveFunctionBase:
 
     @kappa.setter
     def kappa(self, k: list[float]) -> None:
+        """Moves the expansion point (ups_wavefunction.py:321-334).  Setting
+        the kappa already set, with no expansion point pending, changes
+        nothing, so the integrals and everything built from them are kept
+        (the optimisers set the same parameters for the energy and then the
+        gradient)."""
+        if list(k) == self._kappa and self._kappa_old == self._kappa:
+            return
         self._h_mo = None
         self._g_mo = None
         self.__dict__["_mo_dev_cache"] = {}
