@@ -134,6 +134,58 @@ __global__ void k_axpby_eye(int n, double a, const double* __restrict__ x, doubl
   out[i] = a * x[i] + ((i / n == i % n) ? b : 0.0);
 }
 
+// exp(X) and c_new = c_old exp(X) in one block for n <= kSmallExpm (X, E and
+// T in LDS, 3 n^2 doubles): the Horner recurrence E = I + X/14,
+// E = I + X E / j (j = 13 .. 1) and s squarings of the GEMM path, with the
+// block's threads taking the entries of each product (a few microseconds for
+// the orbital counts of small molecules, where the GEMM path is ~30 launches).
+constexpr int kSmallExpm = 48;
+
+__global__ __launch_bounds__(256) void k_expm_small(int n, int n_ao, int s, const double* __restrict__ X,
+                                                    const double* __restrict__ c_old, double* __restrict__ c_new) {
+  extern __shared__ double sm[];
+  const int nn = n * n;
+  double* sX = sm;
+  double* sE = sm + nn;
+  double* sT = sm + 2 * nn;
+  for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+    const double x = X[i];
+    sX[i] = x;
+    sE[i] = (i / n == i % n ? 1.0 : 0.0) + x / 14.0;
+  }
+  __syncthreads();
+  for (int j = 13; j >= 1; --j) {
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+      const int r = i / n, c = i - r * n;
+      double acc = 0.0;
+      for (int l = 0; l < n; ++l) acc += sX[r * n + l] * sE[l * n + c];
+      sT[i] = (r == c ? 1.0 : 0.0) + acc / j;
+    }
+    __syncthreads();  // every read of sE is done before it becomes the next output
+    double* t = sE;
+    sE = sT;
+    sT = t;
+  }
+  for (int q = 0; q < s; ++q) {
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+      const int r = i / n, c = i - r * n;
+      double acc = 0.0;
+      for (int l = 0; l < n; ++l) acc += sE[r * n + l] * sE[l * n + c];
+      sT[i] = acc;
+    }
+    __syncthreads();
+    double* t = sE;
+    sE = sT;
+    sT = t;
+  }
+  for (int i = threadIdx.x; i < n_ao * n; i += blockDim.x) {
+    const int a = i / n, c = i - a * n;
+    double acc = 0.0;
+    for (int l = 0; l < n; ++l) acc += c_old[(size_t)a * n + l] * sE[l * n + c];
+    c_new[i] = acc;
+  }
+}
+
 // Full-space Gamma blocks over the o = ni + na inactive + active orbitals
 // (density_matrix.py:5-136): G1[p][q] (N x o, rows >= o zero) and the
 // compact G2c[p][q][r][s] (o^4).
@@ -267,6 +319,12 @@ int sq_orbital_rotation(int n, int n_ao, const double* c_old, const double* kapp
     double* E = work + nn;
     double* T = work + 2 * nn;
     SQ_HIP(hipMemcpyAsync(dX, X.data(), sizeof(double) * nn, hipMemcpyHostToDevice, st));
+    if (n <= kSmallExpm) {
+      k_expm_small<<<1, 256, sizeof(double) * 3 * nn, st>>>(n, n_ao, s, dX, c_old, c_new);
+      SQ_HIP(hipGetLastError());
+      SQ_HIP(hipStreamSynchronize(st));  // X lives in host memory owned by this call
+      return;
+    }
     const unsigned nb = (unsigned)((nn + 255) / 256);
     // E = I + X/14; E = I + X E / j for j = 13 .. 1
     k_axpby_eye<<<nb, 256, 0, st>>>(n, 1.0 / 14.0, dX, 1.0, E);

@@ -4,7 +4,8 @@ host restatements:
 
 * c_mo = C expm(-kappa) (ups_wavefunction.py:367-385) against
   scipy.linalg.expm, for small and large rotations (the latter exercise the
-  squarings) and a rectangular C (more AOs than MOs);
+  squarings) and a rectangular C (more AOs than MOs), on both sides of the
+  one-block limit (48 orbitals);
 * h_mo, g_mo (integralfunctions.py:186-211) against numpy.einsum, with
   n_ao = n_mo and n_ao > n_mo;
 * get_orbital_gradient (density_matrix.py:181-230) against the oracle's loop
@@ -29,7 +30,9 @@ def _sym_g(n, rng):
     return np.einsum("Lpq,Lrs->pqrs", B, B)
 
 
-@pytest.mark.parametrize("n,n_ao,scale", [(6, 6, 0.05), (10, 10, 3.0), (7, 9, 0.4), (33, 33, 1.0)])
+# n <= 48 runs in one block (k_expm_small), larger n through GEMM launches
+@pytest.mark.parametrize("n,n_ao,scale", [(6, 6, 0.05), (10, 10, 3.0), (7, 9, 0.4), (33, 33, 1.0), (48, 50, 1.0),
+                                          (49, 49, 0.5), (64, 70, 2.0)])
 def test_orbital_rotation_against_expm(n, n_ao, scale):
     from slowquant_amd import orbitals
 
