@@ -186,30 +186,16 @@ __global__ __launch_bounds__(256) void k_expm_small(int n, int n_ao, int s, cons
   }
 }
 
-// Full-space Gamma blocks over the o = ni + na inactive + active orbitals
-// (density_matrix.py:5-136): G1[p][q] (N x o, rows >= o zero) and the
-// compact G2c[p][q][r][s] (o^4).
-__global__ void k_full_rdm1(int N, int ni, int na, const double* __restrict__ rdm1, double* __restrict__ G1) {
-  const int o = ni + na;
-  const int x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= N * o) return;
-  const int p = x / o, q = x % o;
-  double v = 0.0;
-  if (p < o) {
-    if (p >= ni && q >= ni) v = rdm1[(p - ni) * na + (q - ni)];
-    else if (p < ni && q < ni) v = p == q ? 2.0 : 0.0;
-  }
-  G1[x] = v;
+// Full-space Gamma elements over the o = ni + na inactive + active orbitals
+// (density_matrix.py:5-136), from the active rdm1 (na x na) and rdm2 (na^4).
+__device__ __forceinline__ double full_rdm1_elem(int p, int q, int ni, int na, const double* __restrict__ rdm1) {
+  if (p >= ni && q >= ni) return rdm1[(p - ni) * na + (q - ni)];
+  if (p < ni && q < ni) return p == q ? 2.0 : 0.0;
+  return 0.0;
 }
 
-__global__ void k_full_rdm2(int ni, int na, const double* __restrict__ rdm1, const double* __restrict__ rdm2,
-                            double* __restrict__ G2) {
-  const int o = ni + na;
-  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-  const int64_t o4 = (int64_t)o * o * o * o;
-  if (x >= o4) return;
-  const int s = (int)(x % o), r = (int)((x / o) % o), q = (int)((x / ((int64_t)o * o)) % o),
-            p = (int)(x / ((int64_t)o * o * o));
+__device__ __forceinline__ double full_rdm2_elem(int p, int q, int r, int s, int ni, int na,
+                                                 const double* __restrict__ rdm1, const double* __restrict__ rdm2) {
   const bool ap = p >= ni, aq = q >= ni, ar = r >= ni, as = s >= ni;
   auto d1 = [&](int a, int b) { return rdm1[(a - ni) * na + (b - ni)]; };
   double v = 0.0;
@@ -227,7 +213,72 @@ __global__ void k_full_rdm2(int ni, int na, const double* __restrict__ rdm1, con
     if (p == q && r == s) v += 4.0;
     if (q == r && p == s) v -= 2.0;
   }
-  G2[x] = v;
+  return v;
+}
+
+// G1[p][q] (N x o, rows >= o zero) and the compact G2c[p][q][r][s] (o^4).
+__global__ void k_full_rdm1(int N, int ni, int na, const double* __restrict__ rdm1, double* __restrict__ G1) {
+  const int o = ni + na;
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= N * o) return;
+  const int p = x / o, q = x % o;
+  G1[x] = p < o ? full_rdm1_elem(p, q, ni, na, rdm1) : 0.0;
+}
+
+__global__ void k_full_rdm2(int ni, int na, const double* __restrict__ rdm1, const double* __restrict__ rdm2,
+                            double* __restrict__ G2) {
+  const int o = ni + na;
+  const int64_t x = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  const int64_t o4 = (int64_t)o * o * o * o;
+  if (x >= o4) return;
+  const int s = (int)(x % o), r = (int)((x / o) % o), q = (int)((x / ((int64_t)o * o)) % o),
+            p = (int)(x / ((int64_t)o * o * o));
+  G2[x] = full_rdm2_elem(p, q, r, s, ni, na, rdm1, rdm2);
+}
+
+// The whole orbital gradient in one block for o <= kSmallGradO: Gamma1 and
+// Gamma2 over the o orbitals in LDS; one wave per kappa element (m, n), its
+// lanes splitting the o^3 index triples of the four sums Z(n,m), W(m,n),
+// Z(m,n), W(n,m) of k_kappa_grad (and the o terms of Y and V), summed by a
+// fixed-order shuffle reduction; g is read from global memory.
+constexpr int kSmallGradO = 8;
+
+__global__ __launch_bounds__(256) void k_kappa_grad_small(int N, int ni, int na, const double* __restrict__ h,
+                                                          const double* __restrict__ g,
+                                                          const double* __restrict__ rdm1,
+                                                          const double* __restrict__ rdm2, int nk,
+                                                          const int32_t* __restrict__ idx, double* __restrict__ grad) {
+  __shared__ double sG1[kSmallGradO * kSmallGradO];
+  __shared__ double sG2[kSmallGradO * kSmallGradO * kSmallGradO * kSmallGradO];
+  const int o = ni + na, o2 = o * o, o3 = o2 * o, o4 = o3 * o;
+  for (int x = threadIdx.x; x < o2; x += blockDim.x) sG1[x] = full_rdm1_elem(x / o, x % o, ni, na, rdm1);
+  for (int x = threadIdx.x; x < o4; x += blockDim.x)
+    sG2[x] = full_rdm2_elem(x / o3, (x / o2) % o, (x / o) % o, x % o, ni, na, rdm1, rdm2);
+  __syncthreads();
+  const int64_t N2 = (int64_t)N * N, N3 = N2 * N;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = (int)(blockDim.x >> 6);
+  for (int i = wave; i < nk; i += nw) {
+    const int m = idx[2 * i], n = idx[2 * i + 1];
+    double t = 0.0;
+    // grad = 2 Y[n][m] - 2 V[m][n] + Z(n,m) - W(m,n) - Z(m,n) + W(n,m), with
+    // Z(a,b) = sum_pqr g[a][p][q][r] G2[b][p][q][r], W(a,b) = sum_pqr g[p][a][q][r] G2[p][b][q][r]
+    // (terms with an index b >= o vanish with Gamma)
+    for (int x = lane; x < o3; x += 64) {
+      const int p = x / o2, q = (x / o) % o, r = x % o;
+      const int64_t qr = (int64_t)q * N + r;
+      const int pqr = p * o2 + q * o + r;  // = x
+      if (m < o)
+        t += g[n * N3 + p * N2 + qr] * sG2[m * o3 + pqr] + g[p * N3 + n * N2 + qr] * sG2[p * o3 + m * o2 + q * o + r];
+      if (n < o)
+        t -= g[m * N3 + p * N2 + qr] * sG2[n * o3 + pqr] + g[p * N3 + m * N2 + qr] * sG2[p * o3 + n * o2 + q * o + r];
+    }
+    if (lane < o) {
+      if (m < o) t += 2.0 * h[(int64_t)n * N + lane] * sG1[m * o + lane];
+      if (n < o) t -= 2.0 * h[(int64_t)lane * N + m] * sG1[lane * o + n];
+    }
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_down(t, off, 64);
+    if (lane == 0) grad[i] = t;
+  }
 }
 
 // gc[a][pqr] = g[a][p][q][r] and gt[a][pqr] = g[p][a][q][r] over p, q, r < o.
@@ -404,6 +455,13 @@ int sq_orbital_gradient(int n_orb, int n_inactive, int n_active, const double* h
     double* dgrad = W + (int64_t)N * o;
     int32_t* didx = reinterpret_cast<int32_t*>(dgrad + n_kappa);
     SQ_HIP(hipMemcpyAsync(didx, kappa_idx, sizeof(int32_t) * 2 * n_kappa, hipMemcpyHostToDevice, st));
+    if (o <= kSmallGradO) {
+      k_kappa_grad_small<<<1, 256, 0, st>>>(N, ni, na, h_mo, g_mo, rdm1, rdm2, n_kappa, didx, dgrad);
+      SQ_HIP(hipGetLastError());
+      SQ_HIP(hipMemcpyAsync(grad, dgrad, sizeof(double) * n_kappa, hipMemcpyDeviceToHost, st));
+      SQ_HIP(hipStreamSynchronize(st));
+      return;
+    }
     k_full_rdm1<<<(unsigned)((N * o + 255) / 256), 256, 0, st>>>(N, ni, na, rdm1, G1);
     SQ_HIP(hipGetLastError());
     k_full_rdm2<<<(unsigned)((o * o3 + 255) / 256), 256, 0, st>>>(ni, na, rdm1, rdm2, G2);

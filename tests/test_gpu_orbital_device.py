@@ -65,7 +65,8 @@ def test_ao2mo_against_einsum(n_ao, n_mo):
     assert g2 is None and np.abs(h2.cpu().numpy() - eh).max() < 1e-12 * np.abs(eh).max()
 
 
-@pytest.mark.parametrize("ni,na,nv", [(0, 4, 0), (2, 2, 2), (1, 4, 1), (3, 5, 4)])
+# inactive + active <= 8 runs in one block (k_kappa_grad_small), more through GEMM launches
+@pytest.mark.parametrize("ni,na,nv", [(0, 4, 0), (2, 2, 2), (1, 4, 1), (3, 5, 4), (2, 7, 3), (4, 6, 6)])
 def test_orbital_gradient_against_oracle(ni, na, nv):
     from oracle.orbital import get_orbital_gradient as oracle_gradient
     from slowquant_amd import orbitals
