@@ -159,6 +159,12 @@ class WaveFunctionUCC(WaveFunctionBase):
         if kappa_optimization:
             E = get_electronic_energy(self.h_mo, self.g_mo, self.num_inactive_orbs, self.num_active_orbs,
                                       self.rdm1, self.rdm2)
+        elif theta_optimization and self.gradient_method == "analytic":
+            # the optimisers ask for the gradient at the same point next: one
+            # engine call gives the energy and the analytic gradient
+            H, prog = self._hamiltonian(), self._prog()
+            E, g = self.energy_and_gradient()
+            self._grad_cache = (np.array(parameters, dtype=np.float64), H, prog, g)
         else:
             E = self.energy_elec
         self._E_opt_old = E
@@ -176,8 +182,12 @@ class WaveFunctionUCC(WaveFunctionBase):
         if kappa_optimization:
             gradient[:num_kappa] = self._orbital_gradient(self.rdm1, self.rdm2)
         if theta_optimization:
+            c = getattr(self, "_grad_cache", None)
             if self.gradient_method == "finite_difference":
                 gradient[num_kappa:] = self._finite_difference_gradient()
+            elif (not kappa_optimization and c is not None and c[1] is self._hamiltonian() and c[2] is self._prog()
+                  and np.array_equal(c[0], np.asarray(parameters, dtype=np.float64))):
+                gradient[num_kappa:] = c[3]  # from the energy call at these parameters
             else:
                 gradient[num_kappa:] = self.energy_and_gradient()[1]
         return gradient

@@ -280,3 +280,39 @@ def test_operator_matrix_against_oracle(n, na, nb):
             assert np.abs(M + M.T).max() < 1e-13
         if kind == "sym":
             assert np.abs(M - M.T).max() < 1e-12 * np.abs(M).max()
+
+
+def test_ucc_optimizer_calls_pair_energy_and_gradient():
+    """WaveFunctionUCC's optimiser entry points (ucc_wavefunction.py:723-965):
+    the energy call computes the analytic gradient with the energy and the
+    gradient call at the same parameters returns it; other parameters and the
+    finite-difference method are computed afresh.  Against fresh engine calls
+    (1e-12) and the central difference of the engine energy (1e-7)."""
+    from slowquant_amd.ucc_wavefunction import WaveFunctionUCC
+
+    sq = _h4_rect()
+    wf = WaveFunctionUCC((4, 4), sq.hartree_fock.mo_coeff, sq, "SD")
+    n = wf.ucc_layout.n_params
+    rng = np.random.default_rng(23)
+    p1, p2 = rng.normal(0.0, 0.1, n), rng.normal(0.0, 0.1, n)
+    wf._old_opt_parameters = np.full(n, 1e20)
+    wf._E_opt_old = 0.0
+    e1 = wf._calc_energy_optimization(p1, True, False)
+    g1 = wf._calc_gradient_optimization(p1, True, False)
+    ef, gf = wf.energy_and_gradient(p1)
+    assert abs(e1 - ef) < 1e-12 and np.abs(g1 - gf).max() < 1e-12
+    g2 = wf._calc_gradient_optimization(p2, True, False)
+    assert np.abs(g2 - wf.energy_and_gradient(p2)[1]).max() < 1e-12
+    wf._calc_energy_optimization(p1, True, False)
+    wf.gradient_method = "finite_difference"
+    try:
+        g3 = wf._calc_gradient_optimization(p1, True, False)
+    finally:
+        wf.gradient_method = "analytic"
+    assert np.abs(g3 - gf).max() < 1e-6  # the reference's forward difference, h = sqrt(eps)
+    h = 1e-5
+    for i in (0, n - 1):
+        d = np.zeros(n)
+        d[i] = h
+        fd = (wf.energy_and_gradient(p1 + d)[0] - wf.energy_and_gradient(p1 - d)[0]) / (2 * h)
+        assert abs(fd - g1[i]) < 1e-7
