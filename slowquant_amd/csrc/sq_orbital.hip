@@ -186,6 +186,73 @@ __global__ __launch_bounds__(256) void k_expm_small(int n, int n_ao, int s, cons
   }
 }
 
+// h_mo = C^T h C and the four quarter transforms of g in one block for
+// n_ao, n_mo <= kSmallAo (C is n_ao x n_mo row-major): the intermediates
+// T1 (m a^3), T2 (m^2 a^2), T1 (m^3 a) live in two LDS buffers of kSmallAo^4
+// doubles, with a barrier between passes; the same contractions as the GEMM
+// path of sq_ao2mo, for the basis sizes of small molecules.
+constexpr int kSmallAo = 8;
+
+__global__ __launch_bounds__(256) void k_ao2mo_small(int a, int m, const double* __restrict__ c,
+                                                     const double* __restrict__ h_ao, const double* __restrict__ g_ao,
+                                                     double* __restrict__ h_mo, double* __restrict__ g_mo) {
+  extern __shared__ double sm[];
+  constexpr int kBuf = kSmallAo * kSmallAo * kSmallAo * kSmallAo;
+  double* A = sm;
+  double* B = sm + kBuf;
+  const int a2 = a * a, a3 = a2 * a, m2 = m * m, m3 = m2 * m;
+  if (h_ao) {
+    // Q[i][b] = sum_x C[x][i] h[x][b];  h_mo[i][j] = sum_b Q[i][b] C[b][j]
+    for (int x = threadIdx.x; x < m * a; x += blockDim.x) {
+      const int i = x / a, b = x % a;
+      double v = 0.0;
+      for (int y = 0; y < a; ++y) v += c[y * m + i] * h_ao[y * a + b];
+      A[x] = v;
+    }
+    __syncthreads();
+    for (int x = threadIdx.x; x < m2; x += blockDim.x) {
+      const int i = x / m, j = x % m;
+      double v = 0.0;
+      for (int b = 0; b < a; ++b) v += A[i * a + b] * c[b * m + j];
+      h_mo[x] = v;
+    }
+    __syncthreads();
+  }
+  if (g_ao) {
+    // A[i][bcd] = sum_x C[x][i] g[x][bcd]
+    for (int x = threadIdx.x; x < m * a3; x += blockDim.x) {
+      const int i = x / a3, r = x % a3;
+      double v = 0.0;
+      for (int y = 0; y < a; ++y) v += c[y * m + i] * g_ao[(int64_t)y * a3 + r];
+      A[x] = v;
+    }
+    __syncthreads();
+    // B[i][j][cd] = sum_b C[b][j] A[i][b][cd]
+    for (int x = threadIdx.x; x < m2 * a2; x += blockDim.x) {
+      const int i = x / (m * a2), j = (x / a2) % m, r = x % a2;
+      double v = 0.0;
+      for (int b = 0; b < a; ++b) v += c[b * m + j] * A[i * a3 + b * a2 + r];
+      B[x] = v;
+    }
+    __syncthreads();
+    // A[i][j][k][d] = sum_x C[x][k] B[i][j][x][d]
+    for (int x = threadIdx.x; x < m3 * a; x += blockDim.x) {
+      const int ij = x / (m * a), k = (x / a) % m, d = x % a;
+      double v = 0.0;
+      for (int y = 0; y < a; ++y) v += c[y * m + k] * B[ij * a2 + y * a + d];
+      A[x] = v;
+    }
+    __syncthreads();
+    // g_mo[ijk][l] = sum_d A[ijk][d] C[d][l]
+    for (int x = threadIdx.x; x < m3 * m; x += blockDim.x) {
+      const int ijk = x / m, l = x % m;
+      double v = 0.0;
+      for (int d = 0; d < a; ++d) v += A[ijk * a + d] * c[d * m + l];
+      g_mo[x] = v;
+    }
+  }
+}
+
 // Full-space Gamma elements over the o = ni + na inactive + active orbitals
 // (density_matrix.py:5-136), from the active rdm1 (na x na) and rdm2 (na^4).
 __device__ __forceinline__ double full_rdm1_elem(int p, int q, int ni, int na, const double* __restrict__ rdm1) {
@@ -403,6 +470,12 @@ int sq_ao2mo(int n_ao, int n_mo, const double* c, const double* h_ao, const doub
     SQ_CHECK((h_ao != nullptr) == (h_mo != nullptr) && (g_ao != nullptr) == (g_mo != nullptr), SQ_ERR_ARG,
              "each transform needs both its input and its output");
     hipStream_t st = as_stream(stream);
+    if (n_ao <= kSmallAo && n_mo <= kSmallAo) {
+      const size_t lds = sizeof(double) * 2 * kSmallAo * kSmallAo * kSmallAo * kSmallAo;
+      k_ao2mo_small<<<1, 256, lds, st>>>(n_ao, n_mo, c, h_ao, g_ao, h_mo, g_mo);
+      SQ_HIP(hipGetLastError());
+      return;
+    }
     const int64_t a = n_ao, m = n_mo;
     const int64_t tmax = std::max({m * a * a * a, m * m * a * a, m * m * m * a});
     double* T1 = work;

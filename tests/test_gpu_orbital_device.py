@@ -46,7 +46,8 @@ def test_orbital_rotation_against_expm(n, n_ao, scale):
     assert np.abs(got - expect).max() < 1e-12 * max(1.0, np.abs(expect).max())
 
 
-@pytest.mark.parametrize("n_ao,n_mo", [(4, 4), (10, 10), (12, 9), (20, 20)])
+# n_ao, n_mo <= 8 run in one block (k_ao2mo_small), larger through GEMM launches
+@pytest.mark.parametrize("n_ao,n_mo", [(4, 4), (8, 8), (8, 6), (9, 9), (10, 10), (12, 9), (20, 20)])
 def test_ao2mo_against_einsum(n_ao, n_mo):
     from slowquant_amd import orbitals
 
