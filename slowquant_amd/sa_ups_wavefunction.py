@@ -240,6 +240,20 @@ class WaveFunctionSAUPS(WaveFunctionBase):
         self._batch_result = (th.tobytes(), H, res)  # H kept: compared by identity
         return res
 
+    def _loop_run(self):
+        """(energies[K], gradients[K]) of the current theta, one adjoint
+        evaluation per state (spaces without the state batch), kept for the
+        optimiser's gradient call at the same theta."""
+        th = np.asarray(self._thetas, dtype=np.float64)
+        H, prog = self._hamiltonian(), self._prog()
+        hit = getattr(self, "_loop_result", None)
+        if hit is not None and hit[0] == th.tobytes() and hit[1] is H and hit[2] is prog:
+            return hit[3]
+        out = [prog.energy_grad(H, th, csf, "adjoint") for csf in self._csfs_dev()]
+        res = (np.array([e for e, _ in out]), [g for _, g in out])
+        self._loop_result = (th.tobytes(), H, prog, res)  # H and prog kept: compared by identity
+        return res
+
     def _state_energy_list(self) -> np.ndarray:
         if self._state_batch() is not None:
             return np.asarray(self._batch_run()[0], dtype=np.float64)
@@ -327,7 +341,11 @@ class WaveFunctionSAUPS(WaveFunctionBase):
             self.kappa = list(parameters[:num_kappa])
         if theta_optimization:
             self.thetas = list(parameters[num_kappa:])
-        energies = self._state_energy_list()
+        if (theta_optimization and not kappa_optimization and self.gradient_method == "adjoint"
+                and self._state_batch() is None):
+            energies = self._loop_run()[0]  # the gradient call at these parameters reuses the evaluations
+        else:
+            energies = self._state_energy_list()
         if return_all_states:
             self._E_opt_old = np.copy(energies)
             self._old_opt_parameters = np.copy(parameters)
@@ -356,6 +374,9 @@ class WaveFunctionSAUPS(WaveFunctionBase):
             g = np.zeros(len(th))
             if self.gradient_method == "adjoint" and self._state_batch() is not None:
                 for gk in self._batch_run()[1]:
+                    g += gk
+            elif self.gradient_method == "adjoint":
+                for gk in self._loop_run()[1]:
                     g += gk
             else:
                 for csf in self._csfs_dev():

@@ -184,3 +184,13 @@ def test_sa_h4_batched_states_equal_state_loop(monkeypatch):
     g_l = wf2._calc_gradient_optimization(theta, True, False)
     assert np.abs(e_b - e_l).max() < 1e-12
     assert np.abs(g_b - g_l).max() < 1e-12
+    # the state-by-state optimiser path: the energy call's per-state adjoint
+    # evaluations serve the gradient call at the same parameters
+    wf2._old_opt_parameters = np.full(theta.size, 1e20)
+    wf2._E_opt_old = 0.0
+    theta2 = theta + rng.normal(0.0, 0.05, theta.size)
+    e_opt = wf2._calc_energy_optimization(theta2, True, False)
+    g_opt = wf2._calc_gradient_optimization(theta2, True, False)
+    wf.thetas = theta2.tolist()
+    assert abs(e_opt - float(np.mean(wf._state_energy_list()))) < 1e-12
+    assert np.abs(g_opt - wf._calc_gradient_optimization(theta2, True, False)).max() < 1e-12
