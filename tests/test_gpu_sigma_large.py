@@ -5,7 +5,8 @@ gradient of the headline benchmark; here it is compared, on random vectors,
 with the oracle's restatement of the reference's determinant scan
 (oracle/csrc/scan.c, apply_operator_threaded, operator_state_algebra.py:139-219)
 run on all host threads, at (10e,10o), (11e,11o) (6 alpha, 5 beta) and
-(12e,12o) with synthetic rank-2n Hamiltonians.  Tolerance 1e-11 relative to
+(12e,12o), and at 13 and 14 orbitals with 4-6 electrons, with synthetic
+rank-2n Hamiltonians.  Tolerance 1e-11 relative to
 max|sigma| element-wise, energies 1e-10.
 
 The tUPS (12e,12o) 4-layer state is also checked through an independent
@@ -28,7 +29,10 @@ def _threads() -> int:
     return max(1, min(16, os.cpu_count() or 1))
 
 
-@pytest.mark.parametrize("n,na,nb", [(10, 5, 5), (11, 6, 5), (12, 6, 6)])
+# 13 and 14 orbitals with few electrons: the row-sigma kernel shapes of the
+# (13e,13o) and (14e,14o) spaces (they depend on the orbital count only) on
+# spaces the scan finishes in seconds
+@pytest.mark.parametrize("n,na,nb", [(10, 5, 5), (11, 6, 5), (12, 6, 6), (13, 2, 2), (13, 3, 2), (14, 2, 2), (14, 3, 3)])
 def test_sigma_against_oracle_scan(n, na, nb):
     from oracle import cscan
     from oracle.ci import get_indexing
