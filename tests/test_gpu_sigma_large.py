@@ -79,3 +79,38 @@ def test_tups12_energy_from_rdms_equals_sigma_energy():
     assert abs(np.trace(r1) - 12.0) < 1e-10
     assert np.abs(np.einsum("pqrr->pq", r2) - 11.0 * r1).max() < 1e-9
     assert np.abs(r1 - r1.T).max() < 1e-12
+
+
+@pytest.mark.parametrize("n,na,nb", [(13, 2, 2), (14, 2, 2), (14, 3, 2)])
+def test_tups_13_14_orbitals_few_electrons_against_oracle(n, na, nb):
+    """tUPS (one layer) at 13 and 14 orbitals with 4-5 electrons: the brick
+    stages over 13-14 orbitals, the sigma and the reverse sweep against the
+    oracle: the state (1e-12), the energy (1e-10) and the adjoint gradient
+    against the oracle's product-rule sweep (ups_wavefunction.py:1090-1141,
+    1e-10)."""
+    from oracle.apply import expectation_value
+    from oracle.ci import get_indexing
+    from oracle.fermion import hamiltonian_active_fast
+    from oracle.ups import construct_ups_state, gradient_reference
+    from slowquant_amd import engine as E
+    from slowquant_amd.synthetic import synthetic_hamiltonian
+    from slowquant_amd.util import UpsStructure
+
+    lay = UpsStructure()
+    lay.create_tiled(n, {"n_layers": 1, "do_tups": True})
+    ci = get_indexing(0, n, 0, na, nb)
+    theta = np.random.default_rng(n + na).normal(0.0, 0.2, lay.n_params)
+    csf = np.zeros(len(ci.idx2det))
+    csf[0] = 1.0
+    h, g = synthetic_hamiltonian(n, 2 * n, np.random.default_rng(n))
+    sp = E.Space(n, na, nb)
+    prog = E.Program(sp, lay.excitation_operator_type, lay.excitation_indices)
+    H = E.Hamiltonian(sp, h, g)
+    psi_ref = construct_ups_state(csf, ci, theta, lay)
+    psi = prog.state_prep(theta, sp.from_reference(csf))
+    assert np.abs(sp.to_reference_numpy(psi) - psi_ref).max() < 1e-12
+    Ho = hamiltonian_active_fast(h, g, n)
+    e_ref = expectation_value(psi_ref, [Ho], psi_ref, ci, do_folding=False)
+    e, grad = prog.energy_grad(H, theta, sp.from_reference(csf))
+    assert abs(e - e_ref) < 1e-10
+    assert np.abs(grad - gradient_reference(Ho, csf, ci, theta, lay)).max() < 1e-10
