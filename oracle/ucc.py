@@ -48,6 +48,26 @@ def construct_ucc_state(state, ci, thetas, layout, dagger=False):
     return ssl.expm_multiply(Tmat, np.asarray(state, dtype=np.float64), traceA=0.0)
 
 
+def construct_ucc_state_matfree(state, ci, thetas, layout, nthreads=1):
+    """construct_ucc_state without the dense matrix: the same generator
+    get_ucc_T applied by the C scan (oracle/csrc/scan.c, the restatement of
+    operator_state_algebra.py:139-219) inside a LinearOperator, exponentiated
+    by scipy.sparse.linalg.expm_multiply (traceA = 0), for CI spaces where the
+    dense T of :413-469 is too slow to build (minutes at (8e,8o))."""
+    from . import cscan
+
+    hs = cscan.HashedSpace(ci)
+    Top = get_ucc_T(thetas, layout)
+    d = len(ci.idx2det)
+
+    def mv(v):
+        return cscan.propagate_state_c([Top], np.ascontiguousarray(v, dtype=np.float64).ravel(), hs,
+                                       nthreads=nthreads)
+
+    A = ssl.LinearOperator((d, d), matvec=mv, rmatvec=lambda v: -mv(v), dtype=np.float64)
+    return ssl.expm_multiply(A, np.asarray(state, dtype=np.float64), traceA=0.0)
+
+
 def gradient_forward_fd(H_mat, csf, ci, thetas, layout):
     """Forward-difference gradient of ucc_wavefunction.py:1069-1097 (step sqrt(eps))."""
     eps = np.finfo(np.float64).eps

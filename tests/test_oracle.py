@@ -307,6 +307,30 @@ def test_ucc_expm_multiply_matches_expm():
     assert np.abs(construct_ucc_state(csf, ci, th, u) - sl.expm(T) @ csf).max() < 1e-13
 
 
+@pytest.mark.parametrize("n", [4, 6])
+def test_ucc_matrix_free_matches_dense(n):
+    """The matrix-free oracle UCC state (C scan of T inside expm_multiply)
+    equals the dense-matrix form (UCCSD, (4e,4o) and (6e,6o)) and, at (4e,4o),
+    scipy.linalg.expm of the dense T."""
+    from oracle.ucc import construct_ucc_state, construct_ucc_state_matfree, get_ucc_T
+
+    from slowquant_amd.util import UccStructure
+
+    half = n // 2
+    u = UccStructure()
+    u.add_sa_singles(list(range(half)), list(range(half, n)))
+    u.add_sa_doubles(list(range(half)), list(range(half, n)))
+    th = np.random.default_rng(30 + n).normal(0, 0.2, u.n_params)
+    ci = get_indexing(0, n, 0, half, half)
+    csf = np.zeros(len(ci.idx2det))
+    csf[0] = 1.0
+    mf = construct_ucc_state_matfree(csf, ci, th, u)
+    assert np.abs(mf - construct_ucc_state(csf, ci, th, u)).max() < 1e-13
+    if n == 4:
+        T = operator_matrix(get_ucc_T(th, u), ci)
+        assert np.abs(mf - sl.expm(T) @ csf).max() < 1e-13
+
+
 def test_c_scan_matches_numpy_scan():
     from oracle import cscan
     from slowquant_amd.synthetic import synthetic_hamiltonian
