@@ -327,3 +327,20 @@ def test_batch_group_members_own_start_states():
     _close(*b.run(thetas), e_ref, g_ref)
     for _ in range(2):
         _close(*b.run(thetas, graph=True), e_ref, g_ref)
+
+
+def test_batch_with_12_orbital_instances():
+    """Two (12e,12o) tUPS instances (853776 determinants each, the headline
+    space; grouped through the tile-stage and fused row-sigma kernels) beside
+    a (6e,6o) instance: eagerly and over graph replays equal to the
+    per-instance loop to 1e-10."""
+    from slowquant_amd import engine as E
+    from slowquant_amd.batch import InstanceSpec, build_instances, loop_energy_grad
+
+    specs = [InstanceSpec(12, 6, 6, 1, 51), InstanceSpec(12, 6, 6, 1, 52), InstanceSpec(6, 3, 3, 2, 53)]
+    inst, thetas = build_instances(specs)
+    e_l, g_l = loop_energy_grad(inst, thetas)
+    b = E.Batch(inst, 4)
+    _close(*b.run(thetas), e_l, g_l)
+    for _ in range(2):
+        _close(*b.run(thetas, graph=True), e_l, g_l)
